@@ -1,0 +1,240 @@
+/*
+ * gas.h -- C ABI of the MI355X Gather-Apply-Scatter engine (libgas.so).
+ *
+ * Every entry point returns an int status (GAS_OK == 0) and leaves a
+ * human-readable message for the calling thread in gas_last_error().
+ * Pointers are plain host or device pointers; sizes are element counts
+ * unless the name says bytes.  Vertex and edge ids are int32 (the
+ * reference's default Int, gpugas.h:75 / refgas.h:31).
+ *
+ * Groups:
+ *   gasrt_*      thin HIP runtime layer.  Replaces the reference's missing
+ *                cuda_util.h helpers gpuAlloc/gpuFree/copyToGPU/copyToHost/
+ *                CHECK/SYNC_CHECK (used at gpugas.h:177-195, 227-318).
+ *   gas_edges_*  host edge lists: seeded generators and the MatrixMarket
+ *                loader (graphio.cpp:190-262).
+ *   gas_graph_*  a graph resident in HBM as CSC (gather) + CSR (scatter),
+ *                the layout GASEngineGPU::setGraph builds (gpugas.h:212-322).
+ *   gas_engine_* the reference engine API, step by step, for the four
+ *                programs (gpugas.h:212-772, refgas.h:81-224).
+ *   gas_bfs/gas_sssp/gas_cc/gas_pagerank  whole runs, the app drivers'
+ *                run<Engine>() (bfs.cu:91-112, sssp.cu:67-82,
+ *                connected_component.cu:66-81, pagerank.cu:85-101).
+ *   gas_ms_*     64-source batched BFS / SSSP / personalised PageRank.
+ *   gas_server_* continuous-batching query server over resident graphs.
+ */
+#ifndef GAS_H_
+#define GAS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GAS_OK              0
+#define GAS_ERR_ARG         1   /* invalid argument                          */
+#define GAS_ERR_HIP         2   /* HIP runtime error                         */
+#define GAS_ERR_NOMEM       3   /* allocation refused (device or host)       */
+#define GAS_ERR_IO          4   /* file could not be read / parsed           */
+#define GAS_ERR_STATE       5   /* call not valid in the object's state      */
+#define GAS_ERR_FULL        6   /* server admission refused (no slot/memory) */
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char *gas_last_error(void);
+
+/* ---- runtime layer (replaces cuda_util.h, gpugas.h:65) ------------------ */
+typedef void *gasrt_stream_t;
+typedef void *gasrt_event_t;
+typedef void *gasrt_graph_exec_t;
+
+int gasrt_device_count(int *count);
+int gasrt_set_device(int device);
+/* Device properties used by the engine's launch sizing. */
+int gasrt_device_info(int device, int *cu_count, int *lds_bytes_per_block,
+                      size_t *hbm_bytes, char *name, size_t name_len);
+int gasrt_malloc(void **ptr, size_t bytes);                 /* gpuAlloc   */
+int gasrt_free(void *ptr);                                  /* gpuFree    */
+int gasrt_host_alloc(void **ptr, size_t bytes);             /* pinned     */
+int gasrt_host_free(void *ptr);
+int gasrt_memcpy_h2d(void *dst, const void *src, size_t bytes,
+                     gasrt_stream_t stream);                /* copyToGPU  */
+int gasrt_memcpy_d2h(void *dst, const void *src, size_t bytes,
+                     gasrt_stream_t stream);                /* copyToHost */
+int gasrt_memcpy_d2d(void *dst, const void *src, size_t bytes,
+                     gasrt_stream_t stream);
+/* Asynchronous device-to-host copy (no synchronisation; dst should be
+ * pinned memory from gasrt_host_alloc). */
+int gasrt_memcpy_d2h_async(void *dst, const void *src, size_t bytes,
+                           gasrt_stream_t stream);
+int gasrt_memset(void *dst, int value, size_t bytes, gasrt_stream_t stream);
+int gasrt_stream_create(gasrt_stream_t *stream);
+int gasrt_stream_destroy(gasrt_stream_t stream);
+int gasrt_stream_synchronize(gasrt_stream_t stream);        /* SYNC_CHECK */
+int gasrt_event_create(gasrt_event_t *event);
+int gasrt_event_destroy(gasrt_event_t event);
+int gasrt_event_record(gasrt_event_t event, gasrt_stream_t stream);
+int gasrt_event_synchronize(gasrt_event_t event);
+int gasrt_event_query(gasrt_event_t event, int *done);
+int gasrt_event_elapsed_ms(gasrt_event_t start, gasrt_event_t stop, float *ms);
+/* Launch a __global__ function by its host stub pointer. */
+int gasrt_launch(const void *func, unsigned grid_x, unsigned block_x,
+                 void **args, size_t shared_bytes, gasrt_stream_t stream);
+int gasrt_capture_begin(gasrt_stream_t stream);
+int gasrt_capture_end(gasrt_stream_t stream, gasrt_graph_exec_t *exec);
+int gasrt_graph_launch(gasrt_graph_exec_t exec, gasrt_stream_t stream);
+int gasrt_graph_destroy(gasrt_graph_exec_t exec);
+int gasrt_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- host edge lists ---------------------------------------------------- */
+typedef struct gas_edges *gas_edges_t;
+
+/* Borrowed view; arrays stay valid until gas_edges_free. */
+typedef struct {
+  int32_t n_vertices;
+  int32_t n_edges;
+  const int32_t *srcs;
+  const int32_t *dsts;
+  const int32_t *weights; /* NULL when the list carries no edge values */
+} gas_edges_view;
+
+int gas_edges_from_arrays(int32_t n_vertices, int32_t n_edges,
+                          const int32_t *srcs, const int32_t *dsts,
+                          const int32_t *weights /* may be NULL */,
+                          gas_edges_t *out);
+int gas_edges_view_get(gas_edges_t e, gas_edges_view *view);
+int gas_edges_free(gas_edges_t e);
+/* rows x cols 4-neighbour grid, id = r*cols + c, both directions, unit
+ * weights. */
+int gas_gen_grid(int32_t rows, int32_t cols, gas_edges_t *out);
+/* Road-like grid: drop floor(drop_frac*L) of the L undirected grid links,
+ * add floor(long_frac*L) random long links (distinct, no self loops, no
+ * duplicates), symmetric weights U[wmin,wmax]. */
+int gas_gen_road(int32_t rows, int32_t cols, double drop_frac,
+                 double long_frac, int32_t wmin, int32_t wmax, uint64_t seed,
+                 gas_edges_t *out);
+/* R-MAT: edge_factor * 2^scale samples with quadrant probabilities a,b,c
+ * (d = 1-a-b-c); self loops and duplicates removed; optionally
+ * symmetrised; weights U[wmin,wmax] shared by both directions of a pair
+ * (wmax == 0: no weights). */
+int gas_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                 double c, uint64_t seed, int symmetrize, int32_t wmin,
+                 int32_t wmax, gas_edges_t *out);
+/* MatrixMarket coordinate loader with the reference's rules
+ * (graphio.cpp:69-262): n_vertices = max id + 1, self loops dropped,
+ * symmetric files expanded to both directions, values read as float and
+ * stored as int.  want_values mirrors passing edgeValues (sssp.cu:112):
+ * with it a self loop does not count toward max id, as in the reference. */
+int gas_load_mtx(const char *path, int want_values, gas_edges_t *out);
+/* The reference's -m option (bfs.cu:148-166): vertex of maximum out-degree,
+ * lowest id on ties. */
+int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,
+                                   int32_t *degree);
+
+/* ---- device-resident graph ---------------------------------------------- */
+typedef struct gas_graph *gas_graph_t;
+
+typedef struct {
+  int32_t n_vertices;
+  int32_t n_edges;
+  const int32_t *csc_offsets; /* device, V+1: in-edges of v at [o[v],o[v+1]) */
+  const int32_t *csc_srcs;    /* device, E                                  */
+  const int32_t *csr_offsets; /* device, V+1: out-edges                      */
+  const int32_t *csr_dsts;    /* device, E                                  */
+  const int32_t *csc_weights; /* device, E, or NULL                         */
+  size_t device_bytes;
+} gas_graph_view;
+
+/* Builds CSC and CSR on the host (stable counting sort by key, the layout
+ * of util.h:79-121) and uploads them. */
+int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
+int gas_graph_view_get(gas_graph_t g, gas_graph_view *view);
+/* Host copies for checking: arrays of V+1 / E int32 (any may be NULL). */
+int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,
+                       int32_t *edge_index);
+int gas_graph_host_csr(gas_graph_t g, int32_t *offsets, int32_t *dsts,
+                       int32_t *edge_index);
+/* Reorder per-edge records of elem_bytes each from edge-list order into
+ * CSC order (host to host). */
+int gas_graph_permute_to_csc(gas_graph_t g, const void *in, void *out,
+                             size_t elem_bytes);
+int gas_graph_free(gas_graph_t g);
+
+/* ---- run statistics ----------------------------------------------------- */
+typedef struct {
+  int32_t supersteps;        /* BSP supersteps executed                    */
+  int32_t reserved;
+  int64_t gathered_edges;    /* sum over supersteps of in-edges gathered   */
+  int64_t scattered_edges;   /* sum over supersteps of out-edges scattered */
+  int64_t active_total;      /* sum over supersteps of |active|            */
+  double  ms;                /* device time of the run (HIP events)        */
+  double  gather_ms;         /* device time in the gather kernel           */
+  double  scatter_ms;        /* device time in the scatter kernel          */
+  int32_t gather_launches;
+  int32_t scatter_launches;
+} gas_run_stats;
+
+/* Set to nonzero to time every gather / scatter launch with HIP events
+ * (adds host synchronisation; used for per-kernel roofline figures). */
+int gas_set_kernel_timing(int enable);
+
+/* ---- whole runs (app drivers) ------------------------------------------- */
+/* BFS (bfs.cu:31-112): depth[v] = superstep at which v is first reached,
+ * -1 if unreached.  stats may be NULL. */
+int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
+            gas_run_stats *stats);
+/* SSSP (sssp.cu:25-82): dist = min over paths of summed int32 weights;
+ * unreached vertices keep gatherZero = 2147383647. */
+int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
+             gas_run_stats *stats);
+/* Connected components by min-label propagation over in-edges
+ * (connected_component.cu:25-81); label[v] initialised to v. */
+int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats);
+/* PageRank (pagerank.cu:27-101): rank = 0.15 + 0.85 * sum rank/outdeg,
+ * FP32 with compensated summation.  iterations > 0: that many sweeps with
+ * every vertex active; iterations == 0: the reference's tolerance-driven
+ * run (tol 0.01).  residual_out (may be NULL) receives, per sweep, the
+ * FP64 sum of |new - old| reduced on the GPU. */
+int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
+                 double *residual_out, gas_run_stats *stats);
+
+/* ---- the engine API, step by step --------------------------------------- */
+#define GAS_PROGRAM_BFS       0  /* VertexData int depth; no EdgeData      */
+#define GAS_PROGRAM_SSSP      1  /* VertexData int;  EdgeData int weight   */
+#define GAS_PROGRAM_PAGERANK  2  /* VertexData {float rank; int outdeg}    */
+#define GAS_PROGRAM_CC        3  /* VertexData int label                   */
+
+typedef struct gas_engine *gas_engine_t;
+
+/* GASEngine*::setGraph (gpugas.h:212, refgas.h:81).  vertex_data is the
+ * caller's array of n_vertices program VertexData records, retained and
+ * written back by gas_engine_get_results; edge_data (n_edges records in
+ * edge-list order, or NULL) is read here only. */
+int gas_engine_create(int program, int32_t n_vertices, void *vertex_data,
+                      int32_t n_edges, const void *edge_data,
+                      const int32_t *srcs, const int32_t *dsts,
+                      gas_engine_t *out);
+/* Same, on a graph already resident in HBM; edge_data may be NULL to use
+ * the graph's own int32 weights. */
+int gas_engine_create_on_graph(int program, gas_graph_t g, void *vertex_data,
+                               const void *edge_data, gas_engine_t *out);
+int gas_engine_set_active(gas_engine_t e, int32_t start, int32_t end);
+int gas_engine_count_active(gas_engine_t e, int32_t *count);
+int gas_engine_gather_apply(gas_engine_t e, int have_gather);
+int gas_engine_scatter_activate(gas_engine_t e, int have_scatter);
+int gas_engine_next_iter(gas_engine_t e, int32_t *count);
+int gas_engine_run(gas_engine_t e);
+int gas_engine_get_results(gas_engine_t e);
+/* Superstep counter the BFS program reads as its level (the reference's
+ * g_iterationCount, bfs.cu:27-28). */
+int gas_engine_set_iteration(gas_engine_t e, int32_t iteration);
+int gas_engine_stats(gas_engine_t e, gas_run_stats *stats);
+int gas_engine_destroy(gas_engine_t e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAS_H_ */

@@ -1,0 +1,193 @@
+"""CPU oracle loader -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  It binds:
+
+  _lib/liboracle.so   the C restatement of the reference CPU engine
+                      (gasref.c; refgas.h:30-225, util.h:48-121) and the
+                      independent checkers;
+  _ref/librefgas.so   the reference's own refgas.h/util.h/util.cpp compiled
+                      from its sources (absent when they were not available
+                      at build time).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "_lib", "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "librefgas.so")
+SSSP_UNREACHED = 2147483647 - 100000
+
+
+def build() -> None:
+    """make -C oracle (liboracle always; _ref when the reference is present)."""
+    res = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+
+
+class Stats(C.Structure):
+    _fields_ = [("supersteps", C.c_int64), ("active_total", C.c_int64),
+                ("gathered", C.c_int64), ("returned", C.c_int64),
+                ("scattered", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_o = None
+_r = None
+
+
+def lib() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_LIB):
+            build()
+        _o = C.CDLL(ORACLE_LIB)
+    return _o
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref() -> C.CDLL:
+    global _r
+    if _r is None:
+        _r = C.CDLL(REF_LIB)
+        for f in ("ref_bfs", "ref_sssp", "ref_cc", "ref_pagerank"):
+            getattr(_r, f).restype = C.c_int64
+    return _r
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t=C.c_int32):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def bfs(nV, srcs, dsts, source):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    st = Stats()
+    assert lib().gasref_bfs(nV, len(s), _p(s), _p(d), source, _p(out), C.byref(st)) == 0
+    return out, st
+
+
+def sssp(nV, srcs, dsts, weights, source):
+    s, d, w = _i32(srcs), _i32(dsts), _i32(weights)
+    out = np.zeros(nV, np.int32)
+    st = Stats()
+    assert lib().gasref_sssp(nV, len(s), _p(s), _p(d), _p(w), source, _p(out),
+                             C.byref(st)) == 0
+    return out, st
+
+
+def cc(nV, srcs, dsts):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    st = Stats()
+    assert lib().gasref_cc(nV, len(s), _p(s), _p(d), _p(out), C.byref(st)) == 0
+    return out, st
+
+
+def pagerank(nV, srcs, dsts, iterations):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.float32)
+    st = Stats()
+    assert lib().gasref_pagerank(nV, len(s), _p(s), _p(d), iterations,
+                                 _p(out, C.c_float), C.byref(st)) == 0
+    return out, st
+
+
+def check_bfs(nV, srcs, dsts, source):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    assert lib().gasref_check_bfs(nV, len(s), _p(s), _p(d), source, _p(out)) == 0
+    return out
+
+
+def check_dijkstra(nV, srcs, dsts, weights, source):
+    s, d, w = _i32(srcs), _i32(dsts), _i32(weights)
+    out = np.zeros(nV, np.int32)
+    assert lib().gasref_check_dijkstra(nV, len(s), _p(s), _p(d), _p(w), source,
+                                       _p(out)) == 0
+    return out
+
+
+def check_min_reaching(nV, srcs, dsts):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    assert lib().gasref_check_min_reaching(nV, len(s), _p(s), _p(d), _p(out)) == 0
+    return out
+
+
+def check_pagerank_fp64(nV, srcs, dsts, iterations):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.float64)
+    assert lib().gasref_check_pagerank_fp64(nV, len(s), _p(s), _p(d), iterations,
+                                            _p(out, C.c_double)) == 0
+    return out
+
+
+def check_ppr_fp64(nV, srcs, dsts, source, iterations):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.float64)
+    assert lib().gasref_check_ppr_fp64(nV, len(s), _p(s), _p(d), source, iterations,
+                                       _p(out, C.c_double)) == 0
+    return out
+
+
+def edge_list_to_csr(nV, srcs, dsts):
+    s, d = _i32(srcs), _i32(dsts)
+    off = np.zeros(nV + 1, np.int32)
+    out = np.zeros(len(s), np.int32)
+    eid = np.zeros(len(s), np.int32)
+    lib().gasref_edge_list_to_csr(nV, len(s), _p(s), _p(d), _p(off), _p(out), _p(eid))
+    return off, out, eid
+
+
+# ---- the reference's own engine (oracle/_ref) ------------------------------
+def ref_bfs(nV, srcs, dsts, source):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    us = ref().ref_bfs(nV, len(s), _p(s), _p(d), source, _p(out))
+    return out, us
+
+
+def ref_sssp(nV, srcs, dsts, weights, source):
+    s, d, w = _i32(srcs), _i32(dsts), _i32(weights)
+    out = np.zeros(nV, np.int32)
+    us = ref().ref_sssp(nV, len(s), _p(s), _p(d), _p(w), source, _p(out))
+    return out, us
+
+
+def ref_cc(nV, srcs, dsts):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.int32)
+    us = ref().ref_cc(nV, len(s), _p(s), _p(d), _p(out))
+    return out, us
+
+
+def ref_pagerank(nV, srcs, dsts, iterations):
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.float32)
+    us = ref().ref_pagerank(nV, len(s), _p(s), _p(d), iterations, _p(out, C.c_float))
+    return out, us
+
+
+def ref_edge_list_to_csr(nV, srcs, dsts):
+    s, d = _i32(srcs), _i32(dsts)
+    off = np.zeros(nV + 1, np.int32)
+    out = np.zeros(len(s), np.int32)
+    eid = np.zeros(len(s), np.int32)
+    ref().ref_edge_list_to_csr(nV, len(s), _p(s), _p(d), _p(off), _p(out), _p(eid))
+    return off, out, eid

@@ -1,0 +1,208 @@
+// ref_driver.cpp -- drives the reference's own CPU engine, GASEngineRef
+// (refgas.h) and CSR/CSC helpers (util.h), compiled from the reference
+// sources where they lie (see oracle/Makefile).  TEST INFRASTRUCTURE ONLY:
+// it validates the C restatement in gasref.c and is the CPU baseline timed
+// by bench.py.  Nothing here is part of libgas.so.
+//
+// The four program structs restate bfs.cu:31-78, sssp.cu:25-64,
+// pagerank.cu:27-73 and connected_component.cu:25-63 on the host path
+// (those .cu files also include gpugas.h, which needs moderngpu, so they are
+// not compiled here).  The run functions restate each driver's run<Engine>()
+// and initialisation.
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "refgas.h"
+#include "util.h"
+
+namespace {
+
+int g_iterationCount;  // bfs.cu:27, host path of bfs.cu:62-66
+
+struct BFS {
+  struct VertexData {
+    int depth;
+  };
+  struct EdgeData {};
+  typedef int GatherResult;
+  static const int gatherZero = INT_MAX - 1;
+  static int gatherReduce(const int &, const int &) { return 0; }
+  static int gatherMap(const VertexData *, const VertexData *,
+                       const EdgeData *) {
+    return 0;
+  }
+  static bool apply(VertexData *vert, int) {
+    if (vert->depth == -1) {
+      vert->depth = g_iterationCount;
+      return true;
+    }
+    return false;
+  }
+  static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+struct SSSP {
+  typedef int VertexData;
+  typedef int EdgeData;
+  typedef int GatherResult;
+  static const int maxLength = 100000;
+  static const int gatherZero = INT_MAX - maxLength;
+  static int gatherReduce(const int &left, const int &right) {
+    return std::min(left, right);
+  }
+  static int gatherMap(const VertexData *, const VertexData *srcDist,
+                       const EdgeData *edgeLen) {
+    return *srcDist + *edgeLen;
+  }
+  static bool apply(VertexData *curDist, GatherResult dist) {
+    bool changed = dist < *curDist;
+    *curDist = std::min(*curDist, dist);
+    return changed;
+  }
+  static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+struct PageRank {
+  static constexpr float pageConst = 0.15f;
+  static constexpr float tol = 0.01f;
+  struct VertexData {
+    float rank;
+    int numOutEdges;
+  };
+  struct EdgeData {};
+  typedef float GatherResult;
+  static constexpr float gatherZero = 0.0f;
+  static float gatherMap(const VertexData *, const VertexData *src,
+                         const EdgeData *) {
+    return src->rank / src->numOutEdges;
+  }
+  static float gatherReduce(const float &left, const float &right) {
+    return left + right;
+  }
+  static bool apply(VertexData *vertexData, const float &gatherResult) {
+    float newRank = pageConst + (1.0f - pageConst) * gatherResult;
+    bool ret = std::fabs(newRank - vertexData->rank) >= tol;
+    vertexData->rank = newRank;
+    return ret;
+  }
+  static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+struct CC {
+  typedef int VertexData;
+  struct EdgeData {};
+  typedef int GatherResult;
+  static const int gatherZero = INT_MAX;
+  static int gatherReduce(const int &left, const int &right) {
+    return std::min(left, right);
+  }
+  static int gatherMap(const VertexData *, const VertexData *srcLabel,
+                       const EdgeData *) {
+    return *srcLabel;
+  }
+  static bool apply(VertexData *curLabel, GatherResult label) {
+    bool changed = label < *curLabel;
+    *curLabel = std::min(*curLabel, label);
+    return changed;
+  }
+  static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+}  // namespace
+
+extern "C" {
+
+// bfs.cu:91-112; returns microseconds of the timed window (frontier init
+// through results, as "Took" in the reference).
+int64_t ref_bfs(int nV, int nE, const int *srcs, const int *dsts, int source,
+                int *depth) {
+  std::vector<BFS::VertexData> vd(nV);
+  for (int i = 0; i < nV; ++i) vd[i].depth = -1;
+  GASEngineRef<BFS> engine;
+  engine.setGraph(nV, vd.data(), nE, 0, srcs, dsts);
+  engine.setActive(source, source + 1);
+  int iter = 0;
+  g_iterationCount = iter;
+  int64_t t0 = currentTime();
+  while (engine.countActive()) {
+    engine.gatherApply(false);
+    engine.scatterActivate(false);
+    engine.nextIter();
+    g_iterationCount = ++iter;
+  }
+  engine.getResults();
+  int64_t t1 = currentTime();
+  for (int i = 0; i < nV; ++i) depth[i] = vd[i].depth;
+  return t1 - t0;
+}
+
+// sssp.cu:67-82 with sssp.cu:358-362
+int64_t ref_sssp(int nV, int nE, const int *srcs, const int *dsts,
+                 const int *weights, int source, int *dist) {
+  std::vector<int> vd(nV, SSSP::gatherZero);
+  vd[source] = 0;
+  std::vector<int> w(weights, weights + nE);
+  GASEngineRef<SSSP> engine;
+  engine.setGraph(nV, vd.data(), nE, w.data(), srcs, dsts);
+  engine.setActive(0, nV);
+  int64_t t0 = currentTime();
+  engine.run();
+  engine.getResults();
+  int64_t t1 = currentTime();
+  for (int i = 0; i < nV; ++i) dist[i] = vd[i];
+  return t1 - t0;
+}
+
+// connected_component.cu:66-81 with :112-114
+int64_t ref_cc(int nV, int nE, const int *srcs, const int *dsts, int *label) {
+  std::vector<int> vd(nV);
+  for (int i = 0; i < nV; ++i) vd[i] = i;
+  GASEngineRef<CC> engine;
+  engine.setGraph(nV, vd.data(), nE, 0, srcs, dsts);
+  engine.setActive(0, nV);
+  int64_t t0 = currentTime();
+  engine.run();
+  engine.getResults();
+  int64_t t1 = currentTime();
+  for (int i = 0; i < nV; ++i) label[i] = vd[i];
+  return t1 - t0;
+}
+
+// pagerank.cu:85-101 with pagerank.cu:126-132.  iterations > 0 replaces
+// run() by that many all-active sweeps (setActive(0,V); gatherApply()).
+int64_t ref_pagerank(int nV, int nE, const int *srcs, const int *dsts,
+                     int iterations, float *rank) {
+  std::vector<int> srcOffsets(nV + 1);
+  edgeListToCSR<int>(nV, nE, srcs, dsts, &srcOffsets[0], 0, 0);
+  std::vector<PageRank::VertexData> vd(nV);
+  for (int i = 0; i < nV; ++i) {
+    vd[i].numOutEdges = srcOffsets[i + 1] - srcOffsets[i];
+    vd[i].rank = PageRank::pageConst;
+  }
+  GASEngineRef<PageRank> engine;
+  engine.setGraph(nV, vd.data(), nE, 0, srcs, dsts);
+  int64_t t0 = currentTime();
+  if (iterations > 0) {
+    for (int it = 0; it < iterations; ++it) {
+      engine.setActive(0, nV);
+      engine.gatherApply();
+    }
+  } else {
+    engine.setActive(0, nV);
+    engine.run();
+  }
+  engine.getResults();
+  int64_t t1 = currentTime();
+  for (int i = 0; i < nV; ++i) rank[i] = vd[i].rank;
+  return t1 - t0;
+}
+
+// util.h:79-110, exposed for layout checks.
+void ref_edge_list_to_csr(int nV, int nE, const int *srcs, const int *dsts,
+                          int *offsets, int *outDsts, int *sortIndices) {
+  edgeListToCSR<int>(nV, nE, srcs, dsts, offsets, outDsts, sortIndices);
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+"""vertexapi2_amd -- Python bindings of libgas.so, the MI355X GAS engine.
+
+The library is the product; this module only binds its C ABI (include/gas.h)
+with ctypes and moves numpy arrays across it.  Loading fails loudly when
+libgas.so has not been built: there is no fallback implementation.
+
+    import vertexapi2_amd as gas
+    el = gas.Edges.road(1024, 1024, seed=1)
+    g = gas.Graph(el)
+    depth, stats = g.bfs(0)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libgas.so")
+
+PROGRAM_BFS, PROGRAM_SSSP, PROGRAM_PAGERANK, PROGRAM_CC = 0, 1, 2, 3
+SSSP_UNREACHED = 2147483647 - 100000  # sssp.cu:32-33 gatherZero
+
+
+class GasError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class EdgesView(C.Structure):
+    _fields_ = [("n_vertices", C.c_int32), ("n_edges", C.c_int32),
+                ("srcs", C.POINTER(C.c_int32)), ("dsts", C.POINTER(C.c_int32)),
+                ("weights", C.POINTER(C.c_int32))]
+
+
+class GraphView(C.Structure):
+    _fields_ = [("n_vertices", C.c_int32), ("n_edges", C.c_int32),
+                ("csc_offsets", C.c_void_p), ("csc_srcs", C.c_void_p),
+                ("csr_offsets", C.c_void_p), ("csr_dsts", C.c_void_p),
+                ("csc_weights", C.c_void_p), ("device_bytes", C.c_size_t)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("supersteps", C.c_int32), ("reserved", C.c_int32),
+                ("gathered_edges", C.c_int64), ("scattered_edges", C.c_int64),
+                ("active_total", C.c_int64), ("ms", C.c_double),
+                ("gather_ms", C.c_double), ("scatter_ms", C.c_double),
+                ("gather_launches", C.c_int32), ("scatter_launches", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libgas.so (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m vertexapi2_amd.build` (hipcc, gfx950)")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.gas_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GasError(rc, lib().gas_last_error().decode())
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a: np.ndarray, ctype=C.c_int32):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().gasrt_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def device_info(device: int = 0) -> dict:
+    cu, lds, hbm = C.c_int(0), C.c_int(0), C.c_size_t(0)
+    name = C.create_string_buffer(256)
+    _check(lib().gasrt_device_info(device, C.byref(cu), C.byref(lds), C.byref(hbm),
+                                   name, C.c_size_t(256)))
+    return {"cu_count": cu.value, "lds_per_block": lds.value,
+            "hbm_bytes": hbm.value, "name": name.value.decode()}
+
+
+def set_kernel_timing(enable: bool) -> None:
+    _check(lib().gas_set_kernel_timing(1 if enable else 0))
+
+
+class Edges:
+    """Host edge list (gas_edges_t)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gas_edges_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def from_arrays(cls, n_vertices: int, srcs, dsts, weights=None) -> "Edges":
+        s, d = _i32(srcs), _i32(dsts)
+        w = None if weights is None else _i32(weights)
+        h = C.c_void_p()
+        _check(lib().gas_edges_from_arrays(
+            C.c_int32(n_vertices), C.c_int32(len(s)), _ptr(s), _ptr(d),
+            _ptr(w) if w is not None else None, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def grid(cls, rows: int, cols: int) -> "Edges":
+        h = C.c_void_p()
+        _check(lib().gas_gen_grid(C.c_int32(rows), C.c_int32(cols), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def road(cls, rows: int, cols: int, drop_frac: float = 0.03,
+             long_frac: float = 0.01, wmin: int = 1, wmax: int = 64,
+             seed: int = 1) -> "Edges":
+        h = C.c_void_p()
+        _check(lib().gas_gen_road(C.c_int32(rows), C.c_int32(cols),
+                                  C.c_double(drop_frac), C.c_double(long_frac),
+                                  C.c_int32(wmin), C.c_int32(wmax),
+                                  C.c_uint64(seed), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57,
+             b: float = 0.19, c: float = 0.19, seed: int = 1,
+             symmetrize: bool = False, wmin: int = 1, wmax: int = 0) -> "Edges":
+        h = C.c_void_p()
+        _check(lib().gas_gen_rmat(C.c_int32(scale), C.c_int32(edge_factor),
+                                  C.c_double(a), C.c_double(b), C.c_double(c),
+                                  C.c_uint64(seed), C.c_int(1 if symmetrize else 0),
+                                  C.c_int32(wmin), C.c_int32(wmax), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_mtx(cls, path: str, want_values: bool = False) -> "Edges":
+        h = C.c_void_p()
+        _check(lib().gas_load_mtx(path.encode(), C.c_int(1 if want_values else 0),
+                                  C.byref(h)))
+        return cls(h)
+
+    def view(self) -> EdgesView:
+        v = EdgesView()
+        _check(lib().gas_edges_view_get(self._h, C.byref(v)))
+        return v
+
+    @property
+    def n_vertices(self) -> int:
+        return self.view().n_vertices
+
+    @property
+    def n_edges(self) -> int:
+        return self.view().n_edges
+
+    def arrays(self):
+        """(srcs, dsts, weights or None) as numpy copies."""
+        v = self.view()
+        n = v.n_edges
+        if n == 0:
+            e = np.zeros(0, np.int32)
+            return e, e.copy(), (None if not v.weights else e.copy())
+        s = np.ctypeslib.as_array(v.srcs, shape=(n,)).copy()
+        d = np.ctypeslib.as_array(v.dsts, shape=(n,)).copy()
+        w = np.ctypeslib.as_array(v.weights, shape=(n,)).copy() if v.weights else None
+        return s, d, w
+
+    def max_outdegree_vertex(self):
+        v, d = C.c_int32(0), C.c_int32(0)
+        _check(lib().gas_edges_max_outdegree_vertex(self._h, C.byref(v), C.byref(d)))
+        return v.value, d.value
+
+
+class Graph:
+    """A graph resident in HBM (gas_graph_t): CSC + CSR + CSC-order weights."""
+
+    def __init__(self, edges: Edges):
+        h = C.c_void_p()
+        _check(lib().gas_graph_create(edges.handle, C.byref(h)))
+        self._h = h
+        v = self.view()
+        self.n_vertices, self.n_edges = v.n_vertices, v.n_edges
+        self.has_weights = bool(v.csc_weights)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gas_graph_free(self._h)
+            self._h = None
+
+    def free(self) -> None:
+        self.__del__()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def view(self) -> GraphView:
+        v = GraphView()
+        _check(lib().gas_graph_view_get(self._h, C.byref(v)))
+        return v
+
+    def host_csc(self):
+        off = np.zeros(self.n_vertices + 1, np.int32)
+        src = np.zeros(self.n_edges, np.int32)
+        eid = np.zeros(self.n_edges, np.int32)
+        _check(lib().gas_graph_host_csc(self._h, _ptr(off), _ptr(src), _ptr(eid)))
+        return off, src, eid
+
+    def host_csr(self):
+        off = np.zeros(self.n_vertices + 1, np.int32)
+        dst = np.zeros(self.n_edges, np.int32)
+        eid = np.zeros(self.n_edges, np.int32)
+        _check(lib().gas_graph_host_csr(self._h, _ptr(off), _ptr(dst), _ptr(eid)))
+        return off, dst, eid
+
+    def bfs(self, source: int):
+        out = np.zeros(self.n_vertices, np.int32)
+        st = RunStats()
+        _check(lib().gas_bfs(self._h, C.c_int32(source), _ptr(out), C.byref(st)))
+        return out, st
+
+    def sssp(self, source: int):
+        out = np.zeros(self.n_vertices, np.int32)
+        st = RunStats()
+        _check(lib().gas_sssp(self._h, C.c_int32(source), _ptr(out), C.byref(st)))
+        return out, st
+
+    def cc(self):
+        out = np.zeros(self.n_vertices, np.int32)
+        st = RunStats()
+        _check(lib().gas_cc(self._h, _ptr(out), C.byref(st)))
+        return out, st
+
+    def pagerank(self, iterations: int = 20):
+        out = np.zeros(self.n_vertices, np.float32)
+        res = np.zeros(max(iterations, 1), np.float64)
+        st = RunStats()
+        _check(lib().gas_pagerank(self._h, C.c_int32(iterations), _ptr(out, C.c_float),
+                                  _ptr(res, C.c_double), C.byref(st)))
+        return out, (res[:iterations] if iterations > 0 else None), st
+
+
+# VertexData record layouts of the four programs (include/gas/programs.hpp).
+VERTEX_DTYPES = {
+    PROGRAM_BFS: np.dtype([("depth", np.int32)]),
+    PROGRAM_SSSP: np.dtype(np.int32),
+    PROGRAM_PAGERANK: np.dtype([("rank", np.float32), ("numOutEdges", np.int32)]),
+    PROGRAM_CC: np.dtype(np.int32),
+}
+
+
+class Engine:
+    """The reference engine API step by step (gpugas.h:212-772).
+
+    `vertex_data` is a numpy array of the program's VertexData records; it
+    is retained and written back by get_results(), as in the reference.
+    """
+
+    def __init__(self, program: int, n_vertices: int, vertex_data: np.ndarray,
+                 srcs, dsts, edge_data=None):
+        assert vertex_data.dtype == VERTEX_DTYPES[program]
+        assert vertex_data.flags["C_CONTIGUOUS"] and len(vertex_data) == n_vertices
+        self.vertex_data = vertex_data
+        s, d = _i32(srcs), _i32(dsts)
+        self._ed = None if edge_data is None else _i32(edge_data)
+        h = C.c_void_p()
+        _check(lib().gas_engine_create(
+            C.c_int(program), C.c_int32(n_vertices), vertex_data.ctypes.data_as(C.c_void_p),
+            C.c_int32(len(s)),
+            self._ed.ctypes.data_as(C.c_void_p) if self._ed is not None else None,
+            _ptr(s), _ptr(d), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gas_engine_destroy(self._h)
+            self._h = None
+
+    def setActive(self, start: int, end: int) -> None:
+        _check(lib().gas_engine_set_active(self._h, C.c_int32(start), C.c_int32(end)))
+
+    def countActive(self) -> int:
+        n = C.c_int32(0)
+        _check(lib().gas_engine_count_active(self._h, C.byref(n)))
+        return n.value
+
+    def gatherApply(self, haveGather: bool = True) -> None:
+        _check(lib().gas_engine_gather_apply(self._h, C.c_int(1 if haveGather else 0)))
+
+    def scatterActivate(self, haveScatter: bool = True) -> None:
+        _check(lib().gas_engine_scatter_activate(self._h, C.c_int(1 if haveScatter else 0)))
+
+    def nextIter(self) -> int:
+        n = C.c_int32(0)
+        _check(lib().gas_engine_next_iter(self._h, C.byref(n)))
+        return n.value
+
+    def run(self) -> None:
+        _check(lib().gas_engine_run(self._h))
+
+    def getResults(self) -> None:
+        _check(lib().gas_engine_get_results(self._h))
+
+    def setIteration(self, it: int) -> None:
+        _check(lib().gas_engine_set_iteration(self._h, C.c_int32(it)))
+
+    def stats(self) -> RunStats:
+        st = RunStats()
+        _check(lib().gas_engine_stats(self._h, C.byref(st)))
+        return st

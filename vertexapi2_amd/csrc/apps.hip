@@ -1,0 +1,395 @@
+// C ABI over GASEngineGPU for the four vertex programs: the step-by-step
+// engine API (gas_engine_*) and the app drivers' whole runs (gas_bfs,
+// gas_sssp, gas_cc, gas_pagerank).
+//
+// Whole runs follow the drivers' initialisation and loops:
+//   BFS       bfs.cu:91-112, 144-146   depth = -1, setActive(src, src+1),
+//                                      gatherApply(false) +
+//                                      scatterActivate(false) per level
+//   SSSP      sssp.cu:67-82, 141-144   dist = gatherZero, dist[src] = 0,
+//                                      setActive(0, V), run()
+//   CC        connected_component.cu:66-81, 112-114   label = id, run()
+//   PageRank  pagerank.cu:85-101, 126-132   rank = 0.15, numOutEdges from
+//                                      CSR, setActive(0, V), run(); or a
+//                                      fixed number of all-active sweeps
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "gas.h"
+#include "gas/engine.hpp"
+#include "gas/programs.hpp"
+#include "gas_internal.hpp"
+
+namespace gas {
+
+// Sum of the apply kernel's per-chunk FP64 residuals into hist[slot]
+// (single workgroup, fixed order).
+__global__ __launch_bounds__(NT) void k_residual_sum(const double *rpart,
+                                                     double *hist, int slot) {
+  __shared__ double lds[NT / WAVE];
+  double s = 0;
+  for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) s += rpart[i];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) hist[slot] = s;
+}
+
+// Type-erased engine for the C ABI.
+struct AnyEngine {
+  virtual ~AnyEngine() {}
+  virtual void setActive(int s, int e) = 0;
+  virtual int countActive() = 0;
+  virtual void gatherApply(bool g) = 0;
+  virtual void scatterActivate(bool s) = 0;
+  virtual int nextIter() = 0;
+  virtual void run() = 0;
+  virtual void getResults() = 0;
+  virtual void setIteration(int it) = 0;
+  virtual void stats(gas_run_stats *st) = 0;
+};
+
+template <class P>
+struct EngineImpl : AnyEngine {
+  GASEngineGPU<P> e;
+  void setActive(int s, int en) override { e.setActive(s, en); }
+  int countActive() override { return e.countActive(); }
+  void gatherApply(bool g) override { e.gatherApply(g); }
+  void scatterActivate(bool s) override { e.scatterActivate(s); }
+  int nextIter() override { return e.nextIter(); }
+  void run() override { e.run(); }
+  void getResults() override { e.getResults(); }
+  void setIteration(int it) override { e.setIteration(it); }
+  void stats(gas_run_stats *st) override {
+    e.stats(st);
+    st->ms = 0;
+  }
+};
+
+template <class F>
+static int guarded(F f) {
+  try {
+    clear_error();
+    f();
+    return GAS_OK;
+  } catch (const Error &x) {
+    return set_error(x.code ? x.code : GAS_ERR_HIP, "%s", x.what());
+  } catch (const std::bad_alloc &) {
+    return set_error(GAS_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &x) {
+    return set_error(GAS_ERR_STATE, "%s", x.what());
+  }
+}
+
+template <class P>
+static GASEngineGPU<P> &cached_engine(gas_graph_t g, int slot,
+                                      typename P::VertexData *vd) {
+  if (!g->engine_cache[slot]) {
+    auto *e = new GASEngineGPU<P>();
+    try {
+      e->setGraph(g, vd, nullptr);
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    g->engine_cache[slot] = e;
+    g->engine_release[slot] = [](void *p) {
+      delete static_cast<GASEngineGPU<P> *>(p);
+    };
+  } else {
+    static_cast<GASEngineGPU<P> *>(g->engine_cache[slot])->resetVertexData(vd);
+  }
+  return *static_cast<GASEngineGPU<P> *>(g->engine_cache[slot]);
+}
+
+// Times a whole run on the engine's stream with HIP events.
+struct RunTimer {
+  gasrt_event_t a = nullptr, b = nullptr;
+  gasrt_stream_t s;
+  explicit RunTimer(gasrt_stream_t st) : s(st) {
+    check(gasrt_event_create(&a), "event");
+    check(gasrt_event_create(&b), "event");
+    check(gasrt_event_record(a, s), "event");
+  }
+  double stop() {
+    check(gasrt_event_record(b, s), "event");
+    check(gasrt_event_synchronize(b), "event");
+    float ms = 0;
+    check(gasrt_event_elapsed_ms(a, b, &ms), "elapsed");
+    return ms;
+  }
+  ~RunTimer() {
+    gasrt_event_destroy(a);
+    gasrt_event_destroy(b);
+  }
+};
+
+}  // namespace gas
+
+using namespace gas;
+
+extern "C" {
+
+int gas_set_kernel_timing(int enable) {
+  kernel_timing_flag() = enable ? 1 : 0;
+  return GAS_OK;
+}
+
+int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
+            gas_run_stats *stats) {
+  if (!g || !depth_out || source < 0 || source >= g->nV)
+    return set_error(GAS_ERR_ARG, "gas_bfs: bad arguments");
+  return guarded([&] {
+    std::vector<BFS::VertexData> vd(g->nV);
+    for (auto &x : vd) x.depth = -1;
+    auto &e = cached_engine<BFS>(g, GAS_PROGRAM_BFS, vd.data());
+    e.resetStats();
+    RunTimer tm(e.stream());
+    e.setActive(source, source + 1);
+    e.setIteration(0);
+    e.runUntilEmpty(false, false);
+    double ms = tm.stop();
+    e.getResults();
+    for (int32_t v = 0; v < g->nV; ++v) depth_out[v] = vd[v].depth;
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
+             gas_run_stats *stats) {
+  if (!g || !dist_out || source < 0 || source >= g->nV)
+    return set_error(GAS_ERR_ARG, "gas_sssp: bad arguments");
+  if (!g->csc_w && g->nE > 0)
+    return set_error(GAS_ERR_ARG, "gas_sssp: graph has no edge weights");
+  return guarded([&] {
+    std::vector<int> vd(g->nV, SSSP::gatherZero);
+    vd[source] = 0;
+    auto &e = cached_engine<SSSP>(g, GAS_PROGRAM_SSSP, vd.data());
+    e.resetStats();
+    RunTimer tm(e.stream());
+    e.setActive(0, g->nV);
+    e.setIteration(0);
+    e.runUntilEmpty(true, true);
+    double ms = tm.stop();
+    e.getResults();
+    std::copy(vd.begin(), vd.end(), dist_out);
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats) {
+  if (!g || !label_out) return set_error(GAS_ERR_ARG, "gas_cc: bad arguments");
+  return guarded([&] {
+    std::vector<int> vd(g->nV);
+    for (int32_t v = 0; v < g->nV; ++v) vd[v] = v;
+    auto &e = cached_engine<CC>(g, GAS_PROGRAM_CC, vd.data());
+    e.resetStats();
+    RunTimer tm(e.stream());
+    e.setActive(0, g->nV);
+    e.setIteration(0);
+    e.runUntilEmpty(true, true);
+    double ms = tm.stop();
+    e.getResults();
+    std::copy(vd.begin(), vd.end(), label_out);
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
+                 double *residual_out, gas_run_stats *stats) {
+  if (!g || !rank_out || iterations < 0)
+    return set_error(GAS_ERR_ARG, "gas_pagerank: bad arguments");
+  return guarded([&] {
+    std::vector<PageRank::VertexData> vd(g->nV);
+    for (int32_t v = 0; v < g->nV; ++v) {
+      vd[v].rank = PageRank::pageConst;
+      vd[v].numOutEdges = g->h_csr_off[v + 1] - g->h_csr_off[v];
+    }
+    auto &e = cached_engine<PageRank>(g, GAS_PROGRAM_PAGERANK, vd.data());
+    e.resetStats();
+    DevBuf<double> hist;
+    hist.alloc(iterations > 0 ? (size_t)iterations : 1);
+    RunTimer tm(e.stream());
+    if (iterations > 0) {
+      for (int it = 0; it < iterations; ++it) {
+        e.setActive(0, g->nV);
+        e.gatherApply(true);
+        launch(k_residual_sum, 1, e.stream(), e.dev().rpart, hist.p, it);
+      }
+    } else {
+      e.setActive(0, g->nV);
+      e.setIteration(0);
+      e.runUntilEmpty(true, true);
+    }
+    double ms = tm.stop();
+    e.getResults();
+    for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[v].rank;
+    if (residual_out && iterations > 0)
+      check(gasrt_memcpy_d2h(residual_out, hist.p,
+                             sizeof(double) * (size_t)iterations, e.stream()),
+            "residuals");
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+      if (iterations > 0) stats->supersteps = iterations;
+    }
+  });
+}
+
+// ---- step-by-step engine API ------------------------------------------------
+
+struct gas_engine {
+  std::unique_ptr<AnyEngine> impl;
+};
+
+static AnyEngine *make_engine(int program) {
+  switch (program) {
+    case GAS_PROGRAM_BFS: return new EngineImpl<BFS>();
+    case GAS_PROGRAM_SSSP: return new EngineImpl<SSSP>();
+    case GAS_PROGRAM_PAGERANK: return new EngineImpl<PageRank>();
+    case GAS_PROGRAM_CC: return new EngineImpl<CC>();
+    default: return nullptr;
+  }
+}
+
+int gas_engine_create(int program, int32_t n_vertices, void *vertex_data,
+                      int32_t n_edges, const void *edge_data,
+                      const int32_t *srcs, const int32_t *dsts,
+                      gas_engine_t *out) {
+  if (!out || !vertex_data || n_vertices < 0 || n_edges < 0)
+    return set_error(GAS_ERR_ARG, "gas_engine_create: bad arguments");
+  if (program < 0 || program > 3)
+    return set_error(GAS_ERR_ARG, "gas_engine_create: unknown program %d",
+                     program);
+  return guarded([&] {
+    std::unique_ptr<gas_engine> h(new gas_engine);
+    h->impl.reset(make_engine(program));
+    switch (program) {
+      case GAS_PROGRAM_BFS:
+        static_cast<EngineImpl<BFS> *>(h->impl.get())
+            ->e.setGraph(n_vertices, (BFS::VertexData *)vertex_data, n_edges,
+                         nullptr, srcs, dsts);
+        break;
+      case GAS_PROGRAM_SSSP:
+        if (!edge_data && n_edges > 0)
+          throw Error(GAS_ERR_ARG, "SSSP needs edge data (weights)");
+        static_cast<EngineImpl<SSSP> *>(h->impl.get())
+            ->e.setGraph(n_vertices, (int *)vertex_data, n_edges,
+                         (int *)edge_data, srcs, dsts);
+        break;
+      case GAS_PROGRAM_PAGERANK:
+        static_cast<EngineImpl<PageRank> *>(h->impl.get())
+            ->e.setGraph(n_vertices, (PageRank::VertexData *)vertex_data,
+                         n_edges, nullptr, srcs, dsts);
+        break;
+      case GAS_PROGRAM_CC:
+        static_cast<EngineImpl<CC> *>(h->impl.get())
+            ->e.setGraph(n_vertices, (int *)vertex_data, n_edges, nullptr,
+                         srcs, dsts);
+        break;
+    }
+    *out = h.release();
+  });
+}
+
+int gas_engine_create_on_graph(int program, gas_graph_t g, void *vertex_data,
+                               const void *edge_data, gas_engine_t *out) {
+  if (!out || !g || !vertex_data)
+    return set_error(GAS_ERR_ARG, "gas_engine_create_on_graph: bad arguments");
+  if (program < 0 || program > 3)
+    return set_error(GAS_ERR_ARG, "gas_engine_create_on_graph: program %d",
+                     program);
+  return guarded([&] {
+    std::unique_ptr<gas_engine> h(new gas_engine);
+    h->impl.reset(make_engine(program));
+    switch (program) {
+      case GAS_PROGRAM_BFS:
+        static_cast<EngineImpl<BFS> *>(h->impl.get())
+            ->e.setGraph(g, (BFS::VertexData *)vertex_data, nullptr);
+        break;
+      case GAS_PROGRAM_SSSP:
+        static_cast<EngineImpl<SSSP> *>(h->impl.get())
+            ->e.setGraph(g, (int *)vertex_data, (const int *)edge_data);
+        break;
+      case GAS_PROGRAM_PAGERANK:
+        static_cast<EngineImpl<PageRank> *>(h->impl.get())
+            ->e.setGraph(g, (PageRank::VertexData *)vertex_data, nullptr);
+        break;
+      case GAS_PROGRAM_CC:
+        static_cast<EngineImpl<CC> *>(h->impl.get())
+            ->e.setGraph(g, (int *)vertex_data, nullptr);
+        break;
+    }
+    *out = h.release();
+  });
+}
+
+#define ENGINE_OR_FAIL(e)                                            \
+  if (!(e) || !(e)->impl)                                            \
+  return set_error(GAS_ERR_ARG, "%s: null engine", __func__)
+
+int gas_engine_set_active(gas_engine_t e, int32_t start, int32_t end) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->setActive(start, end); });
+}
+
+int gas_engine_count_active(gas_engine_t e, int32_t *count) {
+  ENGINE_OR_FAIL(e);
+  if (!count) return set_error(GAS_ERR_ARG, "gas_engine_count_active: null");
+  return guarded([&] { *count = e->impl->countActive(); });
+}
+
+int gas_engine_gather_apply(gas_engine_t e, int have_gather) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->gatherApply(have_gather != 0); });
+}
+
+int gas_engine_scatter_activate(gas_engine_t e, int have_scatter) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->scatterActivate(have_scatter != 0); });
+}
+
+int gas_engine_next_iter(gas_engine_t e, int32_t *count) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] {
+    int n = e->impl->nextIter();
+    if (count) *count = n;
+  });
+}
+
+int gas_engine_run(gas_engine_t e) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->run(); });
+}
+
+int gas_engine_get_results(gas_engine_t e) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->getResults(); });
+}
+
+int gas_engine_set_iteration(gas_engine_t e, int32_t iteration) {
+  ENGINE_OR_FAIL(e);
+  return guarded([&] { e->impl->setIteration(iteration); });
+}
+
+int gas_engine_stats(gas_engine_t e, gas_run_stats *stats) {
+  ENGINE_OR_FAIL(e);
+  if (!stats) return set_error(GAS_ERR_ARG, "gas_engine_stats: null");
+  return guarded([&] { e->impl->stats(stats); });
+}
+
+int gas_engine_destroy(gas_engine_t e) {
+  delete e;
+  return GAS_OK;
+}
+
+}  // extern "C"

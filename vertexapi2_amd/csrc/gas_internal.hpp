@@ -1,0 +1,59 @@
+// Internal declarations shared by the libgas.so translation units.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gas.h"
+
+namespace gas {
+
+int set_error(int code, const char *fmt, ...)
+    __attribute__((format(printf, 2, 3)));
+void clear_error();
+int hip_status(int hip_err, const char *what);
+
+// Returns from the enclosing function with the status of `expr` if nonzero.
+#define GAS_TRY(expr)              \
+  do {                             \
+    int gas_rc_ = (expr);          \
+    if (gas_rc_) return gas_rc_;   \
+  } while (0)
+
+}  // namespace gas
+
+// Host edge list (owned).
+struct gas_edges {
+  int32_t nV = 0;
+  std::vector<int32_t> src, dst, w;  // w empty when the list has no values
+};
+
+// Graph resident in HBM plus the host-side index maps needed to reorder
+// per-edge user data.
+struct gas_graph {
+  int32_t nV = 0, nE = 0;
+  std::vector<int32_t> h_csc_off, h_csc_src, h_csc_eid;
+  std::vector<int32_t> h_csr_off, h_csr_dst, h_csr_eid;
+  int32_t *csc_off = nullptr, *csc_src = nullptr;
+  int32_t *csr_off = nullptr, *csr_dst = nullptr;
+  int32_t *csc_w = nullptr;
+  size_t device_bytes = 0;
+  // Engines kept bound to this graph by the whole-run entry points
+  // (gas_bfs, ...), one per program, released with the graph.
+  void *engine_cache[4] = {nullptr, nullptr, nullptr, nullptr};
+  void (*engine_release[4])(void *) = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace gas {
+
+// Stable counting sort of edges by key: offsets (V+1), values permuted into
+// key order, and the original edge index of each slot.  Equal keys keep
+// edge-list order (one valid order of the reference's unstable std::sort,
+// util.h:48-72).
+void build_compressed(int32_t nV, int32_t nE, const int32_t *keys,
+                      const int32_t *vals, std::vector<int32_t> &off,
+                      std::vector<int32_t> &out_vals,
+                      std::vector<int32_t> &eid);
+
+}  // namespace gas

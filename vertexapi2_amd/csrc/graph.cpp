@@ -1,0 +1,568 @@
+// Host graph toolkit: seeded synthetic generators, the MatrixMarket loader,
+// CSR/CSC construction and upload of a graph into HBM.
+//
+// CSR/CSC layout follows util.h:79-121 (offsets of V+1 entries, neighbour
+// ids sorted by key vertex, plus the original edge index of every slot);
+// the sort here is a stable counting sort instead of the reference's
+// std::sort on (key, index) pairs (util.h:48-72).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <thread>
+#include <unordered_set>
+
+#include "gas.h"
+#include "gas_internal.hpp"
+
+namespace gas {
+
+// ---------------------------------------------------------------------------
+// Random numbers: SplitMix64 for seeding, xoshiro256** for streams
+// (Blackman & Vigna, "Scrambled linear pseudorandom number generators").
+static inline uint64_t splitmix64(uint64_t &x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct Xoshiro256 {
+  uint64_t s[4];
+  explicit Xoshiro256(uint64_t seed) {
+    uint64_t x = seed;
+    for (auto &v : s) v = splitmix64(x);
+  }
+  static inline uint64_t rotl(uint64_t x, int k) {
+    return (x << k) | (x >> (64 - k));
+  }
+  inline uint64_t next() {
+    uint64_t r = rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  // uniform in [0, 1) with 53 bits
+  inline double uniform() { return (next() >> 11) * 0x1.0p-53; }
+  // uniform integer in [0, n) (Lemire's multiply-shift; bias < 2^-32 for
+  // the n used here)
+  inline uint64_t below(uint64_t n) {
+    return (uint64_t)(((unsigned __int128)next() * n) >> 64);
+  }
+};
+
+// Stateless hash of a vertex pair, used so both directions of a link get
+// the same weight regardless of generation order.
+static inline uint64_t pair_hash(uint64_t seed, uint32_t u, uint32_t v) {
+  uint64_t x = seed ^ (((uint64_t)u << 32) | v);
+  splitmix64(x);
+  return splitmix64(x);
+}
+
+static int hw_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 4;
+  return (int)std::min(n, 16u);
+}
+
+template <class F>
+static void parallel_for(int64_t n, F f) {
+  int nt = hw_threads();
+  if (n < 1 << 16 || nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    int64_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=] { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+void build_compressed(int32_t nV, int32_t nE, const int32_t *keys,
+                      const int32_t *vals, std::vector<int32_t> &off,
+                      std::vector<int32_t> &out_vals,
+                      std::vector<int32_t> &eid) {
+  off.assign((size_t)nV + 1, 0);
+  for (int32_t k = 0; k < nE; ++k) off[(size_t)keys[k] + 1]++;
+  for (int32_t v = 0; v < nV; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> cur(off.begin(), off.end() - 1);
+  out_vals.resize(nE);
+  eid.resize(nE);
+  for (int32_t k = 0; k < nE; ++k) {
+    int32_t p = cur[keys[k]]++;
+    out_vals[p] = vals[k];
+    eid[p] = k;
+  }
+}
+
+// Sorts every adjacency list of a CSR by neighbour id and drops repeats;
+// returns the new edge list in (src, dst) order.
+static void dedupe_csr(int32_t nV, std::vector<int32_t> &off,
+                       std::vector<int32_t> &adj) {
+  std::vector<int32_t> cnt(nV, 0);
+  parallel_for(nV, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) {
+      int32_t *p = adj.data() + off[v];
+      int32_t *q = adj.data() + off[v + 1];
+      std::sort(p, q);
+      cnt[v] = (int32_t)(std::unique(p, q) - p);
+    }
+  });
+  std::vector<int32_t> noff((size_t)nV + 1, 0);
+  for (int32_t v = 0; v < nV; ++v) noff[v + 1] = noff[v] + cnt[v];
+  std::vector<int32_t> nadj(noff[nV]);
+  parallel_for(nV, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v)
+      std::copy(adj.begin() + off[v], adj.begin() + off[v] + cnt[v],
+                nadj.begin() + noff[v]);
+  });
+  off.swap(noff);
+  adj.swap(nadj);
+}
+
+}  // namespace gas
+
+using gas::set_error;
+
+extern "C" {
+
+int gas_edges_from_arrays(int32_t n_vertices, int32_t n_edges,
+                          const int32_t *srcs, const int32_t *dsts,
+                          const int32_t *weights, gas_edges_t *out) {
+  if (!out || n_vertices < 0 || n_edges < 0 ||
+      (n_edges && (!srcs || !dsts)))
+    return set_error(GAS_ERR_ARG, "gas_edges_from_arrays: bad arguments");
+  for (int32_t k = 0; k < n_edges; ++k) {
+    if (srcs[k] < 0 || srcs[k] >= n_vertices || dsts[k] < 0 ||
+        dsts[k] >= n_vertices)
+      return set_error(GAS_ERR_ARG,
+                       "gas_edges_from_arrays: edge %d (%d -> %d) outside "
+                       "[0, %d)",
+                       k, srcs[k], dsts[k], n_vertices);
+  }
+  auto *e = new gas_edges;
+  e->nV = n_vertices;
+  e->src.assign(srcs, srcs + n_edges);
+  e->dst.assign(dsts, dsts + n_edges);
+  if (weights) e->w.assign(weights, weights + n_edges);
+  *out = e;
+  return GAS_OK;
+}
+
+int gas_edges_view_get(gas_edges_t e, gas_edges_view *view) {
+  if (!e || !view) return set_error(GAS_ERR_ARG, "gas_edges_view_get: null");
+  view->n_vertices = e->nV;
+  view->n_edges = (int32_t)e->src.size();
+  view->srcs = e->src.data();
+  view->dsts = e->dst.data();
+  view->weights = e->w.empty() ? nullptr : e->w.data();
+  return GAS_OK;
+}
+
+int gas_edges_free(gas_edges_t e) {
+  delete e;
+  return GAS_OK;
+}
+
+int gas_gen_grid(int32_t rows, int32_t cols, gas_edges_t *out) {
+  if (!out || rows <= 0 || cols <= 0 || (int64_t)rows * cols > INT32_MAX)
+    return set_error(GAS_ERR_ARG, "gas_gen_grid: bad size %d x %d", rows,
+                     cols);
+  auto *e = new gas_edges;
+  e->nV = rows * cols;
+  int64_t links = (int64_t)rows * (cols - 1) + (int64_t)(rows - 1) * cols;
+  e->src.reserve(2 * links);
+  e->dst.reserve(2 * links);
+  for (int32_t r = 0; r < rows; ++r)
+    for (int32_t c = 0; c < cols; ++c) {
+      int32_t v = r * cols + c;
+      if (c + 1 < cols) {
+        e->src.push_back(v); e->dst.push_back(v + 1);
+        e->src.push_back(v + 1); e->dst.push_back(v);
+      }
+      if (r + 1 < rows) {
+        e->src.push_back(v); e->dst.push_back(v + cols);
+        e->src.push_back(v + cols); e->dst.push_back(v);
+      }
+    }
+  e->w.assign(e->src.size(), 1);
+  *out = e;
+  return GAS_OK;
+}
+
+int gas_gen_road(int32_t rows, int32_t cols, double drop_frac,
+                 double long_frac, int32_t wmin, int32_t wmax, uint64_t seed,
+                 gas_edges_t *out) {
+  if (!out || rows <= 1 || cols <= 1 || (int64_t)rows * cols > INT32_MAX ||
+      drop_frac < 0 || drop_frac > 1 || long_frac < 0 || wmin < 1 ||
+      wmax < wmin)
+    return set_error(GAS_ERR_ARG, "gas_gen_road: bad arguments");
+  const int64_t V = (int64_t)rows * cols;
+  const int64_t H = (int64_t)rows * (cols - 1);   // horizontal links
+  const int64_t L = H + (int64_t)(rows - 1) * cols;  // all grid links
+  const int64_t n_drop = (int64_t)std::floor(drop_frac * (double)L);
+  const int64_t n_long = (int64_t)std::floor(long_frac * (double)L);
+  if (n_long > V * (V - 1) / 2 - L)
+    return set_error(GAS_ERR_ARG, "gas_gen_road: too many long links");
+  gas::Xoshiro256 rng(seed);
+  // Partial Fisher-Yates: the first n_drop slots are the dropped links.
+  std::vector<int64_t> perm(L);
+  for (int64_t i = 0; i < L; ++i) perm[i] = i;
+  for (int64_t i = 0; i < n_drop; ++i) {
+    int64_t j = i + (int64_t)rng.below((uint64_t)(L - i));
+    std::swap(perm[i], perm[j]);
+  }
+  std::vector<uint8_t> dropped(L, 0);
+  for (int64_t i = 0; i < n_drop; ++i) dropped[perm[i]] = 1;
+  perm.clear();
+  perm.shrink_to_fit();
+
+  auto link_ends = [&](int64_t l, int32_t &u, int32_t &v) {
+    if (l < H) {
+      int64_t r = l / (cols - 1), c = l % (cols - 1);
+      u = (int32_t)(r * cols + c);
+      v = u + 1;
+    } else {
+      int64_t k = l - H;
+      u = (int32_t)k;  // (r, c) with r < rows-1, linear id r*cols + c
+      v = u + cols;
+    }
+  };
+  auto grid_adjacent = [&](int64_t a, int64_t b) {
+    if (a > b) std::swap(a, b);
+    if (b - a == cols) return true;
+    return b - a == 1 && (a % cols) != cols - 1;
+  };
+
+  auto *e = new gas_edges;
+  e->nV = (int32_t)V;
+  size_t m = 2 * (size_t)(L - n_drop + n_long);
+  e->src.reserve(m);
+  e->dst.reserve(m);
+  e->w.reserve(m);
+  const int32_t span = wmax - wmin + 1;
+  auto emit = [&](int32_t u, int32_t v) {
+    uint32_t a = (uint32_t)std::min(u, v), b = (uint32_t)std::max(u, v);
+    int32_t w = wmin + (int32_t)(gas::pair_hash(seed, a, b) % (uint64_t)span);
+    e->src.push_back(u); e->dst.push_back(v); e->w.push_back(w);
+    e->src.push_back(v); e->dst.push_back(u); e->w.push_back(w);
+  };
+  for (int64_t l = 0; l < L; ++l) {
+    if (dropped[l]) continue;
+    int32_t u, v;
+    link_ends(l, u, v);
+    emit(u, v);
+  }
+  std::unordered_set<uint64_t> added;
+  added.reserve((size_t)n_long * 2);
+  int64_t n_added = 0;
+  while (n_added < n_long) {
+    int64_t u = (int64_t)rng.below((uint64_t)V);
+    int64_t v = (int64_t)rng.below((uint64_t)V);
+    if (u == v || grid_adjacent(u, v)) continue;
+    uint64_t key = ((uint64_t)std::min(u, v) << 32) | (uint64_t)std::max(u, v);
+    if (!added.insert(key).second) continue;
+    emit((int32_t)u, (int32_t)v);
+    ++n_added;
+  }
+  *out = e;
+  return GAS_OK;
+}
+
+int gas_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                 double c, uint64_t seed, int symmetrize, int32_t wmin,
+                 int32_t wmax, gas_edges_t *out) {
+  if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 ||
+      b < 0 || c < 0 || a + b + c > 1.0 + 1e-12 ||
+      (wmax != 0 && (wmin < 1 || wmax < wmin)))
+    return set_error(GAS_ERR_ARG, "gas_gen_rmat: bad arguments");
+  const int64_t nV = (int64_t)1 << scale;
+  const int64_t m = (int64_t)edge_factor * nV;
+  if (m * (symmetrize ? 2 : 1) > INT32_MAX)
+    return set_error(GAS_ERR_ARG, "gas_gen_rmat: too many edges for int32");
+  std::vector<int32_t> us(m), vs(m);
+  const int64_t CH = 1 << 16;
+  const int64_t nch = (m + CH - 1) / CH;
+  const double ab = a + b, abc = a + b + c;
+  gas::parallel_for(nch, [&](int64_t cb, int64_t ce) {
+    for (int64_t ch = cb; ch < ce; ++ch) {
+      gas::Xoshiro256 rng(seed * 0x100000001B3ull + (uint64_t)ch);
+      int64_t e0 = ch * CH, e1 = std::min(m, e0 + CH);
+      for (int64_t k = e0; k < e1; ++k) {
+        uint32_t u = 0, v = 0;
+        for (int32_t lvl = 0; lvl < scale; ++lvl) {
+          uint32_t bit = 1u << (scale - 1 - lvl);
+          double r = rng.uniform();
+          if (r < a) {
+          } else if (r < ab) {
+            v |= bit;
+          } else if (r < abc) {
+            u |= bit;
+          } else {
+            u |= bit;
+            v |= bit;
+          }
+        }
+        us[k] = (int32_t)u;
+        vs[k] = (int32_t)v;
+      }
+    }
+  });
+  // Drop self loops; optionally add reverse edges.
+  std::vector<int32_t> ks, vals;
+  ks.reserve(symmetrize ? 2 * m : m);
+  vals.reserve(symmetrize ? 2 * m : m);
+  for (int64_t k = 0; k < m; ++k) {
+    if (us[k] == vs[k]) continue;
+    ks.push_back(us[k]);
+    vals.push_back(vs[k]);
+    if (symmetrize) {
+      ks.push_back(vs[k]);
+      vals.push_back(us[k]);
+    }
+  }
+  us.clear(); us.shrink_to_fit();
+  vs.clear(); vs.shrink_to_fit();
+  std::vector<int32_t> off, adj, eid;
+  gas::build_compressed((int32_t)nV, (int32_t)ks.size(), ks.data(),
+                        vals.data(), off, adj, eid);
+  ks.clear(); ks.shrink_to_fit();
+  vals.clear(); vals.shrink_to_fit();
+  eid.clear(); eid.shrink_to_fit();
+  gas::dedupe_csr((int32_t)nV, off, adj);
+  auto *e = new gas_edges;
+  e->nV = (int32_t)nV;
+  int64_t E = off[nV];
+  e->src.resize(E);
+  e->dst.swap(adj);
+  gas::parallel_for(nV, [&](int64_t vb, int64_t ve) {
+    for (int64_t v = vb; v < ve; ++v)
+      for (int32_t k = off[v]; k < off[v + 1]; ++k) e->src[k] = (int32_t)v;
+  });
+  if (wmax != 0) {
+    e->w.resize(E);
+    const uint64_t span = (uint64_t)(wmax - wmin + 1);
+    gas::parallel_for(E, [&](int64_t kb, int64_t ke) {
+      for (int64_t k = kb; k < ke; ++k) {
+        uint32_t x = (uint32_t)e->src[k], y = (uint32_t)e->dst[k];
+        uint32_t lo = std::min(x, y), hi = std::max(x, y);
+        e->w[k] = wmin + (int32_t)(gas::pair_hash(seed, lo, hi) % span);
+      }
+    });
+  }
+  *out = e;
+  return GAS_OK;
+}
+
+int gas_load_mtx(const char *path, int want_values, gas_edges_t *out) {
+  if (!path || !out) return set_error(GAS_ERR_ARG, "gas_load_mtx: null");
+  std::ifstream f(path);
+  if (!f) return set_error(GAS_ERR_IO, "gas_load_mtx: cannot open %s", path);
+  std::string line;
+  if (!std::getline(f, line))
+    return set_error(GAS_ERR_IO, "gas_load_mtx: error reading header");
+  std::istringstream hs(line);
+  std::string banner, object, format, field, symmetry;
+  hs >> banner >> object >> format >> field >> symmetry;
+  if (object != "matrix")
+    return set_error(GAS_ERR_IO, "gas_load_mtx: unrecognized token %s",
+                     object.c_str());
+  if (format != "coordinate")
+    return set_error(GAS_ERR_IO,
+                     "gas_load_mtx: only coordinate format is supported");
+  bool values = want_values != 0;
+  if (field == "pattern") values = false;
+  if (field == "complex")
+    return set_error(GAS_ERR_IO, "gas_load_mtx: complex values unsupported");
+  enum { NONE, SYM, SKEW, HERM } st;
+  if (symmetry == "general") st = NONE;
+  else if (symmetry == "symmetric") st = SYM;
+  else if (symmetry == "skew-symmetric") st = SKEW;
+  else if (symmetry == "hermitian") st = HERM;
+  else
+    return set_error(GAS_ERR_IO, "gas_load_mtx: unrecognized symmetry '%s'",
+                     symmetry.c_str());
+  if (st == HERM && values)
+    return set_error(GAS_ERR_IO,
+                     "gas_load_mtx: hermitian values are not supported");
+  auto *e = new gas_edges;
+  int64_t max_vertex = 0;
+  bool first_data = true;
+  int64_t line_no = 1;
+  while (std::getline(f, line)) {
+    ++line_no;
+    size_t p = line.find_first_not_of(" \t\r\n");
+    if (p == std::string::npos) continue;  // blank line
+    if (line[0] == '%') continue;           // comment
+    if (first_data) {                       // "rows cols nnz"
+      first_data = false;
+      continue;
+    }
+    long s, d;
+    int nb = 0;
+    if (sscanf(line.c_str(), "%ld%ld%n", &s, &d, &nb) != 2) {
+      delete e;
+      return set_error(GAS_ERR_IO,
+                       "gas_load_mtx: error parsing src/dst at line %ld",
+                       (long)line_no);
+    }
+    --s;
+    --d;
+    if (s < 0 || d < 0 || s > INT32_MAX - 1 || d > INT32_MAX - 1) {
+      delete e;
+      return set_error(GAS_ERR_IO, "gas_load_mtx: bad vertex id at line %ld",
+                       (long)line_no);
+    }
+    if (s == d) {
+      // graphio.cpp:133-141: a self loop is dropped; with edge values the
+      // line is skipped before it can raise the maximum id.
+      if (values) continue;
+    } else {
+      e->src.push_back((int32_t)s);
+      e->dst.push_back((int32_t)d);
+      if (st != NONE) {
+        e->src.push_back((int32_t)d);
+        e->dst.push_back((int32_t)s);
+      }
+      if (values) {
+        float val = 0.f;
+        sscanf(line.c_str() + nb, "%f", &val);
+        e->w.push_back((int32_t)val);
+        if (st == SYM) e->w.push_back((int32_t)val);
+        if (st == SKEW) e->w.push_back((int32_t)(-val));
+      }
+    }
+    max_vertex = std::max(max_vertex, (int64_t)std::max(s, d));
+  }
+  e->nV = (int32_t)(max_vertex + 1);
+  *out = e;
+  return GAS_OK;
+}
+
+int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,
+                                   int32_t *degree) {
+  if (!e || !vertex)
+    return set_error(GAS_ERR_ARG, "gas_edges_max_outdegree_vertex: null");
+  std::vector<int32_t> deg(e->nV, 0);
+  for (int32_t s : e->src) deg[s]++;
+  int32_t best = -1, bv = -1;
+  for (int32_t v = 0; v < e->nV; ++v)
+    if (deg[v] > best) {
+      best = deg[v];
+      bv = v;
+    }
+  *vertex = bv;
+  if (degree) *degree = best;
+  return GAS_OK;
+}
+
+static void graph_release(gas_graph *g) {
+  for (int i = 0; i < 4; ++i)
+    if (g->engine_cache[i] && g->engine_release[i])
+      g->engine_release[i](g->engine_cache[i]);
+  gasrt_free(g->csc_off);
+  gasrt_free(g->csc_src);
+  gasrt_free(g->csr_off);
+  gasrt_free(g->csr_dst);
+  gasrt_free(g->csc_w);
+}
+
+int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
+  if (!edges || !out) return set_error(GAS_ERR_ARG, "gas_graph_create: null");
+  auto *g = new gas_graph;
+  g->nV = edges->nV;
+  g->nE = (int32_t)edges->src.size();
+  gas::build_compressed(g->nV, g->nE, edges->dst.data(), edges->src.data(),
+                        g->h_csc_off, g->h_csc_src, g->h_csc_eid);
+  gas::build_compressed(g->nV, g->nE, edges->src.data(), edges->dst.data(),
+                        g->h_csr_off, g->h_csr_dst, g->h_csr_eid);
+  const size_t vb = ((size_t)g->nV + 1) * 4, eb = (size_t)g->nE * 4;
+  int rc = 0;
+  auto up = [&](int32_t *&dptr, const void *src, size_t bytes) {
+    if (rc) return;
+    rc = gasrt_malloc((void **)&dptr, bytes ? bytes : 4);
+    if (!rc) rc = gasrt_memcpy_h2d(dptr, src, bytes, nullptr);
+    g->device_bytes += bytes;
+  };
+  up(g->csc_off, g->h_csc_off.data(), vb);
+  up(g->csc_src, g->h_csc_src.data(), eb);
+  up(g->csr_off, g->h_csr_off.data(), vb);
+  up(g->csr_dst, g->h_csr_dst.data(), eb);
+  if (!edges->w.empty()) {
+    std::vector<int32_t> wc(g->nE);
+    for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csc_eid[k]];
+    up(g->csc_w, wc.data(), eb);
+  }
+  if (rc) {
+    graph_release(g);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return GAS_OK;
+}
+
+int gas_graph_view_get(gas_graph_t g, gas_graph_view *v) {
+  if (!g || !v) return set_error(GAS_ERR_ARG, "gas_graph_view_get: null");
+  v->n_vertices = g->nV;
+  v->n_edges = g->nE;
+  v->csc_offsets = g->csc_off;
+  v->csc_srcs = g->csc_src;
+  v->csr_offsets = g->csr_off;
+  v->csr_dsts = g->csr_dst;
+  v->csc_weights = g->csc_w;
+  v->device_bytes = g->device_bytes;
+  return GAS_OK;
+}
+
+int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,
+                       int32_t *edge_index) {
+  if (!g) return set_error(GAS_ERR_ARG, "gas_graph_host_csc: null");
+  if (offsets) std::copy(g->h_csc_off.begin(), g->h_csc_off.end(), offsets);
+  if (srcs) std::copy(g->h_csc_src.begin(), g->h_csc_src.end(), srcs);
+  if (edge_index)
+    std::copy(g->h_csc_eid.begin(), g->h_csc_eid.end(), edge_index);
+  return GAS_OK;
+}
+
+int gas_graph_host_csr(gas_graph_t g, int32_t *offsets, int32_t *dsts,
+                       int32_t *edge_index) {
+  if (!g) return set_error(GAS_ERR_ARG, "gas_graph_host_csr: null");
+  if (offsets) std::copy(g->h_csr_off.begin(), g->h_csr_off.end(), offsets);
+  if (dsts) std::copy(g->h_csr_dst.begin(), g->h_csr_dst.end(), dsts);
+  if (edge_index)
+    std::copy(g->h_csr_eid.begin(), g->h_csr_eid.end(), edge_index);
+  return GAS_OK;
+}
+
+int gas_graph_permute_to_csc(gas_graph_t g, const void *in, void *out,
+                             size_t elem_bytes) {
+  if (!g || (!in && g->nE) || (!out && g->nE) || !elem_bytes)
+    return set_error(GAS_ERR_ARG, "gas_graph_permute_to_csc: bad arguments");
+  const char *src = static_cast<const char *>(in);
+  char *dst = static_cast<char *>(out);
+  for (int32_t k = 0; k < g->nE; ++k)
+    memcpy(dst + (size_t)k * elem_bytes,
+           src + (size_t)g->h_csc_eid[k] * elem_bytes, elem_bytes);
+  return GAS_OK;
+}
+
+int gas_graph_free(gas_graph_t g) {
+  if (!g) return GAS_OK;
+  graph_release(g);
+  delete g;
+  return GAS_OK;
+}
+
+}  // extern "C"
