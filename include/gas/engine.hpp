@@ -175,35 +175,37 @@ class GASEngineGPU {
         1024, std::max<int64_t>(1, (std::max<int64_t>(m_dev.nwords,
                                                       vertexEnd - vertexStart) +
                                     NT - 1) / NT));
-    launch(k_set_active<Program>, grid, m_stream, m_dev, vertexStart,
+    launch(k_set_active<Program>, grid, m_stream, m_dev, m_par, vertexStart,
            vertexEnd);
   }
 
   Int countActive() {
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(Ctl), m_stream),
           "countActive");
-    return m_hctl[0].n_active;
+    return m_hctl[0].n_active[m_par];
   }
 
   void gatherApply(bool haveGather = true) {
     if (haveGather) {
-      launch(k_gscan1<Program>, SCAN_BLOCKS, m_stream, m_dev);
-      launch(k_gscan2<Program>, SCAN_BLOCKS, m_stream, m_dev);
+      launch(k_gscan1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_gscan2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
       timedBegin();
-      launch(k_gather<Program>, MP_BLOCKS, m_stream, m_dev);
+      launch(k_gather<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
       timedEnd(0);
-      launch(k_apply<Program, true>, SCAN_BLOCKS, m_stream, m_dev);
+      launch(k_apply<Program, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     } else {
-      launch(k_apply<Program, false>, SCAN_BLOCKS, m_stream, m_dev);
+      launch(k_apply<Program, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     }
   }
 
   void scatterActivate(bool /*haveScatter*/ = true) {
-    launch(k_sscan2<Program>, SCAN_BLOCKS, m_stream, m_dev);
+    launch(k_sscan2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     timedBegin();
-    launch(k_scatter<Program>, MP_BLOCKS, m_stream, m_dev);
+    launch(k_scatter<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
     timedEnd(1);
-    launch(k_compact<Program>, COMPACT_BLOCKS, m_stream, m_dev);
+    launch(k_compact1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    launch(k_compact2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    m_par ^= 1;
   }
 
   Int nextIter() { return countActive(); }
@@ -220,6 +222,7 @@ class GASEngineGPU {
       gatherApply(haveGather);
       scatterActivate(haveScatter);
       const int slot = j % kRing;
+      m_ring_par[slot] = m_par;
       check(gasrt_memcpy_d2h_async(&m_hctl[slot], m_ctl.p, sizeof(Ctl), m_stream),
             "count readback");
       check(gasrt_event_record(m_ring_ev[slot], m_stream), "event");
@@ -228,7 +231,7 @@ class GASEngineGPU {
         int o = inflight.front();
         inflight.pop_front();
         check(gasrt_event_synchronize(m_ring_ev[o % kRing]), "event sync");
-        if (m_hctl[o % kRing].n_active == 0) break;
+        if (m_hctl[o % kRing].n_active[m_ring_par[o % kRing]] == 0) break;
       }
     }
     check(gasrt_stream_synchronize(m_stream), "sync");
@@ -342,6 +345,9 @@ class GASEngineGPU {
     const size_t tiles = ((size_t)m_nE + V) / MP_NV + 2;
     m_pfirst.alloc(tiles);
     m_plast.alloc(tiles);
+    m_gpart.alloc(tiles);
+    m_apart.alloc(tiles);
+    m_spart.alloc(tiles);
     check(gasrt_memset(m_bm.p, 0, sizeof(uint32_t) * nwords, m_stream),
           "memset");
     check(gasrt_memset(m_rpart.p, 0, sizeof(double) * SCAN_BLOCKS, m_stream),
@@ -361,8 +367,11 @@ class GASEngineGPU {
     d.edata = ed_dev;
     d.vdata = m_vdata.p;
     d.ctl = m_ctl.p;
-    d.q0 = m_q0.p;
-    d.q1 = m_q1.p;
+    d.q[0] = m_q0.p;
+    d.q[1] = m_q1.p;
+    d.gpart = m_gpart.p;
+    d.apart = m_apart.p;
+    d.spart = m_spart.p;
     d.bm = m_bm.p;
     d.nwords = nwords;
     d.goff = m_goff.p;
@@ -373,6 +382,10 @@ class GASEngineGPU {
     d.gtmp = m_gtmp.p;
     d.part_first = m_pfirst.p;
     d.part_last = m_plast.p;
+    m_par = 0;
+    launch(k_static_partition<Program>,
+           (unsigned)std::min<int64_t>(1024, (V + NT - 1) / NT + 1), m_stream,
+           m_dev);
   }
 
   void timedBegin() {
@@ -424,6 +437,9 @@ class GASEngineGPU {
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;
   DevBuf<int32_t> m_q0, m_q1, m_goff, m_soff, m_pdeg, m_bsum;
+  DevBuf<int32_t> m_gpart, m_apart, m_spart;
+  int m_par = 0;  // parity of the next superstep
+  int m_ring_par[kRing] = {};
   DevBuf<uint32_t> m_bm;
   DevBuf<double> m_rpart;
   DevBuf<GatherResult> m_gtmp;

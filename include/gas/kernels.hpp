@@ -4,11 +4,16 @@
 // refgas.h:143-210) runs here as a fixed sequence of kernels whose sizes
 // all come from a device-resident control block, so a superstep can be
 // launched (or captured in a hipGraph) without the host knowing the
-// frontier size:
+// frontier size.  Supersteps alternate between two parities p = 0, 1: the
+// active list of a superstep with parity p lives in queue p and its size in
+// n_active[p]; the superstep builds the next list in queue p^1.  The host
+// passes p as a kernel argument, so no kernel has to publish a superstep's
+// end to the others.
 //
 //   k_gscan1 / k_gscan2  exclusive scan of the active vertices' in-degrees
-//                        (skipped when every vertex is active: the CSC
-//                        offsets already are that scan)
+//                        plus the merge-path tile partition (skipped when
+//                        every vertex is active: the CSC offsets already
+//                        are that scan and the partition is precomputed)
 //   k_gather             merge-path gather over the active vertices' CSC
 //                        in-edges with the segmented reduce fused in; a
 //                        segment that crosses a tile boundary leaves its
@@ -17,13 +22,13 @@
 //                        runs Program::apply, records the predicated
 //                        out-degree of each returning vertex (the
 //                        PredicatedEdgeCountIterator of gpugas.h:387-408)
-//   k_sscan2             exclusive scan of those out-degrees
+//   k_sscan2             exclusive scan of those out-degrees + partition
 //   k_scatter            merge-path walk of the returning vertices' CSR
 //                        out-edges setting frontier bits; in queue mode the
 //                        first setter of a bit also appends the vertex
-//   k_compact            in bitmap mode, compacts the frontier bitmap into
-//                        the next active list (ballot/popcount + LDS scan)
-//                        and ends the superstep
+//   k_compact1 / 2       bitmap mode only: ordered compaction of the
+//                        frontier bitmap into the next active list
+//                        (popcount + LDS block scan, no atomics)
 //
 // Wave size is 64 (CDNA).  No MFMA: there is no dense contraction here.
 #pragma once
@@ -40,8 +45,7 @@ constexpr int WAVE = 64;
 constexpr int SCAN_BLOCKS = 1024;   // chunks of the two-pass scans
 constexpr int MP_VT = 8;            // merge-path items per thread
 constexpr int MP_NV = NT * MP_VT;   // merge-path items per tile
-constexpr int MP_BLOCKS = 2048;     // grid of the merge-path kernels
-constexpr int COMPACT_BLOCKS = 512; // grid of the compaction kernel
+constexpr int MP_BLOCKS = 1024;     // grid of the merge-path kernels
 
 // ---------------------------------------------------------------------------
 // Superstep number visible to vertex programs.  The engine's apply kernel
@@ -58,17 +62,14 @@ __device__ inline int iteration() { return s_superstep; }
 // ---------------------------------------------------------------------------
 // Control block, one per engine, in device memory.
 struct Ctl {
-  int32_t n_active;       // |A| of the current superstep
-  int32_t all_active;     // 1: A is implicitly [0, V)
+  int32_t n_active[2];    // |A| of the superstep of each parity
+  int32_t all_active[2];  // 1: A is implicitly [0, V)
   int32_t iteration;      // supersteps completed (BFS level)
-  int32_t qsel;           // which queue holds A
   int32_t gather_items;   // sum of in-degrees over A
   int32_t scatter_items;  // sum of predicated out-degrees over A
-  int32_t n_next;         // size of the next active list
   int32_t next_dense;     // 1: next frontier via bitmap + compaction
-  int32_t ticket;         // last-block election in k_compact
   int32_t dense_min;      // scatter_items at or above which bitmap mode runs
-  int32_t pad0, pad1;
+  int32_t pad[3];
   int64_t stat_gathered;  // totals over the run
   int64_t stat_scattered;
   int64_t stat_active;
@@ -189,13 +190,16 @@ struct Dev {
   const ED *edata;  // CSC order
   VD *vdata;
   Ctl *ctl;
-  int32_t *q0, *q1;
+  int32_t *q[2];    // active lists by parity
   uint32_t *bm;     // frontier bitmap, (V+31)/32 words
   int32_t nwords;
   int32_t *goff;    // scan of in-degrees over A (V+1)
   int32_t *soff;    // scan of predicated out-degrees over A (V+1)
   int32_t *pdeg;    // predicated out-degree per active slot
   int32_t *bsum;    // per-chunk sums of the two-pass scans
+  int32_t *gpart;   // first segment of every gather tile (queue mode)
+  int32_t *apart;   // same for the all-active gather (fixed per graph)
+  int32_t *spart;   // same for the scatter
   double *rpart;    // per-chunk FP64 residual sums
   GR *gtmp;         // gather result per active slot
   Acc<P> *part_first, *part_last;  // per-tile partials of crossing segments
@@ -268,21 +272,6 @@ __device__ inline double block_sum(double x, double *lds) {
   return t;
 }
 
-// Number of segment ends strictly before merge-path position d, where
-// segment i (0 <= i < A) occupies positions off[i]+i .. off[i+1]+i and its
-// end item sits at off[i+1]+i.
-__device__ inline int mp_search(const int32_t *off, int A, int d) {
-  int lo = 0, hi = A;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (off[mid + 1] + mid >= d)
-      hi = mid;
-    else
-      lo = mid + 1;
-  }
-  return lo;
-}
-
 __device__ inline int lds_lower_bound(const int32_t *a, int n, int x) {
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -295,58 +284,11 @@ __device__ inline int lds_lower_bound(const int32_t *a, int n, int x) {
   return lo;
 }
 
-__device__ inline int ld_relaxed(const int32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Chunk [b, e) of n items handled by workgroup `blk` in the two-pass scans.
 __device__ inline void scan_chunk(int n, int blk, int &b, int &e) {
   int c = (n + SCAN_BLOCKS - 1) / SCAN_BLOCKS;
   b = min(n, blk * c);
   e = min(n, b + c);
-}
-
-// ---------------------------------------------------------------------------
-// Frontier initialisation: setActive(start, end) (gpugas.h:342-349) plus
-// clearing the frontier bitmap.
-template <class P>
-__global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int start,
-                                                   int end) {
-  Ctl *c = d.ctl;
-  const bool all = (start == 0 && end == d.nV);
-  const int n = end - start;
-  int32_t *q = c->qsel ? d.q1 : d.q0;
-  const int stride = gridDim.x * NT;
-  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nwords; i += stride)
-    d.bm[i] = 0u;
-  if (!all)
-    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += stride)
-      q[i] = start + i;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->n_active = n;
-    c->all_active = all ? 1 : 0;
-    c->n_next = 0;
-    c->ticket = 0;
-  }
-}
-
-// Pass 1 of the in-degree scan over the active list.
-template <class P>
-__global__ __launch_bounds__(NT) void k_gscan1(Dev<P> d) {
-  __shared__ int lds[NT / WAVE];
-  const Ctl *c = d.ctl;
-  const int A = c->n_active;
-  if (A == 0 || c->all_active) return;
-  const int32_t *q = c->qsel ? d.q1 : d.q0;
-  int b, e;
-  scan_chunk(A, blockIdx.x, b, e);
-  int sum = 0;
-  for (int i = b + threadIdx.x; i < e; i += NT) {
-    int v = q[i];
-    sum += d.csc_off[v + 1] - d.csc_off[v];
-  }
-  sum = block_sum(sum, lds);
-  if (threadIdx.x == 0) d.bsum[blockIdx.x] = sum;
 }
 
 // Prefix of the chunk sums before chunk `blk`, and the grand total.
@@ -362,21 +304,89 @@ __device__ inline void chunk_prefix(const int32_t *bsum, int blk, int &pre,
   total = block_sum(t, lds);
 }
 
-// Pass 2 of the in-degree scan: goff[i] = sum of in-degrees of A[0..i).
+// Merge-path layout: segment i (0 <= i < A) occupies the positions
+// off[i]+i .. off[i+1]+i -- its edges, then its end item.  Tile t covers
+// positions [t*MP_NV, (t+1)*MP_NV); part[t] is the segment holding the
+// tile's first position and part[ntiles] = A.  The writer of segment i
+// records itself for every tile that starts inside it.
+__device__ inline void mark_tiles(int32_t *part, int i, int off_i, int deg) {
+  const int start = off_i + i;
+  const int end = off_i + deg + i;
+  for (int t = (start + MP_NV - 1) / MP_NV; t * MP_NV <= end; ++t)
+    part[t] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Frontier initialisation: setActive(start, end) (gpugas.h:342-349) plus
+// clearing the frontier bitmap.
 template <class P>
-__global__ __launch_bounds__(NT) void k_gscan2(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
+                                                   int end) {
+  Ctl *c = d.ctl;
+  const bool all = (start == 0 && end == d.nV);
+  const int n = end - start;
+  int32_t *q = d.q[p];
+  const int stride = gridDim.x * NT;
+  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nwords; i += stride)
+    d.bm[i] = 0u;
+  if (!all)
+    for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += stride)
+      q[i] = start + i;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->n_active[p] = n;
+    c->all_active[p] = all ? 1 : 0;
+  }
+}
+
+// Tile partition of the all-active gather (depends on the graph only).
+template <class P>
+__global__ __launch_bounds__(NT) void k_static_partition(Dev<P> d) {
+  const int V = d.nV;
+  for (int i = blockIdx.x * NT + threadIdx.x; i < V; i += gridDim.x * NT) {
+    int o = d.csc_off[i];
+    mark_tiles(d.apart, i, o, d.csc_off[i + 1] - o);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int N = d.nE + V;
+    d.apart[(N + MP_NV - 1) / MP_NV] = V;
+  }
+}
+
+// Pass 1 of the in-degree scan over the active list.
+template <class P>
+__global__ __launch_bounds__(NT) void k_gscan1(Dev<P> d, int p) {
+  __shared__ int lds[NT / WAVE];
+  const Ctl *c = d.ctl;
+  const int A = c->n_active[p];
+  if (A == 0 || c->all_active[p]) return;
+  const int32_t *q = d.q[p];
+  int b, e;
+  scan_chunk(A, blockIdx.x, b, e);
+  int sum = 0;
+  for (int i = b + threadIdx.x; i < e; i += NT) {
+    int v = q[i];
+    sum += d.csc_off[v + 1] - d.csc_off[v];
+  }
+  sum = block_sum(sum, lds);
+  if (threadIdx.x == 0) d.bsum[blockIdx.x] = sum;
+}
+
+// Pass 2 of the in-degree scan: goff[i] = sum of in-degrees of A[0..i),
+// and the gather tile partition.
+template <class P>
+__global__ __launch_bounds__(NT) void k_gscan2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
-  const int A = c->n_active;
+  const int A = c->n_active[p];
   if (A == 0) return;
-  if (c->all_active) {
+  if (c->all_active[p]) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       c->gather_items = d.nE;
       c->stat_gathered += d.nE;
     }
     return;
   }
-  const int32_t *q = c->qsel ? d.q1 : d.q0;
+  const int32_t *q = d.q[p];
   int b, e, pre, total;
   scan_chunk(A, blockIdx.x, b, e);
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
@@ -390,11 +400,15 @@ __global__ __launch_bounds__(NT) void k_gscan2(Dev<P> d) {
     }
     int tot;
     int ex = block_excl_scan(deg, tot, lds);
-    if (i < e) d.goff[i] = carry + ex;
+    if (i < e) {
+      d.goff[i] = carry + ex;
+      mark_tiles(d.gpart, i, carry + ex, deg);
+    }
     carry += tot;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     d.goff[A] = total;
+    d.gpart[(total + A + MP_NV - 1) / MP_NV] = A;
     c->gather_items = total;
     c->stat_gathered += total;
   }
@@ -410,20 +424,20 @@ __global__ __launch_bounds__(NT) void k_gscan2(Dev<P> d) {
 // end (part_first) and after its last end (part_last), and k_apply folds
 // them.
 template <class P>
-__global__ __launch_bounds__(NT) void k_gather(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_gather(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   __shared__ int32_t s_end[MP_NV + 1];
   __shared__ int32_t s_base[MP_NV + 1];
   __shared__ int32_t s_vtx[MP_NV + 1];
-  __shared__ int32_t s_meta[2];
   __shared__ SegItem<P> s_seg[NT];
 
   const Ctl *c = d.ctl;
-  const int A = c->n_active;
+  const int A = c->n_active[p];
   if (A == 0) return;
-  const bool all = c->all_active != 0;
-  const int32_t *q = c->qsel ? d.q1 : d.q0;
+  const bool all = c->all_active[p] != 0;
+  const int32_t *q = d.q[p];
   const int32_t *off = all ? d.csc_off : d.goff;
+  const int32_t *part = all ? d.apart : d.gpart;
   const int N = c->gather_items + A;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
@@ -431,11 +445,8 @@ __global__ __launch_bounds__(NT) void k_gather(Dev<P> d) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
     const int d1 = min(d0 + MP_NV, N);
-    if (tid == 0) s_meta[0] = mp_search(off, A, d0);
-    if (tid == 1) s_meta[1] = mp_search(off, A, d1);
-    __syncthreads();
-    const int s0 = s_meta[0];
-    const int s1 = s_meta[1];
+    const int s0 = part[t];
+    const int s1 = part[t + 1];
     const int nseg = min(s1, A - 1) - s0 + 1;
     for (int k = tid; k < nseg; k += NT) {
       int s = s0 + k;
@@ -454,18 +465,18 @@ __global__ __launch_bounds__(NT) void k_gather(Dev<P> d) {
     int kind[MP_VT], kk[MP_VT], eidx[MP_VT];
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
-      const int p = p0 + it;
+      const int pos = p0 + it;
       kind[it] = 0;
       kk[it] = 0;
       eidx[it] = 0;
-      if (p < d1) {
+      if (pos < d1) {
         kk[it] = k;
-        if (s_end[k] == p) {
+        if (s_end[k] == pos) {
           kind[it] = 2;
           ++k;
         } else {
           kind[it] = 1;
-          eidx[it] = s_base[k] + (p - s0 - k);
+          eidx[it] = s_base[k] + (pos - s0 - k);
         }
       }
     }
@@ -561,20 +572,28 @@ struct has_residual<P, std::void_t<decltype(P::residual(
     : std::true_type {};
 
 // Apply (kApply, gpugas_kernels.cuh:139-151) fused with the fold of
-// crossing segments and pass 1 of the scatter scan.
+// crossing segments and pass 1 of the scatter scan.  Also resets the next
+// parity's list size, which k_scatter / k_compact2 fill.
 template <class P, bool HaveGather>
-__global__ __launch_bounds__(NT) void k_apply(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   typedef typename P::VertexData VD;
   __shared__ int lds[NT / WAVE];
   __shared__ double ldsd[NT / WAVE];
-  const Ctl *c = d.ctl;
-  const int A = c->n_active;
+  Ctl *c = d.ctl;
+  const int A = c->n_active[p];
+  // The next parity starts empty even when this superstep is: otherwise a
+  // superstep enqueued after the frontier emptied would re-run the list
+  // left in the other queue two supersteps ago.
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->n_active[p ^ 1] = 0;
+    c->all_active[p ^ 1] = 0;
+  }
   if (A == 0) return;
   if (threadIdx.x == 0) s_superstep = c->iteration;
   __syncthreads();
-  const bool all = c->all_active != 0;
-  const int32_t *q = c->qsel ? d.q1 : d.q0;
+  const bool all = c->all_active[p] != 0;
+  const int32_t *q = d.q[p];
   const int32_t *off = all ? d.csc_off : d.goff;
   int b, e;
   scan_chunk(A, blockIdx.x, b, e);
@@ -608,13 +627,14 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d) {
   }
 }
 
-// Pass 2 of the scatter scan; also picks bitmap or queue mode for the next
-// frontier from the number of edges to scatter.
+// Pass 2 of the scatter scan and the scatter tile partition; picks bitmap
+// or queue mode for the next frontier from the number of edges to scatter,
+// and counts the superstep (all applies of this superstep are done).
 template <class P>
-__global__ __launch_bounds__(NT) void k_sscan2(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_sscan2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
-  const int A = c->n_active;
+  const int A = c->n_active[p];
   if (A == 0) return;
   int b, e, pre, total;
   scan_chunk(A, blockIdx.x, b, e);
@@ -625,36 +645,46 @@ __global__ __launch_bounds__(NT) void k_sscan2(Dev<P> d) {
     int x = (i < e) ? d.pdeg[i] : 0;
     int tot;
     int ex = block_excl_scan(x, tot, lds);
-    if (i < e) d.soff[i] = carry + ex;
+    if (i < e) {
+      d.soff[i] = carry + ex;
+      mark_tiles(d.spart, i, carry + ex, x);
+    }
     carry += tot;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     d.soff[A] = total;
+    d.spart[(total + A + MP_NV - 1) / MP_NV] = A;
     c->scatter_items = total;
     c->next_dense = total >= c->dense_min ? 1 : 0;
     c->stat_scattered += total;
+    c->stat_active += A;
+    c->stat_supersteps += 1;
+    c->iteration += 1;
   }
 }
 
 // Scatter/activate (north-star (d); replaces IntervalGather into
 // ActivateOutputIterator / ActivateOutputIteratorSmallSize,
 // gpugas.h:545-597, 693-730).  Bitmap mode: fire-and-forget atomicOr.
-// Queue mode: the lane whose atomicOr flips a bit appends the vertex, one
-// atomicAdd per wave (ballot + mbcnt).
+// Queue mode: the lane whose atomicOr flips a bit appends the vertex; a
+// tile collects its new vertices in LDS (ballot + mbcnt per wave) and
+// reserves space in the next list with a single atomicAdd.
 template <class P>
-__global__ __launch_bounds__(NT) void k_scatter(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_scatter(Dev<P> d, int p) {
   __shared__ int32_t s_end[MP_NV + 1];
   __shared__ int32_t s_base[MP_NV + 1];
-  __shared__ int32_t s_meta[2];
+  __shared__ int32_t s_fresh[MP_NV];
+  __shared__ int s_cnt, s_gbase;
   Ctl *c = d.ctl;
-  const int A = c->n_active;
+  const int A = c->n_active[p];
   if (A == 0) return;
   const int ES = c->scatter_items;
   if (ES == 0) return;
-  const bool all = c->all_active != 0;
+  const bool all = c->all_active[p] != 0;
   const bool dense = c->next_dense != 0;
-  const int32_t *q = c->qsel ? d.q1 : d.q0;
-  int32_t *qn = c->qsel ? d.q0 : d.q1;
+  const int32_t *q = d.q[p];
+  int32_t *qn = d.q[p ^ 1];
+  int32_t *nn = &c->n_active[p ^ 1];
   const int32_t *off = d.soff;
   const int N = ES + A;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
@@ -664,11 +694,8 @@ __global__ __launch_bounds__(NT) void k_scatter(Dev<P> d) {
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
     const int d1 = min(d0 + MP_NV, N);
-    if (tid == 0) s_meta[0] = mp_search(off, A, d0);
-    if (tid == 1) s_meta[1] = mp_search(off, A, d1);
-    __syncthreads();
-    const int s0 = s_meta[0];
-    const int s1 = s_meta[1];
+    const int s0 = d.spart[t];
+    const int s1 = d.spart[t + 1];
     const int nseg = min(s1, A - 1) - s0 + 1;
     for (int k = tid; k < nseg; k += NT) {
       int s = s0 + k;
@@ -683,15 +710,15 @@ __global__ __launch_bounds__(NT) void k_scatter(Dev<P> d) {
     bool isedge[MP_VT];
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
-      const int p = p0 + it;
+      const int pos = p0 + it;
       isedge[it] = false;
       eidx[it] = 0;
-      if (p < d1) {
-        if (s_end[k] == p) {
+      if (pos < d1) {
+        if (s_end[k] == pos) {
           ++k;
         } else {
           isedge[it] = true;
-          eidx[it] = s_base[k] + (p - s0 - k);
+          eidx[it] = s_base[k] + (pos - s0 - k);
         }
       }
     }
@@ -704,6 +731,10 @@ __global__ __launch_bounds__(NT) void k_scatter(Dev<P> d) {
       for (int it = 0; it < MP_VT; ++it)
         if (isedge[it]) atomicOr(&d.bm[w[it] >> 5], 1u << (w[it] & 31));
     } else {
+      // stage this tile's newly activated vertices in LDS, then reserve
+      // their slots in the next list with one atomicAdd per tile
+      if (tid == 0) s_cnt = 0;
+      __syncthreads();
 #pragma unroll
       for (int it = 0; it < MP_VT; ++it) {
         bool fresh = false;
@@ -716,77 +747,71 @@ __global__ __launch_bounds__(NT) void k_scatter(Dev<P> d) {
         if (m) {
           int leader = __ffsll((unsigned long long)m) - 1;
           int base = 0;
-          if (lane == leader) base = atomicAdd(&c->n_next, __popcll(m));
+          if (lane == leader) base = atomicAdd(&s_cnt, __popcll(m));
           base = __shfl(base, leader, WAVE);
           if (fresh) {
             int rank = __builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            qn[base + rank] = w[it];
+            s_fresh[base + rank] = w[it];
           }
         }
       }
+      __syncthreads();
+      const int cnt = s_cnt;
+      if (tid == 0 && cnt) s_gbase = atomicAdd(nn, cnt);
+      __syncthreads();
+      for (int i = tid; i < cnt; i += NT) qn[s_gbase + i] = s_fresh[i];
     }
     __syncthreads();
   }
 }
 
-// Frontier compaction (north-star (e); replaces scatter_if_inputloc_twophase,
-// scatter_if_mgpu.h:130-157) and end of superstep.  Bitmap mode: each
-// workgroup turns NT words per round into vertex ids, ascending within the
-// round (wave popcount + LDS block scan, one atomicAdd per round), and
-// clears the words.  The last workgroup to finish publishes the next
-// active list.
+// Frontier compaction, bitmap mode (north-star (e); replaces
+// scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157): pass 1 counts
+// set bits per chunk of words, pass 2 writes the ids of every chunk in
+// ascending order at its prefix and clears the words.
+__device__ inline bool compact_dense(const Ctl *c, int p) {
+  return c->n_active[p] > 0 && c->next_dense != 0 && c->scatter_items > 0;
+}
+
 template <class P>
-__global__ __launch_bounds__(NT) void k_compact(Dev<P> d) {
+__global__ __launch_bounds__(NT) void k_compact1(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  __shared__ int s_base;
-  __shared__ int s_last;
+  if (!compact_dense(d.ctl, p)) return;
+  int b, e;
+  scan_chunk(d.nwords, blockIdx.x, b, e);
+  int cnt = 0;
+  for (int i = b + threadIdx.x; i < e; i += NT) cnt += __popc(d.bm[i]);
+  cnt = block_sum(cnt, lds);
+  if (threadIdx.x == 0) d.bsum[blockIdx.x] = cnt;
+}
+
+template <class P>
+__global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
+  __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
-  const int A = c->n_active;
-  if (A == 0) return;  // uniform: no superstep ran
-  const bool dense = c->next_dense != 0 && c->scatter_items > 0;
-  int32_t *qn = c->qsel ? d.q0 : d.q1;
-  if (dense) {
-    for (int base = blockIdx.x * NT; base < d.nwords;
-         base += gridDim.x * NT) {
-      int wi = base + threadIdx.x;
-      uint32_t word = (wi < d.nwords) ? d.bm[wi] : 0u;
-      int cnt = __popc(word);
-      int tot;
-      int ex = block_excl_scan(cnt, tot, lds);
-      if (tot) {
-        if (threadIdx.x == 0) s_base = atomicAdd(&c->n_next, tot);
-        __syncthreads();
-        int pos = s_base + ex;
-        while (word) {
-          int bit = __ffs(word) - 1;
-          qn[pos++] = wi * 32 + bit;
-          word &= word - 1;
-        }
-        if (wi < d.nwords) d.bm[wi] = 0u;
-        __syncthreads();
+  if (!compact_dense(c, p)) return;
+  int32_t *qn = d.q[p ^ 1];
+  int b, e, pre, total;
+  scan_chunk(d.nwords, blockIdx.x, b, e);
+  chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
+  int carry = pre;
+  for (int base = b; base < e; base += NT) {
+    int wi = base + threadIdx.x;
+    uint32_t word = (wi < e) ? d.bm[wi] : 0u;
+    int tot;
+    int pos = carry + block_excl_scan(__popc(word), tot, lds);
+    if (word) {
+      d.bm[wi] = 0u;
+      while (word) {
+        int bit = __ffs(word) - 1;
+        qn[pos++] = wi * 32 + bit;
+        word &= word - 1;
       }
     }
+    carry += tot;
   }
-  // last-block election
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0)
-    s_last = (atomicAdd(&c->ticket, 1) == (int)gridDim.x - 1) ? 1 : 0;
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    int nn = ld_relaxed(&c->n_next);
-    c->stat_active += A;
-    c->stat_supersteps += 1;
-    c->n_active = nn;
-    c->n_next = 0;
-    c->qsel ^= 1;
-    c->all_active = 0;
-    c->iteration += 1;
-    c->ticket = 0;
-    c->scatter_items = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) c->n_active[p ^ 1] = total;
 }
 
 // Superstep counter write (host-side setIteration).
