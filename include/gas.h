@@ -180,6 +180,19 @@ typedef struct {
  * (adds host synchronisation; used for per-kernel roofline figures). */
 int gas_set_kernel_timing(int enable);
 
+/* Degree tiers of the engine the whole-run entry points keep for (graph,
+ * program): tiers[0..1] for the gather, tiers[2..3] for the scatter.  A
+ * vertex of degree d is walked by one thread if d <= t1, by one wave64 if
+ * t1 < d <= t2, and split into merge-path tiles above t2 (t1 <= 16).
+ * No counterpart in the reference, whose gather is one merge path
+ * (gpugas.h:421-513). */
+int gas_graph_set_tiers(gas_graph_t g, int program, const int32_t *tiers);
+int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers);
+/* Autotuner: times candidate tiers on this graph for the program (gather
+ * and scatter phases separately, vertex data untouched) and keeps the
+ * fastest; tiers_out (may be NULL) receives them. */
+int gas_graph_autotune(gas_graph_t g, int program, int32_t *tiers_out);
+
 /* ---- whole runs (app drivers) ------------------------------------------- */
 /* BFS (bfs.cu:31-112): depth[v] = superstep at which v is first reached,
  * -1 if unreached.  stats may be NULL. */

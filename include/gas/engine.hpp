@@ -19,8 +19,9 @@
 //  * sizes stay on the device between kernels; the host reads the active
 //    count only in countActive()/nextIter() and, in run(), a few supersteps
 //    behind the device;
-//  * the next active list is unordered in queue mode (every program's
-//    result is independent of the order of the active list);
+//  * run() executes apply and the activation of out-neighbours in one
+//    kernel (both are per active vertex and nothing reads the activation
+//    flags before the superstep ends);
 //  * like the reference GPU engine, Program::scatter is not invoked
 //    (gpugas.h:666-668).
 // Errors throw gas::Error carrying gas_last_error().
@@ -28,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -84,6 +86,11 @@ struct DevBuf {
           "gasrt_malloc");
     n = count;
   }
+  void upload(const std::vector<T> &h, gasrt_stream_t s) {
+    alloc(h.size());
+    if (!h.empty())
+      check(gasrt_memcpy_h2d(p, h.data(), sizeof(T) * h.size(), s), "upload");
+  }
   void release() {
     if (p) gasrt_free(p);
     p = nullptr;
@@ -91,6 +98,22 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
 };
+
+// Merge-path tile partition of a segment list given its exclusive scan
+// (host version of mark_tiles, for lists fixed per graph).
+inline std::vector<int32_t> host_tile_partition(const std::vector<int32_t> &off) {
+  const int64_t n = (int64_t)off.size() - 1;
+  const int64_t N = (n > 0 ? off[n] : 0) + n;
+  const int64_t ntiles = (N + MP_NV - 1) / MP_NV;
+  std::vector<int32_t> part((size_t)ntiles + 1, 0);
+  for (int64_t h = 0; h < n; ++h) {
+    const int64_t start = off[h] + h, end = off[h + 1] + h;
+    for (int64_t t = (start + MP_NV - 1) / MP_NV; t * MP_NV <= end; ++t)
+      part[t] = (int32_t)h;
+  }
+  part[ntiles] = (int32_t)n;
+  return part;
+}
 
 template <class Program, class Int = int32_t>
 class GASEngineGPU {
@@ -160,6 +183,10 @@ class GASEngineGPU {
             "getResults");
   }
 
+  // Forgets the caller's vertex-data array (getResults becomes a no-op
+  // until resetVertexData binds a new one).
+  void detachHost() { m_vd_host = nullptr; }
+
   // Uploads the caller's vertex data again (reuse of a bound graph).
   void resetVertexData(VertexData *vertexData) {
     m_vd_host = vertexData;
@@ -171,10 +198,10 @@ class GASEngineGPU {
   void setActive(Int vertexStart, Int vertexEnd) {
     if (vertexStart < 0 || vertexEnd > m_nV || vertexEnd < vertexStart)
       throw Error(GAS_ERR_ARG, "setActive: range outside [0, nVertices)");
-    unsigned grid = (unsigned)std::min<int64_t>(
-        1024, std::max<int64_t>(1, (std::max<int64_t>(m_dev.nwords,
-                                                      vertexEnd - vertexStart) +
-                                    NT - 1) / NT));
+    const int64_t work =
+        std::max<int64_t>(m_dev.nfwords, (int64_t)vertexEnd - vertexStart);
+    const unsigned grid =
+        (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (work + NT - 1) / NT));
     launch(k_set_active<Program>, grid, m_stream, m_dev, m_par, vertexStart,
            vertexEnd);
   }
@@ -187,43 +214,49 @@ class GASEngineGPU {
 
   void gatherApply(bool haveGather = true) {
     if (haveGather) {
-      launch(k_gscan1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_gscan2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      timedBegin();
-      launch(k_gather<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
-      timedEnd(0);
-      launch(k_apply<Program, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      gatherPhase();
+      launch(k_apply<Program, true, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     } else {
-      launch(k_apply<Program, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_apply<Program, false, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     }
   }
 
   void scatterActivate(bool /*haveScatter*/ = true) {
-    launch(k_sscan2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     timedBegin();
-    launch(k_scatter<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
+    launch(k_scatter_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     timedEnd(1);
-    launch(k_compact1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-    launch(k_compact2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-    m_par ^= 1;
+    endSuperstep();
   }
 
   Int nextIter() { return countActive(); }
 
   void run() { runUntilEmpty(true, true); }
 
+  // One whole superstep: gatherApply + scatterActivate, with the thread and
+  // wave tiers of the activation fused into the apply kernel.
+  void superstep(bool haveGather) {
+    if (haveGather) gatherPhase();
+    timedBegin();
+    if (haveGather)
+      launch(k_apply<Program, true, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    else
+      launch(k_apply<Program, false, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    timedEnd(1);
+    endSuperstep();
+  }
+
   // Supersteps until the frontier is empty, checking the device count a
   // few supersteps behind so the host never idles the GPU.
-  void runUntilEmpty(bool haveGather, bool haveScatter) {
+  void runUntilEmpty(bool haveGather, bool /*haveScatter*/) {
     if (countActive() == 0) return;
     std::deque<int> inflight;
     int j = 0;
     for (;;) {
-      gatherApply(haveGather);
-      scatterActivate(haveScatter);
+      superstep(haveGather);
       const int slot = j % kRing;
       m_ring_par[slot] = m_par;
-      check(gasrt_memcpy_d2h_async(&m_hctl[slot], m_ctl.p, sizeof(Ctl), m_stream),
+      check(gasrt_memcpy_d2h_async(&m_hctl[slot], m_ctl.p, sizeof(Ctl),
+                                   m_stream),
             "count readback");
       check(gasrt_event_record(m_ring_ev[slot], m_stream), "event");
       inflight.push_back(j++);
@@ -241,11 +274,86 @@ class GASEngineGPU {
     launch(k_set_iteration<Program>, 1, m_stream, m_dev, it);
   }
 
-  void setDenseThreshold(int32_t edges) {
-    m_dense_min = edges;
-    Ctl c = readCtl();
-    c.dense_min = edges;
-    check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
+  // Degree tiers of the gather (g) and the scatter (s):
+  // thread <= t1 < wave <= t2 < merge-path hub.
+  void setThresholds(int gt1, int gt2, int st1, int st2) {
+    if (gt1 < 0 || gt1 > T1_MAX || gt2 < gt1 || st1 < 0 || st1 > T1_MAX ||
+        st2 < st1)
+      throw Error(GAS_ERR_ARG,
+                  "setThresholds: need 0 <= t1 <= 16 and t1 <= t2");
+    m_gt1 = gt1;
+    m_gt2 = gt2;
+    m_st1 = st1;
+    m_st2 = st2;
+    if (m_graph) buildHubs();
+  }
+  void thresholds(int *t) const {
+    t[0] = m_gt1;
+    t[1] = m_gt2;
+    t[2] = m_st1;
+    t[3] = m_st2;
+  }
+
+  // Autotuner: times the all-active gather phase and an all-returning
+  // scatter phase (neither changes vertex data) for candidate tiers on
+  // this graph and keeps the fastest.  Leaves the frontier empty.
+  void autotune(bool tuneGather, bool tuneScatter, int reps = 3) {
+    static const int kT1[] = {4, 8, 16};
+    static const int kT2[] = {0, 32, 128, 512, 2048, 1 << 30};
+    gasrt_event_t a, b;
+    check(gasrt_event_create(&a), "event");
+    check(gasrt_event_create(&b), "event");
+    auto time_phase = [&](bool gather) {
+      float best = 1e30f;
+      for (int r = 0; r < reps; ++r) {
+        setActive(0, m_nV);
+        if (!gather)
+          check(gasrt_memset(m_rflag.p, 1, (size_t)m_nV, m_stream), "memset");
+        check(gasrt_event_record(a, m_stream), "event");
+        if (gather) {
+          gatherPhase();
+        } else {
+          launch(k_scatter_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+          endSuperstep();
+        }
+        check(gasrt_event_record(b, m_stream), "event");
+        check(gasrt_event_synchronize(b), "event");
+        float ms = 0;
+        check(gasrt_event_elapsed_ms(a, b, &ms), "elapsed");
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    auto sweep = [&](bool gather) {
+      int bt1 = gather ? m_gt1 : m_st1, bt2 = gather ? m_gt2 : m_st2;
+      float best = 1e30f;
+      const int maxdeg = gather ? m_max_in : m_max_out;
+      for (int t1 : kT1)
+        for (int t2r : kT2) {
+          const int t2 = std::max(t1, std::min(t2r, std::max(maxdeg, t1)));
+          if (gather)
+            setThresholds(t1, t2, m_st1, m_st2);
+          else
+            setThresholds(m_gt1, m_gt2, t1, t2);
+          const float ms = time_phase(gather);
+          if (ms < best) {
+            best = ms;
+            bt1 = t1;
+            bt2 = t2;
+          }
+        }
+      if (gather)
+        setThresholds(bt1, bt2, m_st1, m_st2);
+      else
+        setThresholds(m_gt1, m_gt2, bt1, bt2);
+    };
+    if (tuneGather) sweep(true);
+    if (tuneScatter) sweep(false);
+    gasrt_event_destroy(a);
+    gasrt_event_destroy(b);
+    setActive(0, 0);
+    setIteration(0);
+    resetStats();
   }
 
   Ctl readCtl() {
@@ -261,18 +369,6 @@ class GASEngineGPU {
     check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
     m_gather_ms = m_scatter_ms = 0;
     m_gather_launches = m_scatter_launches = 0;
-  }
-
-  // Sum of per-block FP64 residuals of the last apply (programs with a
-  // residual hook).
-  double lastResidual() {
-    std::vector<double> r(SCAN_BLOCKS);
-    check(gasrt_memcpy_d2h(r.data(), m_rpart.p, sizeof(double) * r.size(),
-                           m_stream),
-          "residual");
-    double s = 0;
-    for (double x : r) s += x;
-    return s;
   }
 
   void stats(gas_run_stats *st) {
@@ -292,12 +388,69 @@ class GASEngineGPU {
   gasrt_stream_t stream() const { return m_stream; }
   Int numVertices() const { return m_nV; }
   Int numEdges() const { return m_nE; }
+  int maxInDegree() const { return m_max_in; }
+  int maxOutDegree() const { return m_max_out; }
   VertexData *deviceVertexData() { return m_vdata.p; }
   const Dev<Program> &dev() const { return m_dev; }
 
  private:
   static constexpr int kRing = 8;
   static constexpr int kDepth = 3;
+
+  bool inHubs() const { return m_max_in > m_gt2; }
+  bool outHubs() const { return m_max_out > m_st2; }
+
+  void gatherPhase() {
+    timedBegin();
+    launch(k_gather_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    if (inHubs()) {
+      launch(k_hscan1<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_hscan2<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_hgather<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_hfix<Program>, 64, m_stream, m_dev, m_par);
+    }
+    timedEnd(0);
+  }
+
+  // Hub scatter, compaction of the activation flags, next parity.
+  void endSuperstep() {
+    if (outHubs()) {
+      timedBegin();
+      launch(k_hscan1<Program, 1>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_hscan2<Program, 1>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      launch(k_hscatter<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
+      timedEnd(1);
+    }
+    launch(k_compact1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    launch(k_compact2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    m_par ^= 1;
+  }
+
+  // Gather hubs of an all-active superstep: vertices with in-degree > t2,
+  // their in-degree scan and tile partition (fixed for the graph).
+  void buildHubs() {
+    std::vector<int32_t> hubs, hoff(1, 0);
+    for (int32_t v = 0; v < m_nV; ++v) {
+      const int32_t deg = m_h_csc_off[v + 1] - m_h_csc_off[v];
+      if (deg > m_gt2) {
+        hubs.push_back(v);
+        hoff.push_back(hoff.back() + deg);
+      }
+    }
+    std::vector<int32_t> part = host_tile_partition(hoff);
+    m_ahub.upload(hubs, m_stream);
+    m_ahoff.upload(hoff, m_stream);
+    m_ahpart.upload(part, m_stream);
+    m_dev.gt1 = m_gt1;
+    m_dev.gt2 = m_gt2;
+    m_dev.st1 = m_st1;
+    m_dev.st2 = m_st2;
+    m_dev.ahub = m_ahub.p;
+    m_dev.ahoff = m_ahoff.p;
+    m_dev.ahpart = m_ahpart.p;
+    m_dev.n_ahub = (int32_t)hubs.size();
+    m_dev.ahub_items = hoff.back();
+  }
 
   void bindGraph(gas_graph_t g, VertexData *vertexData,
                  const EdgeData *edgeData) {
@@ -319,10 +472,7 @@ class GASEngineGPU {
         check(gas_graph_permute_to_csc(g, edgeData, sorted.data(),
                                        sizeof(EdgeData)),
               "permute edge data");
-        m_edata.alloc((size_t)m_nE);
-        check(gasrt_memcpy_h2d(m_edata.p, sorted.data(),
-                               sizeof(EdgeData) * (size_t)m_nE, m_stream),
-              "edge data");
+        m_edata.upload(sorted, m_stream);
         ed_dev = m_edata.p;
       } else if (sizeof(EdgeData) == 4 && gv.csc_weights) {
         ed_dev = reinterpret_cast<const EdgeData *>(gv.csc_weights);
@@ -330,31 +480,45 @@ class GASEngineGPU {
         throw Error(GAS_ERR_ARG, "setGraph: program needs edge data");
       }
     }
+    // degree extremes decide which tiers can occur
+    m_h_csc_off.assign((size_t)m_nV + 1, 0);
+    std::vector<int32_t> csr_off((size_t)m_nV + 1, 0);
+    check(gas_graph_host_csc(g, m_h_csc_off.data(), nullptr, nullptr), "csc");
+    check(gas_graph_host_csr(g, csr_off.data(), nullptr, nullptr), "csr");
+    m_max_in = m_max_out = 0;
+    for (int32_t v = 0; v < m_nV; ++v) {
+      m_max_in = std::max(m_max_in, m_h_csc_off[v + 1] - m_h_csc_off[v]);
+      m_max_out = std::max(m_max_out, csr_off[v + 1] - csr_off[v]);
+    }
+
     const size_t V = (size_t)m_nV;
-    const int nwords = (int)((V + 31) / 32);
+    const int nfwords = (int)((V + 3) / 4);
+    const size_t tiles = ((size_t)m_nE + V) / MP_NV + 2;
     m_ctl.alloc(1);
     m_q0.alloc(V);
     m_q1.alloc(V);
-    m_bm.alloc((size_t)nwords);
-    m_goff.alloc(V + 1);
-    m_soff.alloc(V + 1);
-    m_pdeg.alloc(V);
-    m_bsum.alloc(SCAN_BLOCKS);
-    m_rpart.alloc(SCAN_BLOCKS);
+    m_flags.alloc((size_t)nfwords * 4);
+    m_rflag.alloc(V);
     m_gtmp.alloc(V);
-    const size_t tiles = ((size_t)m_nE + V) / MP_NV + 2;
+    m_ghub.alloc(V);
+    m_ghoff.alloc(V + 1);
+    m_ghpart.alloc(tiles);
+    m_shub.alloc(V);
+    m_shoff.alloc(V + 1);
+    m_shpart.alloc(tiles);
+    m_bsum.alloc(SCAN_BLOCKS);
+    m_stat.alloc(3 * SCAN_BLOCKS);
+    m_rpart.alloc(SCAN_BLOCKS);
     m_pfirst.alloc(tiles);
     m_plast.alloc(tiles);
-    m_gpart.alloc(tiles);
-    m_apart.alloc(tiles);
-    m_spart.alloc(tiles);
-    check(gasrt_memset(m_bm.p, 0, sizeof(uint32_t) * nwords, m_stream),
+    check(gasrt_memset(m_flags.p, 0, (size_t)nfwords * 4, m_stream), "memset");
+    check(gasrt_memset(m_stat.p, 0, sizeof(int32_t) * 3 * SCAN_BLOCKS,
+                       m_stream),
           "memset");
     check(gasrt_memset(m_rpart.p, 0, sizeof(double) * SCAN_BLOCKS, m_stream),
           "memset");
     Ctl c;
     memset(&c, 0, sizeof(c));
-    c.dense_min = m_dense_min >= 0 ? m_dense_min : std::max(1, m_nV / 64);
     check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
 
     Dev<Program> &d = m_dev;
@@ -369,23 +533,23 @@ class GASEngineGPU {
     d.ctl = m_ctl.p;
     d.q[0] = m_q0.p;
     d.q[1] = m_q1.p;
-    d.gpart = m_gpart.p;
-    d.apart = m_apart.p;
-    d.spart = m_spart.p;
-    d.bm = m_bm.p;
-    d.nwords = nwords;
-    d.goff = m_goff.p;
-    d.soff = m_soff.p;
-    d.pdeg = m_pdeg.p;
-    d.bsum = m_bsum.p;
-    d.rpart = m_rpart.p;
+    d.flags = m_flags.p;
+    d.nfwords = nfwords;
+    d.rflag = m_rflag.p;
     d.gtmp = m_gtmp.p;
+    d.ghub = m_ghub.p;
+    d.ghoff = m_ghoff.p;
+    d.ghpart = m_ghpart.p;
+    d.shub = m_shub.p;
+    d.shoff = m_shoff.p;
+    d.shpart = m_shpart.p;
+    d.bsum = m_bsum.p;
+    d.stat_blk = m_stat.p;
+    d.rpart = m_rpart.p;
     d.part_first = m_pfirst.p;
     d.part_last = m_plast.p;
     m_par = 0;
-    launch(k_static_partition<Program>,
-           (unsigned)std::min<int64_t>(1024, (V + NT - 1) / NT + 1), m_stream,
-           m_dev);
+    buildHubs();
   }
 
   void timedBegin() {
@@ -428,19 +592,22 @@ class GASEngineGPU {
   gasrt_stream_t m_stream = nullptr;
   Ctl *m_hctl = nullptr;  // pinned ring of control-block snapshots
   gasrt_event_t m_ring_ev[kRing] = {};
+  int m_ring_par[kRing] = {};
   gas_graph_t m_graph = nullptr;
   gas_graph_t m_own_graph = nullptr;
   Int m_nV = 0, m_nE = 0;
-  int32_t m_dense_min = -1;
+  int m_gt1 = DEFAULT_T1, m_gt2 = DEFAULT_T2;
+  int m_st1 = DEFAULT_T1, m_st2 = DEFAULT_T2;
+  int32_t m_max_in = 0, m_max_out = 0;
+  std::vector<int32_t> m_h_csc_off;
+  int m_par = 0;  // parity of the next superstep
   VertexData *m_vd_host = nullptr;
   DevBuf<VertexData> m_vdata;
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;
-  DevBuf<int32_t> m_q0, m_q1, m_goff, m_soff, m_pdeg, m_bsum;
-  DevBuf<int32_t> m_gpart, m_apart, m_spart;
-  int m_par = 0;  // parity of the next superstep
-  int m_ring_par[kRing] = {};
-  DevBuf<uint32_t> m_bm;
+  DevBuf<int32_t> m_q0, m_q1, m_ghub, m_ghoff, m_ghpart, m_shub, m_shoff,
+      m_shpart, m_bsum, m_stat, m_ahub, m_ahoff, m_ahpart;
+  DevBuf<uint8_t> m_flags, m_rflag;
   DevBuf<double> m_rpart;
   DevBuf<GatherResult> m_gtmp;
   DevBuf<Acc<Program>> m_pfirst, m_plast;

@@ -2,35 +2,39 @@
 //
 // One superstep of the reference engine (gpugas.h:421-751, semantics of
 // refgas.h:143-210) runs here as a fixed sequence of kernels whose sizes
-// all come from a device-resident control block, so a superstep can be
-// launched (or captured in a hipGraph) without the host knowing the
-// frontier size.  Supersteps alternate between two parities p = 0, 1: the
-// active list of a superstep with parity p lives in queue p and its size in
-// n_active[p]; the superstep builds the next list in queue p^1.  The host
-// passes p as a kernel argument, so no kernel has to publish a superstep's
-// end to the others.
+// all come from a device-resident control block, so the host never waits
+// for a frontier size.  Supersteps alternate between two parities p = 0, 1:
+// the active list of a superstep with parity p lives in queue p and its
+// size in n_active[p]; the superstep builds the next list in queue p^1.  The
+// host passes p as a kernel argument.
 //
-//   k_gscan1 / k_gscan2  exclusive scan of the active vertices' in-degrees
-//                        plus the merge-path tile partition (skipped when
-//                        every vertex is active: the CSC offsets already
-//                        are that scan and the partition is precomputed)
-//   k_gather             merge-path gather over the active vertices' CSC
-//                        in-edges with the segmented reduce fused in; a
-//                        segment that crosses a tile boundary leaves its
-//                        per-tile partials behind
-//   k_apply              finishes crossing segments from the tile partials,
-//                        runs Program::apply, records the predicated
-//                        out-degree of each returning vertex (the
-//                        PredicatedEdgeCountIterator of gpugas.h:387-408)
-//   k_sscan2             exclusive scan of those out-degrees + partition
-//   k_scatter            merge-path walk of the returning vertices' CSR
-//                        out-edges setting frontier bits; in queue mode the
-//                        first setter of a bit also appends the vertex
-//   k_compact1 / 2       bitmap mode only: ordered compaction of the
-//                        frontier bitmap into the next active list
-//                        (popcount + LDS block scan, no atomics)
+// Work on an active vertex is split by degree (north-star (a)):
+//   degree <= t1        one thread walks the vertex's edges;
+//   t1 < degree <= t2   one wave64 walks them 64 at a time (coalesced) and
+//                       reduces with cross-lane shuffles;
+//   degree > t2         "hubs": the vertex goes to a hub list whose edges
+//                       are cut into equal merge-path tiles of MP_NV items
+//                       staged through LDS, with a segmented reduce and a
+//                       fold of the per-tile partials of segments that
+//                       cross tiles (north-star (b)).
+// The host knows the graph's maximum in/out degree and launches the hub
+// kernels only when some vertex exceeds t2.
 //
-// Wave size is 64 (CDNA).  No MFMA: there is no dense contraction here.
+//   k_gather_vp          thread/wave tiers of the gather; appends hubs
+//   k_hscan1/2, k_hgather, k_hfix   hub gather (skipped without hubs)
+//   k_apply              Program::apply (north-star (c)); fused with the
+//                        thread/wave tiers of the scatter inside run(), or
+//                        recording the return flag for scatterActivate()
+//   k_scatter_vp         scatter tiers when apply was not fused
+//   k_sscan1/2, k_hscatter          hub scatter (skipped without hubs)
+//   k_compact1/2         ordered compaction of the byte activation flags
+//                        into the next active list (north-star (e))
+//
+// Activation (north-star (d)) writes one byte per destination with a plain
+// store: every writer stores the same value, so no atomics are needed, and
+// the compaction yields the next list in ascending vertex order, as
+// refgas.h:201-210 does.  Wave size is 64.  No MFMA: there is no dense
+// contraction on this path.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,12 +44,16 @@
 
 namespace gas {
 
-constexpr int NT = 256;             // threads per workgroup: 4 waves of 64
+constexpr int NT = 256;            // threads per workgroup: 4 waves of 64
 constexpr int WAVE = 64;
-constexpr int SCAN_BLOCKS = 1024;   // chunks of the two-pass scans
-constexpr int MP_VT = 8;            // merge-path items per thread
-constexpr int MP_NV = NT * MP_VT;   // merge-path items per tile
-constexpr int MP_BLOCKS = 1024;     // grid of the merge-path kernels
+constexpr int SCAN_BLOCKS = 1024;  // chunks of per-vertex kernels and scans
+constexpr int MP_VT = 8;           // merge-path items per thread
+constexpr int MP_NV = NT * MP_VT;  // merge-path items per tile
+constexpr int MP_BLOCKS = 1024;    // grid of the merge-path kernels
+constexpr int T1_MAX = 16;         // largest thread tier (unrolled loads)
+constexpr int MID_UNROLL = 8;      // wave tier: edges per lane per trip
+constexpr int DEFAULT_T1 = 16;     // thread tier: degree <= t1
+constexpr int DEFAULT_T2 = 512;    // wave tier: degree <= t2; above: hubs
 
 // ---------------------------------------------------------------------------
 // Superstep number visible to vertex programs.  The engine's apply kernel
@@ -65,10 +73,10 @@ struct Ctl {
   int32_t n_active[2];    // |A| of the superstep of each parity
   int32_t all_active[2];  // 1: A is implicitly [0, V)
   int32_t iteration;      // supersteps completed (BFS level)
-  int32_t gather_items;   // sum of in-degrees over A
-  int32_t scatter_items;  // sum of predicated out-degrees over A
-  int32_t next_dense;     // 1: next frontier via bitmap + compaction
-  int32_t dense_min;      // scatter_items at or above which bitmap mode runs
+  int32_t n_ghub;         // gather hubs appended this superstep
+  int32_t ghub_items;     // their summed in-degree
+  int32_t n_shub;         // scatter hubs appended this superstep
+  int32_t shub_items;     // their summed out-degree
   int32_t pad[3];
   int64_t stat_gathered;  // totals over the run
   int64_t stat_scattered;
@@ -84,9 +92,18 @@ template <class P>
 struct is_compensated<P, std::void_t<decltype(P::compensatedSum)>>
     : std::integral_constant<bool, P::compensatedSum> {};
 
+template <class T>
+__device__ inline T shfl_xor_4b(T x, int m) {
+  static_assert(sizeof(T) == 4, "cross-lane reduce needs a 4-byte result");
+  int r = __shfl_xor(__builtin_bit_cast(int, x), m, WAVE);
+  return __builtin_bit_cast(T, r);
+}
+
 // Partial gather result with an explicit "holds a value" flag, so partial
-// sums never need an identity element: the seed gatherZero is folded in
-// exactly once when a segment is finished (refgas.h:148-158).
+// results never need an identity element: the seed gatherZero is folded in
+// exactly once when a vertex's gather is finished (refgas.h:148-158).
+// Every program here has a commutative, associative gatherReduce, so the
+// order partials are combined in does not change integer results.
 template <class P, bool Comp = is_compensated<P>::value>
 struct Acc {
   typedef typename P::GatherResult T;
@@ -109,6 +126,12 @@ struct Acc {
   // this holds the earlier items, b the later ones
   __device__ void merge(const Acc &b) {
     if (b.ok) add(b.v);
+  }
+  __device__ Acc xor_peer(int m) const {
+    Acc r;
+    r.v = shfl_xor_4b(v, m);
+    r.ok = __shfl_xor(ok, m, WAVE);
+    return r;
   }
   __device__ T finish() const {
     T z = P::gatherZero;
@@ -158,6 +181,13 @@ struct Acc<P, true> {
     s = two_sum(s, b.s, err);
     c = c + b.c + err;
   }
+  __device__ Acc xor_peer(int m) const {
+    Acc r;
+    r.s = __shfl_xor(s, m, WAVE);
+    r.c = __shfl_xor(c, m, WAVE);
+    r.ok = __shfl_xor(ok, m, WAVE);
+    return r;
+  }
   __device__ T finish() const {
     T z = P::gatherZero;
     return ok ? P::gatherReduce(z, s + c) : z;
@@ -190,18 +220,23 @@ struct Dev {
   const ED *edata;  // CSC order
   VD *vdata;
   Ctl *ctl;
-  int32_t *q[2];    // active lists by parity
-  uint32_t *bm;     // frontier bitmap, (V+31)/32 words
-  int32_t nwords;
-  int32_t *goff;    // scan of in-degrees over A (V+1)
-  int32_t *soff;    // scan of predicated out-degrees over A (V+1)
-  int32_t *pdeg;    // predicated out-degree per active slot
-  int32_t *bsum;    // per-chunk sums of the two-pass scans
-  int32_t *gpart;   // first segment of every gather tile (queue mode)
-  int32_t *apart;   // same for the all-active gather (fixed per graph)
-  int32_t *spart;   // same for the scatter
-  double *rpart;    // per-chunk FP64 residual sums
-  GR *gtmp;         // gather result per active slot
+  int32_t *q[2];     // active lists by parity
+  uint8_t *flags;    // activation byte per vertex (V rounded up to 4)
+  int32_t nfwords;   // flags as 32-bit words
+  uint8_t *rflag;    // apply's return per active slot (unfused path)
+  GR *gtmp;          // gather result per active slot
+  int32_t gt1, gt2;  // gather degree tiers (thread <= gt1 < wave <= gt2 < hub)
+  int32_t st1, st2;  // scatter degree tiers
+  // gather hubs of the current superstep (queue mode): active slots
+  int32_t *ghub, *ghoff, *ghpart;
+  // gather hubs when every vertex is active (fixed per graph): vertex ids
+  const int32_t *ahub, *ahoff, *ahpart;
+  int32_t n_ahub, ahub_items;
+  // scatter hubs of the current superstep: active slots
+  int32_t *shub, *shoff, *shpart;
+  int32_t *bsum;     // per-chunk sums of the two-pass scans
+  int32_t *stat_blk; // per-workgroup counters: gathered | scattered
+  double *rpart;     // per-chunk FP64 residual sums
   Acc<P> *part_first, *part_last;  // per-tile partials of crossing segments
 };
 
@@ -284,7 +319,12 @@ __device__ inline int lds_lower_bound(const int32_t *a, int n, int x) {
   return lo;
 }
 
-// Chunk [b, e) of n items handled by workgroup `blk` in the two-pass scans.
+__device__ inline int lanemask_lt_count(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Chunk [b, e) of n items handled by workgroup `blk`.
 __device__ inline void scan_chunk(int n, int blk, int &b, int &e) {
   int c = (n + SCAN_BLOCKS - 1) / SCAN_BLOCKS;
   b = min(n, blk * c);
@@ -304,21 +344,44 @@ __device__ inline void chunk_prefix(const int32_t *bsum, int blk, int &pre,
   total = block_sum(t, lds);
 }
 
-// Merge-path layout: segment i (0 <= i < A) occupies the positions
-// off[i]+i .. off[i+1]+i -- its edges, then its end item.  Tile t covers
+// Merge-path layout: segment h (0 <= h < n) occupies the positions
+// off[h]+h .. off[h+1]+h -- its edges, then its end item.  Tile t covers
 // positions [t*MP_NV, (t+1)*MP_NV); part[t] is the segment holding the
-// tile's first position and part[ntiles] = A.  The writer of segment i
+// tile's first position and part[ntiles] = n.  The writer of segment h
 // records itself for every tile that starts inside it.
-__device__ inline void mark_tiles(int32_t *part, int i, int off_i, int deg) {
-  const int start = off_i + i;
-  const int end = off_i + deg + i;
+__device__ inline void mark_tiles(int32_t *part, int h, int off_h, int deg) {
+  const int start = off_h + h;
+  const int end = off_h + deg + h;
   for (int t = (start + MP_NV - 1) / MP_NV; t * MP_NV <= end; ++t)
-    part[t] = i;
+    part[t] = h;
+}
+
+// Appends `slot` to a global list for the lanes with `take`, one
+// atomicAdd per wave-round into LDS and one per workgroup round globally.
+// Must be reached by every thread of the workgroup.
+__device__ inline void block_append(bool take, int slot, int32_t *list,
+                                    int32_t *count, int32_t *s_buf,
+                                    int *s_cnt, int *s_base) {
+  if (threadIdx.x == 0) *s_cnt = 0;
+  __syncthreads();
+  uint64_t m = __ballot(take);
+  if (m) {
+    int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(s_cnt, __popcll(m));
+    base = __shfl(base, leader, WAVE);
+    if (take) s_buf[base + lanemask_lt_count(m)] = slot;
+  }
+  __syncthreads();
+  const int n = *s_cnt;
+  if (threadIdx.x == 0 && n) *s_base = atomicAdd(count, n);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += NT) list[*s_base + i] = s_buf[i];
 }
 
 // ---------------------------------------------------------------------------
-// Frontier initialisation: setActive(start, end) (gpugas.h:342-349) plus
-// clearing the frontier bitmap.
+// Frontier initialisation: setActive(start, end) (gpugas.h:342-349); clears
+// the activation flags and the hub counters.
 template <class P>
 __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
                                                    int end) {
@@ -327,137 +390,244 @@ __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
   const int n = end - start;
   int32_t *q = d.q[p];
   const int stride = gridDim.x * NT;
-  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nwords; i += stride)
-    d.bm[i] = 0u;
+  uint32_t *fw = reinterpret_cast<uint32_t *>(d.flags);
+  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nfwords; i += stride)
+    fw[i] = 0u;
   if (!all)
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += stride)
       q[i] = start + i;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     c->n_active[p] = n;
     c->all_active[p] = all ? 1 : 0;
+    c->n_ghub = 0;
+    c->n_shub = 0;
   }
 }
 
-// Tile partition of the all-active gather (depends on the graph only).
+// Thread and wave tiers of the gather; hub slots are appended to ghub
+// (queue mode) or skipped (all active: the fixed hub list covers them).
 template <class P>
-__global__ __launch_bounds__(NT) void k_static_partition(Dev<P> d) {
-  const int V = d.nV;
-  for (int i = blockIdx.x * NT + threadIdx.x; i < V; i += gridDim.x * NT) {
-    int o = d.csc_off[i];
-    mark_tiles(d.apart, i, o, d.csc_off[i + 1] - o);
+__global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
+  typedef typename P::GatherResult GR;
+  __shared__ int32_t s_buf[NT];
+  __shared__ int s_cnt, s_base;
+  __shared__ int lds[NT / WAVE];
+  Ctl *c = d.ctl;
+  const int A = c->n_active[p];
+  if (A == 0) return;
+  const bool all = c->all_active[p] != 0;
+  const int32_t *q = d.q[p];
+  const int stride = gridDim.x * NT;
+  const int rounds = (A + stride - 1) / stride;
+  const int lane = lane_id();
+  int gathered = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
+    const bool valid = i < A;
+    int v = 0, o = 0, deg = 0;
+    if (valid) {
+      v = all ? i : q[i];
+      o = d.csc_off[v];
+      deg = d.csc_off[v + 1] - o;
+      gathered += deg;
+    }
+    const bool small = valid && deg <= d.gt1;
+    const bool mid = valid && deg > d.gt1 && deg <= d.gt2;
+    const bool hub = valid && deg > d.gt2;
+    if (small) {
+      // every load of the vertex issued before the first use
+      int src[T1_MAX];
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        src[k] = (k < deg) ? d.csc_src[o + k] : 0;
+      GR val[T1_MAX];
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        if (k < deg)
+          val[k] = P::gatherMap(d.vdata + v, d.vdata + src[k], d.edata + o + k);
+      Acc<P> acc = Acc<P>::empty();
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        if (k < deg) acc.add(val[k]);
+      d.gtmp[i] = acc.finish();
+    }
+    uint64_t m = __ballot(mid);
+    while (m) {
+      const int j = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int vj = __shfl(v, j, WAVE);
+      const int oj = __shfl(o, j, WAVE);
+      const int dj = __shfl(deg, j, WAVE);
+      Acc<P> acc = Acc<P>::empty();
+      for (int base = 0; base < dj; base += WAVE * MID_UNROLL) {
+        int src[MID_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u) {
+          const int k = base + u * WAVE + lane;
+          src[u] = (k < dj) ? d.csc_src[oj + k] : 0;
+        }
+        GR val[MID_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u) {
+          const int k = base + u * WAVE + lane;
+          if (k < dj)
+            val[u] = P::gatherMap(d.vdata + vj, d.vdata + src[u],
+                                  d.edata + oj + k);
+        }
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u)
+          if (base + u * WAVE + lane < dj) acc.add(val[u]);
+      }
+#pragma unroll
+      for (int x = WAVE / 2; x > 0; x >>= 1) acc.merge(acc.xor_peer(x));
+      if (lane == j) d.gtmp[i] = acc.finish();
+    }
+    if (!all) block_append(hub, i, d.ghub, &c->n_ghub, s_buf, &s_cnt, &s_base);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int N = d.nE + V;
-    d.apart[(N + MP_NV - 1) / MP_NV] = V;
-  }
+  gathered = block_sum(gathered, lds);
+  if (threadIdx.x == 0) d.stat_blk[blockIdx.x] = gathered;
 }
 
-// Pass 1 of the in-degree scan over the active list.
+// Hub list of a superstep: all-active supersteps use the fixed list.
 template <class P>
-__global__ __launch_bounds__(NT) void k_gscan1(Dev<P> d, int p) {
+struct HubList {
+  const int32_t *list, *off, *part;
+  int n, items;
+};
+
+template <class P>
+__device__ inline HubList<P> gather_hubs(const Dev<P> &d, const Ctl *c,
+                                         bool all) {
+  HubList<P> h;
+  if (all) {
+    h.list = d.ahub;
+    h.off = d.ahoff;
+    h.part = d.ahpart;
+    h.n = d.n_ahub;
+    h.items = d.ahub_items;
+  } else {
+    h.list = d.ghub;
+    h.off = d.ghoff;
+    h.part = d.ghpart;
+    h.n = c->n_ghub;
+    h.items = c->ghub_items;
+  }
+  return h;
+}
+
+// Degree scan over a hub list (pass 1: chunk sums).  Which = 0: gather hubs
+// (in-degrees), 1: scatter hubs (out-degrees).
+template <class P, int Which>
+__global__ __launch_bounds__(NT) void k_hscan1(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   const Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0 || c->all_active[p]) return;
+  if (c->n_active[p] == 0) return;
+  const bool all = c->all_active[p] != 0;
+  if (Which == 0 && all) return;  // fixed hub list, scanned on the host
+  const int n = Which == 0 ? c->n_ghub : c->n_shub;
+  const int32_t *list = Which == 0 ? d.ghub : d.shub;
+  const int32_t *gofs = Which == 0 ? d.csc_off : d.csr_off;
   const int32_t *q = d.q[p];
   int b, e;
-  scan_chunk(A, blockIdx.x, b, e);
+  scan_chunk(n, blockIdx.x, b, e);
   int sum = 0;
   for (int i = b + threadIdx.x; i < e; i += NT) {
-    int v = q[i];
-    sum += d.csc_off[v + 1] - d.csc_off[v];
+    const int slot = list[i];
+    const int v = all ? slot : q[slot];
+    sum += gofs[v + 1] - gofs[v];
   }
   sum = block_sum(sum, lds);
   if (threadIdx.x == 0) d.bsum[blockIdx.x] = sum;
 }
 
-// Pass 2 of the in-degree scan: goff[i] = sum of in-degrees of A[0..i),
-// and the gather tile partition.
-template <class P>
-__global__ __launch_bounds__(NT) void k_gscan2(Dev<P> d, int p) {
+// Pass 2: offsets, tile partition and total of the hub list.
+template <class P, int Which>
+__global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  if (c->all_active[p]) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      c->gather_items = d.nE;
-      c->stat_gathered += d.nE;
-    }
-    return;
-  }
+  if (c->n_active[p] == 0) return;
+  const bool all = c->all_active[p] != 0;
+  if (Which == 0 && all) return;
+  const int n = Which == 0 ? c->n_ghub : c->n_shub;
+  if (n == 0) return;
+  const int32_t *list = Which == 0 ? d.ghub : d.shub;
+  int32_t *off = Which == 0 ? d.ghoff : d.shoff;
+  int32_t *part = Which == 0 ? d.ghpart : d.shpart;
+  const int32_t *gofs = Which == 0 ? d.csc_off : d.csr_off;
   const int32_t *q = d.q[p];
   int b, e, pre, total;
-  scan_chunk(A, blockIdx.x, b, e);
+  scan_chunk(n, blockIdx.x, b, e);
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
   int carry = pre;
   for (int base = b; base < e; base += NT) {
-    int i = base + threadIdx.x;
+    const int i = base + threadIdx.x;
     int deg = 0;
     if (i < e) {
-      int v = q[i];
-      deg = d.csc_off[v + 1] - d.csc_off[v];
+      const int slot = list[i];
+      const int v = all ? slot : q[slot];
+      deg = gofs[v + 1] - gofs[v];
     }
     int tot;
-    int ex = block_excl_scan(deg, tot, lds);
+    const int ex = block_excl_scan(deg, tot, lds);
     if (i < e) {
-      d.goff[i] = carry + ex;
-      mark_tiles(d.gpart, i, carry + ex, deg);
+      off[i] = carry + ex;
+      mark_tiles(part, i, carry + ex, deg);
     }
     carry += tot;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d.goff[A] = total;
-    d.gpart[(total + A + MP_NV - 1) / MP_NV] = A;
-    c->gather_items = total;
-    c->stat_gathered += total;
+    off[n] = total;
+    part[(total + n + MP_NV - 1) / MP_NV] = n;
+    if (Which == 0)
+      c->ghub_items = total;
+    else
+      c->shub_items = total;
   }
 }
 
-// Merge-path gather + segmented reduce (north-star (a)+(b); replaces
-// kGatherMap + ReduceByKey, gpugas_kernels.cuh:53-134, gpugas.h:466-500).
-// Each tile takes MP_NV consecutive items of the path formed by every
-// active vertex's in-edges followed by its end marker, so a hub's edge list
-// is split across as many tiles as it needs while low-degree vertices share
-// one.  A segment finished inside a tile is written straight to gtmp; for
-// a segment crossing tiles each tile leaves the partial before its first
-// end (part_first) and after its last end (part_last), and k_apply folds
-// them.
+// Hub gather: merge-path tiles over the hubs' CSC in-edges with the
+// segmented reduce fused in (replaces kGatherMap + ReduceByKey,
+// gpugas_kernels.cuh:53-134, gpugas.h:466-500).  A hub finished inside a
+// tile is written to gtmp straight away; for a hub crossing tiles each tile
+// leaves the partial before its first end (part_first) and after its last
+// end (part_last), and k_hfix folds them.
 template <class P>
-__global__ __launch_bounds__(NT) void k_gather(Dev<P> d, int p) {
+__global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   __shared__ int32_t s_end[MP_NV + 1];
   __shared__ int32_t s_base[MP_NV + 1];
   __shared__ int32_t s_vtx[MP_NV + 1];
+  __shared__ int32_t s_slot[MP_NV + 1];
   __shared__ SegItem<P> s_seg[NT];
 
   const Ctl *c = d.ctl;
   const int A = c->n_active[p];
   if (A == 0) return;
   const bool all = c->all_active[p] != 0;
+  const HubList<P> h = gather_hubs(d, c, all);
+  if (h.n == 0) return;
   const int32_t *q = d.q[p];
-  const int32_t *off = all ? d.csc_off : d.goff;
-  const int32_t *part = all ? d.apart : d.gpart;
-  const int N = c->gather_items + A;
+  const int N = h.items + h.n;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
     const int d1 = min(d0 + MP_NV, N);
-    const int s0 = part[t];
-    const int s1 = part[t + 1];
-    const int nseg = min(s1, A - 1) - s0 + 1;
+    const int s0 = h.part[t];
+    const int s1 = h.part[t + 1];
+    const int nseg = min(s1, h.n - 1) - s0 + 1;
     for (int k = tid; k < nseg; k += NT) {
-      int s = s0 + k;
-      int v = all ? s : q[s];
-      int o = off[s];
+      const int s = s0 + k;
+      const int slot = h.list[s];
+      const int v = all ? slot : q[slot];
       s_vtx[k] = v;
-      s_base[k] = d.csc_off[v] - o;
-      s_end[k] = off[s + 1] + s;
+      s_slot[k] = slot;
+      s_base[k] = d.csc_off[v] - h.off[s];
+      s_end[k] = h.off[s + 1] + s;
     }
     __syncthreads();
-    const bool first_fresh = (off[s0] + s0 == d0);
+    const bool first_fresh = (h.off[s0] + s0 == d0);
 
     // decode this thread's items
     const int p0 = d0 + tid * MP_VT;
@@ -505,7 +675,7 @@ __global__ __launch_bounds__(NT) void k_gather(Dev<P> d, int p) {
           head = acc;
           first_k = kk[it];
         } else {
-          d.gtmp[s0 + kk[it]] = acc.finish();
+          d.gtmp[s_slot[kk[it]]] = acc.finish();
         }
         ++nend;
         acc = Acc<P>::empty();
@@ -533,7 +703,7 @@ __global__ __launch_bounds__(NT) void k_gather(Dev<P> d, int p) {
     if (nend > 0) {
       Acc<P> first = carry.a;
       first.merge(head);
-      if (first_k > 0 || first_fresh) d.gtmp[s0 + first_k] = first.finish();
+      if (first_k > 0 || first_fresh) d.gtmp[s_slot[first_k]] = first.finish();
       if (!carry.flag) d.part_first[t] = first;
     }
     if (tid == NT - 1) {
@@ -546,19 +716,22 @@ __global__ __launch_bounds__(NT) void k_gather(Dev<P> d, int p) {
   }
 }
 
-// Gather result of active slot i: direct, or folded from tile partials
-// when its segment crosses tiles.
+// Folds the tile partials of hubs whose edges cross tiles.
 template <class P>
-__device__ inline typename P::GatherResult gather_result(const Dev<P> &d,
-                                                         const int32_t *off,
-                                                         int i) {
-  const int st = off[i] + i;
-  const int en = off[i + 1] + i;
-  const int t0 = st / MP_NV, t1 = en / MP_NV;
-  if (t0 == t1) return d.gtmp[i];
-  Acc<P> a = (st > t0 * MP_NV) ? d.part_last[t0] : d.part_first[t0];
-  for (int t = t0 + 1; t <= t1; ++t) a.merge(d.part_first[t]);
-  return a.finish();
+__global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
+  const Ctl *c = d.ctl;
+  if (c->n_active[p] == 0) return;
+  const bool all = c->all_active[p] != 0;
+  const HubList<P> h = gather_hubs(d, c, all);
+  for (int s = blockIdx.x * NT + threadIdx.x; s < h.n; s += gridDim.x * NT) {
+    const int st = h.off[s] + s;
+    const int en = h.off[s + 1] + s;
+    const int t0 = st / MP_NV, t1 = en / MP_NV;
+    if (t0 == t1) continue;
+    Acc<P> a = (st > t0 * MP_NV) ? d.part_last[t0] : d.part_first[t0];
+    for (int t = t0 + 1; t <= t1; ++t) a.merge(d.part_first[t]);
+    d.gtmp[h.list[s]] = a.finish();
+  }
 }
 
 // Optional program hook: `static double residual(const VertexData &before,
@@ -571,13 +744,46 @@ struct has_residual<P, std::void_t<decltype(P::residual(
                            std::declval<const typename P::VertexData &>()))>>
     : std::true_type {};
 
-// Apply (kApply, gpugas_kernels.cuh:139-151) fused with the fold of
-// crossing segments and pass 1 of the scatter scan.  Also resets the next
-// parity's list size, which k_scatter / k_compact2 fill.
-template <class P, bool HaveGather>
+// Thread and wave tiers of the scatter for the slots whose apply returned
+// true (`ret`); hubs go to the scatter hub list.  Reached by every thread.
+template <class P>
+__device__ inline int scatter_tiers(const Dev<P> &d, Ctl *c, bool ret,
+                                    int slot, int v, int32_t *s_buf,
+                                    int *s_cnt, int *s_base) {
+  int o = 0, deg = 0;
+  if (ret) {
+    o = d.csr_off[v];
+    deg = d.csr_off[v + 1] - o;
+  }
+  if (ret && deg <= d.st1) {
+#pragma unroll 4
+    for (int k = 0; k < deg; ++k) d.flags[d.csr_dst[o + k]] = 1;
+  }
+  const bool mid = ret && deg > d.st1 && deg <= d.st2;
+  uint64_t m = __ballot(mid);
+  const int lane = lane_id();
+  while (m) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1;
+    const int oj = __shfl(o, j, WAVE);
+    const int dj = __shfl(deg, j, WAVE);
+    for (int k = lane; k < dj; k += WAVE) d.flags[d.csr_dst[oj + k]] = 1;
+  }
+  block_append(ret && deg > d.st2, slot, d.shub, &c->n_shub, s_buf, s_cnt,
+               s_base);
+  return ret ? deg : 0;
+}
+
+// Apply (kApply, gpugas_kernels.cuh:139-151).  Fuse = true also runs the
+// scatter tiers (the whole superstep is being executed, so scatterActivate
+// follows); otherwise the return flag is kept for k_scatter_vp.  Resets the
+// next parity's list size, which k_compact2 fills.
+template <class P, bool HaveGather, bool Fuse>
 __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   typedef typename P::VertexData VD;
+  __shared__ int32_t s_buf[NT];
+  __shared__ int s_cnt, s_base;
   __shared__ int lds[NT / WAVE];
   __shared__ double ldsd[NT / WAVE];
   Ctl *c = d.ctl;
@@ -594,220 +800,176 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   __syncthreads();
   const bool all = c->all_active[p] != 0;
   const int32_t *q = d.q[p];
-  const int32_t *off = all ? d.csc_off : d.goff;
-  int b, e;
-  scan_chunk(A, blockIdx.x, b, e);
-  int sum = 0;
+  const int stride = gridDim.x * NT;
+  const int rounds = (A + stride - 1) / stride;
+  int scattered = 0;
   double res = 0.0;
-  for (int i = b + threadIdx.x; i < e; i += NT) {
-    const int v = all ? i : q[i];
-    GR g;
-    if (HaveGather)
-      g = gather_result(d, off, i);
-    else
-      g = P::gatherZero;
-    bool ret;
-    if constexpr (has_residual<P>::value) {
-      VD before = d.vdata[v];
-      ret = P::apply(d.vdata + v, g);
-      res += P::residual(before, d.vdata[v]);
-    } else {
-      ret = P::apply(d.vdata + v, g);
+  for (int r = 0; r < rounds; ++r) {
+    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
+    const bool valid = i < A;
+    int v = 0;
+    bool ret = false;
+    if (valid) {
+      v = all ? i : q[i];
+      GR g;
+      if (HaveGather)
+        g = d.gtmp[i];
+      else
+        g = P::gatherZero;
+      if constexpr (has_residual<P>::value) {
+        VD before = d.vdata[v];
+        ret = P::apply(d.vdata + v, g);
+        res += P::residual(before, d.vdata[v]);
+      } else {
+        ret = P::apply(d.vdata + v, g);
+      }
     }
-    if (!all) d.bm[v >> 5] = 0u;  // every set bit belongs to A
-    int pd = ret ? d.csr_off[v + 1] - d.csr_off[v] : 0;
-    d.pdeg[i] = pd;
-    sum += pd;
+    if (Fuse) {
+      scattered += scatter_tiers(d, c, ret, i, v, s_buf, &s_cnt, &s_base);
+    } else if (valid) {
+      d.rflag[i] = ret ? 1 : 0;
+    }
   }
-  sum = block_sum(sum, lds);
-  if (threadIdx.x == 0) d.bsum[blockIdx.x] = sum;
+  if (!HaveGather && threadIdx.x == 0) d.stat_blk[blockIdx.x] = 0;
+  scattered = block_sum(scattered, lds);
+  if (threadIdx.x == 0) d.stat_blk[SCAN_BLOCKS + blockIdx.x] = scattered;
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
     if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
   }
 }
 
-// Pass 2 of the scatter scan and the scatter tile partition; picks bitmap
-// or queue mode for the next frontier from the number of edges to scatter,
-// and counts the superstep (all applies of this superstep are done).
+// Scatter tiers after an unfused apply (scatterActivate, gpugas.h:669-751).
 template <class P>
-__global__ __launch_bounds__(NT) void k_sscan2(Dev<P> d, int p) {
+__global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
+  __shared__ int32_t s_buf[NT];
+  __shared__ int s_cnt, s_base;
   __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
   const int A = c->n_active[p];
   if (A == 0) return;
-  int b, e, pre, total;
-  scan_chunk(A, blockIdx.x, b, e);
-  chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
-  int carry = pre;
-  for (int base = b; base < e; base += NT) {
-    int i = base + threadIdx.x;
-    int x = (i < e) ? d.pdeg[i] : 0;
-    int tot;
-    int ex = block_excl_scan(x, tot, lds);
-    if (i < e) {
-      d.soff[i] = carry + ex;
-      mark_tiles(d.spart, i, carry + ex, x);
-    }
-    carry += tot;
+  const bool all = c->all_active[p] != 0;
+  const int32_t *q = d.q[p];
+  const int stride = gridDim.x * NT;
+  const int rounds = (A + stride - 1) / stride;
+  int scattered = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
+    const bool valid = i < A;
+    const bool ret = valid && d.rflag[i] != 0;
+    const int v = valid ? (all ? i : q[i]) : 0;
+    scattered += scatter_tiers(d, c, ret, i, v, s_buf, &s_cnt, &s_base);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d.soff[A] = total;
-    d.spart[(total + A + MP_NV - 1) / MP_NV] = A;
-    c->scatter_items = total;
-    c->next_dense = total >= c->dense_min ? 1 : 0;
-    c->stat_scattered += total;
-    c->stat_active += A;
-    c->stat_supersteps += 1;
-    c->iteration += 1;
-  }
+  scattered = block_sum(scattered, lds);
+  if (threadIdx.x == 0) d.stat_blk[SCAN_BLOCKS + blockIdx.x] = scattered;
 }
 
-// Scatter/activate (north-star (d); replaces IntervalGather into
-// ActivateOutputIterator / ActivateOutputIteratorSmallSize,
-// gpugas.h:545-597, 693-730).  Bitmap mode: fire-and-forget atomicOr.
-// Queue mode: the lane whose atomicOr flips a bit appends the vertex; a
-// tile collects its new vertices in LDS (ballot + mbcnt per wave) and
-// reserves space in the next list with a single atomicAdd.
+// Hub scatter: merge-path tiles over the scatter hubs' CSR out-edges.
 template <class P>
-__global__ __launch_bounds__(NT) void k_scatter(Dev<P> d, int p) {
+__global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
   __shared__ int32_t s_end[MP_NV + 1];
   __shared__ int32_t s_base[MP_NV + 1];
-  __shared__ int32_t s_fresh[MP_NV];
-  __shared__ int s_cnt, s_gbase;
-  Ctl *c = d.ctl;
+  const Ctl *c = d.ctl;
   const int A = c->n_active[p];
   if (A == 0) return;
-  const int ES = c->scatter_items;
-  if (ES == 0) return;
+  const int n = c->n_shub;
+  if (n == 0) return;
   const bool all = c->all_active[p] != 0;
-  const bool dense = c->next_dense != 0;
   const int32_t *q = d.q[p];
-  int32_t *qn = d.q[p ^ 1];
-  int32_t *nn = &c->n_active[p ^ 1];
-  const int32_t *off = d.soff;
-  const int N = ES + A;
+  const int N = c->shub_items + n;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
-  const int lane = lane_id();
-
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
     const int d1 = min(d0 + MP_NV, N);
-    const int s0 = d.spart[t];
-    const int s1 = d.spart[t + 1];
-    const int nseg = min(s1, A - 1) - s0 + 1;
+    const int s0 = d.shpart[t];
+    const int s1 = d.shpart[t + 1];
+    const int nseg = min(s1, n - 1) - s0 + 1;
     for (int k = tid; k < nseg; k += NT) {
-      int s = s0 + k;
-      int v = all ? s : q[s];
-      s_base[k] = d.csr_off[v] - off[s];
-      s_end[k] = off[s + 1] + s;
+      const int s = s0 + k;
+      const int slot = d.shub[s];
+      const int v = all ? slot : q[slot];
+      s_base[k] = d.csr_off[v] - d.shoff[s];
+      s_end[k] = d.shoff[s + 1] + s;
     }
     __syncthreads();
     const int p0 = d0 + tid * MP_VT;
     int k = (p0 < d1) ? lds_lower_bound(s_end, nseg, p0) : 0;
-    int eidx[MP_VT];
-    bool isedge[MP_VT];
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
       const int pos = p0 + it;
-      isedge[it] = false;
-      eidx[it] = 0;
       if (pos < d1) {
-        if (s_end[k] == pos) {
+        if (s_end[k] == pos)
           ++k;
-        } else {
-          isedge[it] = true;
-          eidx[it] = s_base[k] + (pos - s0 - k);
-        }
+        else
+          d.flags[d.csr_dst[s_base[k] + (pos - s0 - k)]] = 1;
       }
-    }
-    int w[MP_VT];
-#pragma unroll
-    for (int it = 0; it < MP_VT; ++it)
-      w[it] = isedge[it] ? d.csr_dst[eidx[it]] : 0;
-    if (dense) {
-#pragma unroll
-      for (int it = 0; it < MP_VT; ++it)
-        if (isedge[it]) atomicOr(&d.bm[w[it] >> 5], 1u << (w[it] & 31));
-    } else {
-      // stage this tile's newly activated vertices in LDS, then reserve
-      // their slots in the next list with one atomicAdd per tile
-      if (tid == 0) s_cnt = 0;
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < MP_VT; ++it) {
-        bool fresh = false;
-        if (isedge[it]) {
-          uint32_t bit = 1u << (w[it] & 31);
-          uint32_t old = atomicOr(&d.bm[w[it] >> 5], bit);
-          fresh = (old & bit) == 0u;
-        }
-        uint64_t m = __ballot(fresh);
-        if (m) {
-          int leader = __ffsll((unsigned long long)m) - 1;
-          int base = 0;
-          if (lane == leader) base = atomicAdd(&s_cnt, __popcll(m));
-          base = __shfl(base, leader, WAVE);
-          if (fresh) {
-            int rank = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            s_fresh[base + rank] = w[it];
-          }
-        }
-      }
-      __syncthreads();
-      const int cnt = s_cnt;
-      if (tid == 0 && cnt) s_gbase = atomicAdd(nn, cnt);
-      __syncthreads();
-      for (int i = tid; i < cnt; i += NT) qn[s_gbase + i] = s_fresh[i];
     }
     __syncthreads();
   }
 }
 
-// Frontier compaction, bitmap mode (north-star (e); replaces
+// Ordered compaction of the activation flags (north-star (e); replaces
 // scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157): pass 1 counts
-// set bits per chunk of words, pass 2 writes the ids of every chunk in
-// ascending order at its prefix and clears the words.
-__device__ inline bool compact_dense(const Ctl *c, int p) {
-  return c->n_active[p] > 0 && c->next_dense != 0 && c->scatter_items > 0;
-}
-
+// set flags per chunk of 32-bit flag words and ends the superstep's
+// bookkeeping; pass 2 writes the ids of every chunk in ascending order at
+// its prefix and clears the words.
 template <class P>
 __global__ __launch_bounds__(NT) void k_compact1(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  if (!compact_dense(d.ctl, p)) return;
+  Ctl *c = d.ctl;
+  const int A = c->n_active[p];
+  if (A == 0) return;
+  const uint32_t *fw = reinterpret_cast<const uint32_t *>(d.flags);
   int b, e;
-  scan_chunk(d.nwords, blockIdx.x, b, e);
+  scan_chunk(d.nfwords, blockIdx.x, b, e);
   int cnt = 0;
-  for (int i = b + threadIdx.x; i < e; i += NT) cnt += __popc(d.bm[i]);
+  for (int i = b + threadIdx.x; i < e; i += NT) cnt += __popc(fw[i]);
   cnt = block_sum(cnt, lds);
   if (threadIdx.x == 0) d.bsum[blockIdx.x] = cnt;
+  if (blockIdx.x == 0) {
+    // superstep totals (all per-chunk counters were written by earlier
+    // kernels of this superstep)
+    int g = 0, s = 0;
+    for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) {
+      g += d.stat_blk[i];
+      s += d.stat_blk[SCAN_BLOCKS + i];
+    }
+    const int gi = block_sum(g, lds);
+    const int si = block_sum(s, lds);
+    if (threadIdx.x == 0) {
+      c->stat_gathered += gi;
+      c->stat_scattered += si;
+      c->stat_active += A;
+      c->stat_supersteps += 1;
+      c->iteration += 1;
+      c->n_ghub = 0;
+      c->n_shub = 0;
+    }
+  }
 }
 
 template <class P>
 __global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   Ctl *c = d.ctl;
-  if (!compact_dense(c, p)) return;
+  if (c->n_active[p] == 0) return;
+  uint32_t *fw = reinterpret_cast<uint32_t *>(d.flags);
   int32_t *qn = d.q[p ^ 1];
   int b, e, pre, total;
-  scan_chunk(d.nwords, blockIdx.x, b, e);
+  scan_chunk(d.nfwords, blockIdx.x, b, e);
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
   int carry = pre;
   for (int base = b; base < e; base += NT) {
-    int wi = base + threadIdx.x;
-    uint32_t word = (wi < e) ? d.bm[wi] : 0u;
+    const int wi = base + threadIdx.x;
+    uint32_t word = (wi < e) ? fw[wi] : 0u;
     int tot;
     int pos = carry + block_excl_scan(__popc(word), tot, lds);
     if (word) {
-      d.bm[wi] = 0u;
-      while (word) {
-        int bit = __ffs(word) - 1;
-        qn[pos++] = wi * 32 + bit;
-        word &= word - 1;
-      }
+      fw[wi] = 0u;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((word >> (8 * k)) & 0xffu) qn[pos++] = wi * 4 + k;
     }
     carry += tot;
   }

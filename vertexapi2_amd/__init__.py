@@ -234,6 +234,23 @@ class Graph:
         _check(lib().gas_graph_host_csr(self._h, _ptr(off), _ptr(dst), _ptr(eid)))
         return off, dst, eid
 
+    def set_tiers(self, program: int, tiers) -> None:
+        t = _i32(tiers)
+        assert t.shape == (4,)
+        _check(lib().gas_graph_set_tiers(self._h, C.c_int(program), _ptr(t)))
+
+    def get_tiers(self, program: int):
+        t = np.zeros(4, np.int32)
+        _check(lib().gas_graph_get_tiers(self._h, C.c_int(program), _ptr(t)))
+        return t.tolist()
+
+    def autotune(self, program: int):
+        """Times candidate degree tiers for `program` on this graph and keeps
+        the fastest; returns [gather t1, gather t2, scatter t1, scatter t2]."""
+        t = np.zeros(4, np.int32)
+        _check(lib().gas_graph_autotune(self._h, C.c_int(program), _ptr(t)))
+        return t.tolist()
+
     def bfs(self, source: int):
         out = np.zeros(self.n_vertices, np.int32)
         st = RunStats()

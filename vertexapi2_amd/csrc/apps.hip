@@ -135,6 +135,81 @@ int gas_set_kernel_timing(int enable) {
   return GAS_OK;
 }
 
+}  // extern "C"
+
+// Runs f on the cached engine of (graph, program), creating it with
+// placeholder vertex data when no whole run has created it yet.
+template <class F>
+static int with_engine(gas_graph_t g, int program, F f) {
+  if (!g) return set_error(GAS_ERR_ARG, "null graph");
+  return guarded([&] {
+    switch (program) {
+      case GAS_PROGRAM_BFS: {
+        std::vector<BFS::VertexData> vd(g->nV, BFS::VertexData{-1});
+        f(cached_engine<BFS>(g, program, vd.data()));
+        static_cast<GASEngineGPU<BFS> *>(g->engine_cache[program])->detachHost();
+        break;
+      }
+      case GAS_PROGRAM_SSSP: {
+        if (!g->csc_w && g->nE > 0)
+          throw Error(GAS_ERR_ARG, "graph has no edge weights");
+        std::vector<int> vd(g->nV, SSSP::gatherZero);
+        f(cached_engine<SSSP>(g, program, vd.data()));
+        static_cast<GASEngineGPU<SSSP> *>(g->engine_cache[program])->detachHost();
+        break;
+      }
+      case GAS_PROGRAM_PAGERANK: {
+        std::vector<PageRank::VertexData> vd(g->nV);
+        for (int32_t v = 0; v < g->nV; ++v)
+          vd[v] = {PageRank::pageConst,
+                   std::max(1, g->h_csr_off[v + 1] - g->h_csr_off[v])};
+        f(cached_engine<PageRank>(g, program, vd.data()));
+        static_cast<GASEngineGPU<PageRank> *>(g->engine_cache[program])
+            ->detachHost();
+        break;
+      }
+      case GAS_PROGRAM_CC: {
+        std::vector<int> vd(g->nV);
+        for (int32_t v = 0; v < g->nV; ++v) vd[v] = v;
+        f(cached_engine<CC>(g, program, vd.data()));
+        static_cast<GASEngineGPU<CC> *>(g->engine_cache[program])->detachHost();
+        break;
+      }
+      default:
+        throw Error(GAS_ERR_ARG, "unknown program");
+    }
+  });
+}
+
+extern "C" {
+
+int gas_graph_set_tiers(gas_graph_t g, int program, const int32_t *tiers) {
+  if (!tiers) return set_error(GAS_ERR_ARG, "gas_graph_set_tiers: null");
+  return with_engine(g, program, [&](auto &e) {
+    e.setThresholds(tiers[0], tiers[1], tiers[2], tiers[3]);
+  });
+}
+
+int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers) {
+  if (!tiers) return set_error(GAS_ERR_ARG, "gas_graph_get_tiers: null");
+  return with_engine(g, program, [&](auto &e) {
+    int t[4];
+    e.thresholds(t);
+    for (int i = 0; i < 4; ++i) tiers[i] = t[i];
+  });
+}
+
+int gas_graph_autotune(gas_graph_t g, int program, int32_t *tiers_out) {
+  return with_engine(g, program, [&](auto &e) {
+    e.autotune(program != GAS_PROGRAM_BFS, true);
+    if (tiers_out) {
+      int t[4];
+      e.thresholds(t);
+      for (int i = 0; i < 4; ++i) tiers_out[i] = t[i];
+    }
+  });
+}
+
 int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
             gas_run_stats *stats) {
   if (!g || !depth_out || source < 0 || source >= g->nV)
@@ -240,7 +315,11 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
-      if (iterations > 0) stats->supersteps = iterations;
+      if (iterations > 0) {
+        stats->supersteps = iterations;
+        stats->gathered_edges = (int64_t)iterations * g->nE;
+        stats->active_total = (int64_t)iterations * g->nV;
+      }
     }
   });
 }
