@@ -144,6 +144,7 @@ typedef struct {
   const int32_t *csr_offsets; /* device, V+1: out-edges                      */
   const int32_t *csr_dsts;    /* device, E                                  */
   const int32_t *csc_weights; /* device, E, or NULL                         */
+  const int32_t *csr_weights; /* device, E, or NULL (same weights, CSR order) */
   size_t device_bytes;
 } gas_graph_view;
 
@@ -212,6 +213,39 @@ int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats);
  * FP64 sum of |new - old| reduced on the GPU. */
 int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
                  double *residual_out, gas_run_stats *stats);
+
+/* ---- batched multi-source queries ------------------------------------------
+ * Up to 64 BFS or SSSP queries on one resident graph, one per lane, sharing
+ * every edge read (no counterpart in the reference, which runs one source
+ * per engine, bfs.cu:91-112, sssp.cu:67-82).  Lanes can be added between
+ * rounds while others are running (continuous batching); a lane is finished
+ * when its bit leaves gas_batch_alive.  Results per lane are identical to
+ * the single-source runs: BFS depth (-1 unreached, at most 254 levels),
+ * SSSP distance (2147383647 unreached). */
+#define GAS_BATCH_BFS   0
+#define GAS_BATCH_SSSP  1
+typedef struct gas_batch *gas_batch_t;
+
+int gas_batch_create(gas_graph_t g, int kind, gas_batch_t *out);
+/* Starts n queries: lanes[i] (0..63, free) runs from sources[i]. */
+int gas_batch_add(gas_batch_t b, const int32_t *lanes, const int32_t *sources,
+                  int32_t n);
+/* Runs `rounds` rounds (BFS levels / SSSP relaxation rounds) of all lanes. */
+int gas_batch_step(gas_batch_t b, int32_t rounds);
+/* Lanes in use whose query still has work. */
+int gas_batch_alive(gas_batch_t b, uint64_t *alive);
+/* Lanes in use (running or finished but not yet released). */
+int gas_batch_used(gas_batch_t b, uint64_t *used);
+int gas_batch_release(gas_batch_t b, uint64_t lanes);
+/* Rounds until every lane in use is finished; stats may be NULL. */
+int gas_batch_run(gas_batch_t b, gas_run_stats *stats);
+/* Result of one lane: V int32 to host memory / to device memory. */
+int gas_batch_result(gas_batch_t b, int32_t lane, int32_t *host_out);
+int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out);
+/* Push-versus-pull switch of batched BFS: pull when the frontier's
+ * out-edges times `divisor` reach |E| (default 1; 0 = never pull). */
+int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
+int gas_batch_free(gas_batch_t b);
 
 /* ---- the engine API, step by step --------------------------------------- */
 #define GAS_PROGRAM_BFS       0  /* VertexData int depth; no EdgeData      */

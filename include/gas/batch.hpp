@@ -1,0 +1,701 @@
+// Batched multi-source traversal: up to 64 BFS or SSSP queries on one
+// resident graph share every edge read.  Lane s of a batch is query s; a
+// wave64 is exactly one vertex's row of lanes.
+//
+//   BFS   per-vertex uint64 masks: visited, frontier (by parity), next;
+//         depth stored as one byte per (lane, vertex) (255 = unreached),
+//         levels up to 254.  A round is either a push from the frontier list
+//         (items of up to ITEM_EDGES out-edges, one wave per item, one lane
+//         per edge, 64-bit atomicOr of the new bits) or a pull over every
+//         vertex still missing lanes (in-edges with early exit once all
+//         missing lanes are found); the direction follows the frontier's
+//         out-edge count.
+//   SSSP  int32 distance lanes dist[v][64] and a per-vertex mask of lanes
+//         that changed in the last round.  A round pushes from the changed
+//         list: one wave per item of out-edges, lane = query, candidate
+//         dist[u][s] + w compared against dist[w][s] and lowered with
+//         atomicMin only where it improves; improved lanes are OR-ed into the
+//         destination's next mask.  Distances only decrease, so reading a
+//         value another wave is lowering is safe: the lowering wave marks
+//         the lane changed and it is pushed again next round.  The fixpoint
+//         is the unique shortest-path distance of each lane, unreached lanes
+//         keep 2147383647 (sssp.cu:32-33).
+//
+// Every round ends with an ordered compaction (two passes, no atomics) of
+// the vertices whose mask is nonzero, which also cuts their out-edges into
+// items.  Lanes are reset and seeded between rounds, so queries can join a
+// running batch (continuous batching) and leave when their lane empties.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace gas {
+
+constexpr int BW = 64;            // lanes per batch (= wave size)
+constexpr int ITEM_EDGES = 64;    // out-edges per push item
+constexpr int SSSP_UNROLL = 8;    // edges per wave trip in the SSSP push
+constexpr int32_t SSSP_INF = INT32_MAX - 100000;  // sssp.cu:32-33
+
+struct BatchCtl {
+  int32_t n_list;       // vertices in the current list
+  int32_t n_items;      // push items over the list
+  int32_t pull;         // BFS: next round pulls
+  int32_t pad;
+  int64_t list_edges;   // summed out-degree of the list
+  uint64_t lanes_alive; // OR of the masks of the current list
+  int64_t stat_edges;   // edges examined so far
+  int64_t stat_rounds;
+};
+
+struct BatchDev {
+  int32_t nV, nE;
+  const int32_t *csc_off, *csc_src, *csr_off, *csr_dst, *csr_w;
+  BatchCtl *ctl;
+  uint64_t *visited, *front[2], *next;  // BFS
+  uint8_t *depth8;                      // BFS [BW][rowV]
+  int32_t rowV;                         // depth row stride (V rounded to 4)
+  int32_t *lane_start;                  // BFS [BW]: round the lane began
+  int32_t *dist;                        // SSSP [V][BW]
+  uint64_t *chg, *chg_next;             // SSSP
+  int32_t *list, *ioff, *iown;          // current list, item offsets/owners
+  int32_t *bsum;                        // [2][SCAN_BLOCKS] count | items
+  int64_t *esum;                        // [SCAN_BLOCKS] out-edges
+  uint64_t *lor;                        // [SCAN_BLOCKS] OR of masks
+  int32_t pull_div;                     // pull when list_edges*div >= E
+                                        // (0: never, SSSP)
+};
+
+__device__ inline uint64_t wave_or(uint64_t x) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)x, o, WAVE);
+    uint32_t hi = __shfl_xor((uint32_t)(x >> 32), o, WAVE);
+    x |= ((uint64_t)hi << 32) | lo;
+  }
+  return x;
+}
+
+__device__ inline uint64_t block_or(uint64_t x, uint64_t *lds) {
+  x = wave_or(x);
+  if (lane_id() == 0) lds[threadIdx.x / WAVE] = x;
+  __syncthreads();
+  uint64_t r = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) r |= lds[i];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline int64_t block_sum64(int64_t x, int64_t *lds) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+  if (lane_id() == 0) lds[threadIdx.x / WAVE] = x;
+  __syncthreads();
+  int64_t r = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) r += lds[i];
+  __syncthreads();
+  return r;
+}
+
+// ---- list construction -------------------------------------------------------
+// Pass 1 over contiguous chunks of vertices with mask array m: per chunk the
+// number of nonzero masks, their push items and out-edges, and the OR.
+__global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
+  __shared__ int lds[NT / WAVE];
+  __shared__ int64_t lds64[NT / WAVE];
+  __shared__ uint64_t ldsu[NT / WAVE];
+  int b, e;
+  scan_chunk(d.nV, blockIdx.x, b, e);
+  int cnt = 0, items = 0;
+  int64_t edges = 0;
+  uint64_t any = 0;
+  for (int v = b + threadIdx.x; v < e; v += NT) {
+    const uint64_t x = m[v];
+    if (x) {
+      const int deg = d.csr_off[v + 1] - d.csr_off[v];
+      ++cnt;
+      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      edges += deg;
+      any |= x;
+    }
+  }
+  cnt = block_sum(cnt, lds);
+  items = block_sum(items, lds);
+  edges = block_sum64(edges, lds64);
+  any = block_or(any, ldsu);
+  if (threadIdx.x == 0) {
+    d.bsum[blockIdx.x] = cnt;
+    d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
+    d.esum[blockIdx.x] = edges;
+    d.lor[blockIdx.x] = any;
+  }
+}
+
+// Pass 2: ordered list of vertices with nonzero masks, item offsets and the
+// owner of every item; workgroup 0 publishes totals and the direction.
+__global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m) {
+  __shared__ int lds[NT / WAVE];
+  __shared__ int64_t lds64[NT / WAVE];
+  __shared__ uint64_t ldsu[NT / WAVE];
+  int b, e, pre, total, ipre, itotal;
+  scan_chunk(d.nV, blockIdx.x, b, e);
+  chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
+  chunk_prefix(d.bsum + SCAN_BLOCKS, blockIdx.x, ipre, itotal, lds);
+  // every workgroup derives the direction from the edge totals; a round
+  // that pulls needs no list
+  int64_t ed = 0;
+  for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) ed += d.esum[i];
+  ed = block_sum64(ed, lds64);
+  const bool pull = d.pull_div > 0 && ed * (int64_t)d.pull_div >= (int64_t)d.nE;
+  int carry = pre, icarry = ipre;
+  for (int base = pull ? e : b; base < e; base += NT) {
+    const int v = base + threadIdx.x;
+    const bool on = v < e && m[v] != 0;
+    int nit = 0;
+    if (on) {
+      const int deg = d.csr_off[v + 1] - d.csr_off[v];
+      nit = (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+    }
+    int tot, itot;
+    const int pos = carry + block_excl_scan(on ? 1 : 0, tot, lds);
+    const int ipos = icarry + block_excl_scan(nit, itot, lds);
+    if (on) {
+      d.list[pos] = v;
+      d.ioff[pos] = ipos;
+      for (int k = 0; k < nit; ++k) d.iown[ipos + k] = pos;
+    }
+    carry += tot;
+    icarry += itot;
+  }
+  if (blockIdx.x == 0) {
+    uint64_t any = 0;
+    for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) any |= d.lor[i];
+    any = block_or(any, ldsu);
+    if (threadIdx.x == 0) {
+      BatchCtl *c = d.ctl;
+      c->n_list = total;
+      c->n_items = itotal;
+      c->list_edges = ed;
+      c->lanes_alive = any;
+      c->pull = pull ? 1 : 0;
+      d.ioff[total] = itotal;
+    }
+  }
+}
+
+// ---- BFS ---------------------------------------------------------------------
+// Clears lanes `reset` (visited bits, frontier bits of parity p, depth rows).
+__global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
+                                                   uint64_t reset) {
+  const int stride = gridDim.x * NT;
+  for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += stride) {
+    d.visited[v] &= ~reset;
+    d.front[p][v] &= ~reset;
+  }
+  // depth rows of the reset lanes, 4 bytes per thread
+  uint32_t *rows = reinterpret_cast<uint32_t *>(d.depth8);
+  const int wpr = d.rowV / 4;
+  for (int s = 0; s < BW; ++s) {
+    if (!((reset >> s) & 1ull)) continue;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < wpr; i += stride)
+      rows[(size_t)s * wpr + i] = 0xffffffffu;
+  }
+}
+
+// One thread per new lane: seed its source.
+__global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
+                            const int32_t *sources, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = lanes[i], v = sources[i];
+  const uint64_t bit = 1ull << s;
+  atomicOr((unsigned long long *)&d.visited[v], (unsigned long long)bit);
+  atomicOr((unsigned long long *)&d.front[p][v], (unsigned long long)bit);
+  d.depth8[(size_t)s * d.rowV + v] = 0;
+}
+
+// Push: one wave per item, one lane per out-edge.
+__global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
+                                                  uint64_t used) {
+  const BatchCtl *c = d.ctl;
+  if (c->pull) return;
+  const int n_items = c->n_items;
+  const int lane = lane_id();
+  const int waves = gridDim.x * (NT / WAVE);
+  for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < n_items;
+       it += waves) {
+    const int li = d.iown[it];
+    const int u = d.list[li];
+    const uint64_t f = d.front[p][u];
+    const int k = it - d.ioff[li];
+    const int e0 = d.csr_off[u] + k * ITEM_EDGES;
+    const int e1 = min(e0 + ITEM_EDGES, d.csr_off[u + 1]);
+    const int ei = e0 + lane;
+    if (ei < e1) {
+      const int w = d.csr_dst[ei];
+      const uint64_t nb = f & used & ~d.visited[w];
+      if (nb)
+        atomicOr((unsigned long long *)&d.next[w], (unsigned long long)nb);
+    }
+  }
+}
+
+// Pull: every vertex missing some used lane ORs its in-neighbours'
+// frontier masks until all missing lanes are found (thread per vertex up to
+// 32 in-edges, wave per vertex above).
+__global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
+                                                  uint64_t used) {
+  const BatchCtl *c = d.ctl;
+  if (!c->pull) return;
+  used &= c->lanes_alive;  // a finished lane cannot reach more vertices
+  const uint64_t *fr = d.front[p];
+  const int lane = lane_id();
+  const int stride = gridDim.x * NT;
+  const int rounds = (d.nV + stride - 1) / stride;
+  for (int r = 0; r < rounds; ++r) {
+    const int v = r * stride + blockIdx.x * NT + threadIdx.x;
+    uint64_t need = 0;
+    int o = 0, deg = 0;
+    if (v < d.nV) {
+      need = used & ~d.visited[v];
+      if (need) {
+        o = d.csc_off[v];
+        deg = d.csc_off[v + 1] - o;
+      }
+    }
+    const bool wide = need && deg > 32;
+    if (need && !wide) {
+      uint64_t acc = 0;
+      for (int k = 0; k < deg; ++k) {
+        acc |= fr[d.csc_src[o + k]];
+        if ((acc & need) == need) break;
+      }
+      if (acc & need) d.next[v] = acc & need;
+    }
+    uint64_t m = __ballot(wide);
+    while (m) {
+      const int j = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int oj = __shfl(o, j, WAVE);
+      const int dj = __shfl(deg, j, WAVE);
+      const uint64_t nj =
+          ((uint64_t)__shfl((uint32_t)(need >> 32), j, WAVE) << 32) |
+          __shfl((uint32_t)need, j, WAVE);
+      uint64_t acc = 0;
+      for (int base = 0; base < dj; base += WAVE) {
+        const int k = base + lane;
+        uint64_t x = (k < dj) ? fr[d.csc_src[oj + k]] : 0ull;
+        acc |= wave_or(x);
+        if ((acc & nj) == nj) break;
+      }
+      if (lane == j && (acc & nj)) d.next[v] = acc & nj;
+    }
+  }
+}
+
+// New bits become visited and form the next frontier (parity p^1); depth
+// rows get round + 1 - lane_start.  Contiguous chunks, with the list
+// counters of the new frontier (pass 1 of kb_list).
+__global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
+                                                      uint64_t used,
+                                                      int round) {
+  __shared__ int lds[NT / WAVE];
+  __shared__ int64_t lds64[NT / WAVE];
+  __shared__ uint64_t ldsu[NT / WAVE];
+  __shared__ int32_t s_start[BW];
+  if (threadIdx.x < BW) s_start[threadIdx.x] = d.lane_start[threadIdx.x];
+  __syncthreads();
+  uint64_t *nf = d.front[p ^ 1];
+  int b, e;
+  scan_chunk(d.nV, blockIdx.x, b, e);
+  int cnt = 0, items = 0;
+  int64_t edges = 0;
+  uint64_t any = 0;
+  for (int v = b + threadIdx.x; v < e; v += NT) {
+    const uint64_t n = d.next[v];
+    uint64_t nv = 0;
+    if (n) {
+      d.next[v] = 0;
+      nv = n & used & ~d.visited[v];
+    }
+    nf[v] = nv;
+    if (nv) {
+      d.visited[v] |= nv;
+      uint64_t x = nv;
+      while (x) {
+        const int s = __ffsll((unsigned long long)x) - 1;
+        x &= x - 1;
+        d.depth8[(size_t)s * d.rowV + v] = (uint8_t)(round + 1 - s_start[s]);
+      }
+      const int deg = d.csr_off[v + 1] - d.csr_off[v];
+      ++cnt;
+      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      edges += deg;
+      any |= nv;
+    }
+  }
+  cnt = block_sum(cnt, lds);
+  items = block_sum(items, lds);
+  edges = block_sum64(edges, lds64);
+  any = block_or(any, ldsu);
+  if (threadIdx.x == 0) {
+    d.bsum[blockIdx.x] = cnt;
+    d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
+    d.esum[blockIdx.x] = edges;
+    d.lor[blockIdx.x] = any;
+  }
+}
+
+// Depth of lane s as int32 (-1 unreached), for the caller.
+__global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
+                                                     int32_t *out) {
+  const uint8_t *row = d.depth8 + (size_t)s * d.rowV;
+  for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
+    const uint8_t x = row[v];
+    out[v] = x == 255 ? -1 : (int32_t)x;
+  }
+}
+
+// ---- SSSP --------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void kb_sssp_reset(BatchDev d,
+                                                    uint64_t reset) {
+  const int64_t total = (int64_t)d.nV * BW;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < total;
+       i += stride) {
+    const int s = (int)(i & (BW - 1));
+    if ((reset >> s) & 1ull) d.dist[i] = SSSP_INF;
+    if (s == 0) d.chg[i / BW] &= ~reset;
+  }
+}
+
+__global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
+                             const int32_t *sources, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = lanes[i], v = sources[i];
+  d.dist[(size_t)v * BW + s] = 0;
+  atomicOr((unsigned long long *)&d.chg[v], (unsigned long long)(1ull << s));
+}
+
+// Push from the changed list: one wave per item, lane = query.  The item's
+// edges are loaded lane-parallel once and broadcast by shuffles; each trip
+// issues SSSP_UNROLL distance-row loads before any compare.  A lane is
+// marked changed in the destination whenever its candidate beats the value
+// it read: a superset of the lanes actually lowered (the read may be older
+// than a concurrent atomicMin), which can only cause a redundant push.
+__global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
+  const BatchCtl *c = d.ctl;
+  const int n_items = c->n_items;
+  const int lane = lane_id();
+  const int waves = gridDim.x * (NT / WAVE);
+  for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < n_items;
+       it += waves) {
+    const int li = d.iown[it];
+    const int u = d.list[li];
+    const uint64_t m = d.chg[u];
+    const bool act = (m >> lane) & 1ull;
+    const int du = d.dist[(size_t)u * BW + lane];
+    const int k = it - d.ioff[li];
+    const int e0 = d.csr_off[u] + k * ITEM_EDGES;
+    const int ne = min(ITEM_EDGES, d.csr_off[u + 1] - e0);
+    int my_w = 0, my_wt = 0;
+    if (lane < ne) {
+      my_w = d.csr_dst[e0 + lane];
+      my_wt = d.csr_w[e0 + lane];
+    }
+    for (int j0 = 0; j0 < ne; j0 += SSSP_UNROLL) {
+      int w[SSSP_UNROLL], cand[SSSP_UNROLL], cur[SSSP_UNROLL];
+#pragma unroll
+      for (int x = 0; x < SSSP_UNROLL; ++x) {
+        w[x] = __shfl(my_w, j0 + x, WAVE);
+        cand[x] = du + __shfl(my_wt, j0 + x, WAVE);
+      }
+#pragma unroll
+      for (int x = 0; x < SSSP_UNROLL; ++x)
+        cur[x] = (act && j0 + x < ne) ? d.dist[(size_t)w[x] * BW + lane]
+                                      : SSSP_INF;
+#pragma unroll
+      for (int x = 0; x < SSSP_UNROLL; ++x) {
+        const bool imp = act && j0 + x < ne && cand[x] < cur[x];
+        if (imp) atomicMin(&d.dist[(size_t)w[x] * BW + lane], cand[x]);
+        const uint64_t bm = __ballot(imp);
+        if (bm && lane == 0)
+          atomicOr((unsigned long long *)&d.chg_next[w[x]],
+                   (unsigned long long)bm);
+      }
+    }
+  }
+}
+
+// chg <- chg_next, chg_next <- 0, with the list counters (pass 1 of kb_list).
+__global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
+  __shared__ int lds[NT / WAVE];
+  __shared__ int64_t lds64[NT / WAVE];
+  __shared__ uint64_t ldsu[NT / WAVE];
+  int b, e;
+  scan_chunk(d.nV, blockIdx.x, b, e);
+  int cnt = 0, items = 0;
+  int64_t edges = 0;
+  uint64_t any = 0;
+  for (int v = b + threadIdx.x; v < e; v += NT) {
+    const uint64_t x = d.chg_next[v];
+    if (x) d.chg_next[v] = 0;
+    d.chg[v] = x;
+    if (x) {
+      const int deg = d.csr_off[v + 1] - d.csr_off[v];
+      ++cnt;
+      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      edges += deg;
+      any |= x;
+    }
+  }
+  cnt = block_sum(cnt, lds);
+  items = block_sum(items, lds);
+  edges = block_sum64(edges, lds64);
+  any = block_or(any, ldsu);
+  if (threadIdx.x == 0) {
+    d.bsum[blockIdx.x] = cnt;
+    d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
+    d.esum[blockIdx.x] = edges;
+    d.lor[blockIdx.x] = any;
+  }
+}
+
+__global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
+                                                      int32_t *out) {
+  for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
+    out[v] = d.dist[(size_t)v * BW + s];
+}
+
+// ---- host side -----------------------------------------------------------------
+// One batch of up to 64 lanes of one kind on a resident graph.  Lanes are
+// added between rounds (reset + seed + list rebuild) and report themselves
+// finished through lanesAlive().
+class BatchEngine {
+ public:
+  enum Kind { BFS_KIND = 0, SSSP_KIND = 1 };
+
+  BatchEngine(gas_graph_t g, Kind kind) : m_kind(kind) {
+    check(gasrt_stream_create(&m_stream), "stream");
+    check(gasrt_host_alloc((void **)&m_hctl, sizeof(BatchCtl) * 2), "pinned");
+    gas_graph_view gv;
+    check(gas_graph_view_get(g, &gv), "graph view");
+    if (kind == SSSP_KIND && !gv.csr_weights && gv.n_edges)
+      throw Error(GAS_ERR_ARG, "batched SSSP needs edge weights");
+    const size_t V = (size_t)gv.n_vertices;
+    m_nV = gv.n_vertices;
+    m_nE = gv.n_edges;
+    BatchDev &d = m_dev;
+    d.nV = gv.n_vertices;
+    d.nE = gv.n_edges;
+    d.csc_off = gv.csc_offsets;
+    d.csc_src = gv.csc_srcs;
+    d.csr_off = gv.csr_offsets;
+    d.csr_dst = gv.csr_dsts;
+    d.csr_w = gv.csr_weights;
+    d.pull_div = kind == BFS_KIND ? 1 : 0;
+    m_ctl.alloc(1);
+    m_list.alloc(V + 1);
+    m_ioff.alloc(V + 1);
+    m_iown.alloc((size_t)gv.n_edges / ITEM_EDGES + V + 1);
+    m_bsum.alloc(2 * SCAN_BLOCKS);
+    m_esum.alloc(SCAN_BLOCKS);
+    m_lor.alloc(SCAN_BLOCKS);
+    m_lanes.alloc(BW);
+    m_sources.alloc(BW);
+    if (kind == BFS_KIND) {
+      d.rowV = (int32_t)((V + 3) / 4 * 4);
+      m_visited.alloc(V);
+      m_front0.alloc(V);
+      m_front1.alloc(V);
+      m_next.alloc(V);
+      m_depth.alloc((size_t)d.rowV * BW);
+      m_lane_start.alloc(BW);
+      check(gasrt_memset(m_visited.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_front0.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_front1.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_next.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW, m_stream),
+            "memset");
+      check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
+      d.visited = m_visited.p;
+      d.front[0] = m_front0.p;
+      d.front[1] = m_front1.p;
+      d.next = m_next.p;
+      d.depth8 = m_depth.p;
+      d.lane_start = m_lane_start.p;
+    } else {
+      m_dist.alloc(V * BW);
+      m_chg.alloc(V);
+      m_chg_next.alloc(V);
+      check(gasrt_memset(m_chg.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_chg_next.p, 0, V * 8, m_stream), "memset");
+      d.dist = m_dist.p;
+      d.chg = m_chg.p;
+      d.chg_next = m_chg_next.p;
+    }
+    d.ctl = m_ctl.p;
+    d.list = m_list.p;
+    d.ioff = m_ioff.p;
+    d.iown = m_iown.p;
+    d.bsum = m_bsum.p;
+    d.esum = m_esum.p;
+    d.lor = m_lor.p;
+    BatchCtl c{};
+    check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(c), m_stream), "ctl");
+    check(gasrt_stream_synchronize(m_stream), "sync");
+  }
+
+  ~BatchEngine() {
+    if (m_stream) gasrt_stream_synchronize(m_stream);
+    gasrt_host_free(m_hctl);
+    gasrt_stream_destroy(m_stream);
+  }
+
+  Kind kind() const { return m_kind; }
+  gasrt_stream_t stream() const { return m_stream; }
+  uint64_t lanesUsed() const { return m_used; }
+  int round() const { return m_round; }
+  void setPullDivisor(int div) {
+    if (m_kind == BFS_KIND) m_dev.pull_div = div;
+  }
+
+  // Starts queries: lane[i] runs from source[i].  The lanes must be free.
+  void addLanes(const int32_t *lanes, const int32_t *sources, int n) {
+    if (n <= 0) return;
+    if (n > BW) throw Error(GAS_ERR_ARG, "more than 64 lanes");
+    uint64_t reset = 0;
+    for (int i = 0; i < n; ++i) {
+      if (lanes[i] < 0 || lanes[i] >= BW || sources[i] < 0 ||
+          sources[i] >= m_nV)
+        throw Error(GAS_ERR_ARG, "lane or source out of range");
+      const uint64_t bit = 1ull << lanes[i];
+      if ((m_used | reset) & bit) throw Error(GAS_ERR_STATE, "lane in use");
+      reset |= bit;
+    }
+    check(gasrt_memcpy_h2d(m_lanes.p, lanes, 4 * (size_t)n, m_stream), "lanes");
+    check(gasrt_memcpy_h2d(m_sources.p, sources, 4 * (size_t)n, m_stream),
+          "sources");
+    const unsigned grid = gridFor(m_nV);
+    if (m_kind == BFS_KIND) {
+      for (int i = 0; i < n; ++i) m_hstart[lanes[i]] = m_round;
+      check(gasrt_memcpy_h2d(m_lane_start.p, m_hstart, sizeof(m_hstart),
+                             m_stream),
+            "lane start");
+      launch(kb_bfs_reset, grid, m_stream, m_dev, m_par, reset);
+      launch(kb_bfs_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
+      m_used |= reset;
+      launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.front[m_par]);
+      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.front[m_par]);
+    } else {
+      launch(kb_sssp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, reset);
+      launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n);
+      m_used |= reset;
+      launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.chg);
+      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.chg);
+    }
+  }
+
+  // Frees lanes (their results stay readable until the lane is reused).
+  void releaseLanes(uint64_t lanes) { m_used &= ~lanes; }
+
+  // One round for every used lane.
+  void step() {
+    if (m_kind == BFS_KIND) {
+      for (int s = 0; s < BW; ++s)
+        if (((m_used >> s) & 1ull) && m_round + 1 - m_hstart[s] > 254)
+          throw Error(GAS_ERR_STATE, "batched BFS deeper than 254 levels");
+      launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
+      launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
+      launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used,
+             m_round);
+      m_par ^= 1;
+      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.front[m_par]);
+    } else {
+      launch(kb_sssp_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
+      launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
+      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+             (const uint64_t *)m_dev.chg);
+    }
+    ++m_round;
+  }
+
+  // Lanes whose query still has work (reads the device; synchronises).
+  uint64_t lanesAlive() {
+    check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(BatchCtl), m_stream),
+          "ctl");
+    return m_hctl[0].lanes_alive & m_used;
+  }
+
+  BatchCtl readCtl() {
+    check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(BatchCtl), m_stream),
+          "ctl");
+    return m_hctl[0];
+  }
+
+  // Rounds until every used lane is finished.  BFS levels of a batch that
+  // started together are at most ~V, so the loop is bounded by the graph.
+  int runToCompletion() {
+    int rounds = 0;
+    while (lanesAlive()) {
+      step();
+      ++rounds;
+    }
+    return rounds;
+  }
+
+  // Result of a lane into a device buffer of V int32 (BFS depth, -1
+  // unreached; SSSP distance, 2147383647 unreached).
+  void laneResultDevice(int lane, int32_t *dev_out) {
+    if (lane < 0 || lane >= BW) throw Error(GAS_ERR_ARG, "lane out of range");
+    if (m_kind == BFS_KIND)
+      launch(kb_bfs_extract, gridFor(m_nV), m_stream, m_dev, lane, dev_out);
+    else
+      launch(kb_sssp_extract, gridFor(m_nV), m_stream, m_dev, lane, dev_out);
+  }
+
+  void laneResult(int lane, int32_t *host_out) {
+    if (m_tmp.n < (size_t)m_nV) m_tmp.alloc((size_t)m_nV);
+    laneResultDevice(lane, m_tmp.p);
+    check(gasrt_memcpy_d2h(host_out, m_tmp.p, 4 * (size_t)m_nV, m_stream),
+          "lane result");
+  }
+
+  void sync() { check(gasrt_stream_synchronize(m_stream), "sync"); }
+
+ private:
+  static unsigned gridFor(int64_t n) {
+    return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + NT - 1) / NT));
+  }
+
+  Kind m_kind;
+  gasrt_stream_t m_stream = nullptr;
+  BatchCtl *m_hctl = nullptr;
+  int32_t m_nV = 0, m_nE = 0;
+  int m_par = 0;
+  int m_round = 0;
+  uint64_t m_used = 0;
+  int32_t m_hstart[BW] = {};
+  BatchDev m_dev{};
+  DevBuf<BatchCtl> m_ctl;
+  DevBuf<int32_t> m_list, m_ioff, m_iown, m_bsum, m_lanes, m_sources,
+      m_lane_start, m_dist, m_tmp;
+  DevBuf<int64_t> m_esum;
+  DevBuf<uint64_t> m_lor, m_visited, m_front0, m_front1, m_next, m_chg,
+      m_chg_next;
+  DevBuf<uint8_t> m_depth;
+};
+
+}  // namespace gas

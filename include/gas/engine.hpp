@@ -277,10 +277,10 @@ class GASEngineGPU {
   // Degree tiers of the gather (g) and the scatter (s):
   // thread <= t1 < wave <= t2 < merge-path hub.
   void setThresholds(int gt1, int gt2, int st1, int st2) {
-    if (gt1 < 0 || gt1 > T1_MAX || gt2 < gt1 || st1 < 0 || st1 > T1_MAX ||
-        st2 < st1)
+    if (gt1 < 0 || gt1 > T1_MAX || gt2 < gt1 || gt2 < HUB_MIN_T2 ||
+        st1 < 0 || st1 > T1_MAX || st2 < st1 || st2 < HUB_MIN_T2)
       throw Error(GAS_ERR_ARG,
-                  "setThresholds: need 0 <= t1 <= 16 and t1 <= t2");
+                  "setThresholds: need 0 <= t1 <= 16, t1 <= t2, t2 >= 4");
     m_gt1 = gt1;
     m_gt2 = gt2;
     m_st1 = st1;
@@ -407,7 +407,7 @@ class GASEngineGPU {
       launch(k_hscan1<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
       launch(k_hscan2<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
       launch(k_hgather<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hfix<Program>, 64, m_stream, m_dev, m_par);
+      launch(k_hfix<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
     }
     timedEnd(0);
   }

@@ -49,11 +49,14 @@ constexpr int WAVE = 64;
 constexpr int SCAN_BLOCKS = 1024;  // chunks of per-vertex kernels and scans
 constexpr int MP_VT = 8;           // merge-path items per thread
 constexpr int MP_NV = NT * MP_VT;  // merge-path items per tile
-constexpr int MP_BLOCKS = 1024;    // grid of the merge-path kernels
+constexpr int MP_BLOCKS = 2048;    // grid of the merge-path kernels
+constexpr int HUB_MIN_T2 = 4;      // hubs have degree > t2 >= 4 ...
+// ... so a segment spans >= 6 merge-path items and a tile touches at most
+constexpr int HUB_SEG_MAX = MP_NV / (HUB_MIN_T2 + 2) + 2;  // segments
 constexpr int T1_MAX = 16;         // largest thread tier (unrolled loads)
 constexpr int MID_UNROLL = 8;      // wave tier: edges per lane per trip
-constexpr int DEFAULT_T1 = 16;     // thread tier: degree <= t1
-constexpr int DEFAULT_T2 = 512;    // wave tier: degree <= t2; above: hubs
+constexpr int DEFAULT_T1 = 8;      // thread tier: degree <= t1
+constexpr int DEFAULT_T2 = 8;      // wave tier: degree <= t2; above: hubs
 
 // ---------------------------------------------------------------------------
 // Superstep number visible to vertex programs.  The engine's apply kernel
@@ -133,6 +136,12 @@ struct Acc {
     r.ok = __shfl_xor(ok, m, WAVE);
     return r;
   }
+  __device__ Acc up_peer(int o) const {
+    Acc r;
+    r.v = __builtin_bit_cast(T, __shfl_up(__builtin_bit_cast(int, v), o, WAVE));
+    r.ok = __shfl_up(ok, o, WAVE);
+    return r;
+  }
   __device__ T finish() const {
     T z = P::gatherZero;
     return ok ? P::gatherReduce(z, v) : z;
@@ -188,6 +197,13 @@ struct Acc<P, true> {
     r.ok = __shfl_xor(ok, m, WAVE);
     return r;
   }
+  __device__ Acc up_peer(int o) const {
+    Acc r;
+    r.s = __shfl_up(s, o, WAVE);
+    r.c = __shfl_up(c, o, WAVE);
+    r.ok = __shfl_up(ok, o, WAVE);
+    return r;
+  }
   __device__ T finish() const {
     T z = P::gatherZero;
     return ok ? P::gatherReduce(z, s + c) : z;
@@ -207,6 +223,44 @@ __device__ inline SegItem<P> seg_combine(const SegItem<P> &x,
   SegItem<P> r = x;
   r.a.merge(y.a);
   return r;
+}
+
+template <class P>
+__device__ inline SegItem<P> seg_empty() {
+  SegItem<P> r;
+  r.a = Acc<P>::empty();
+  r.flag = 0;
+  return r;
+}
+
+// Segmented scan over the workgroup in thread order: returns the exclusive
+// prefix of (segment end seen, partial) for this thread and leaves the
+// workgroup's inclusive total in `total` (all threads).  Wave-level shuffle
+// scan, then one LDS exchange of the NT/WAVE wave totals.
+template <class P>
+__device__ inline SegItem<P> block_seg_scan(SegItem<P> me, SegItem<P> &total,
+                                            SegItem<P> *s_wtot) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    SegItem<P> up;
+    up.a = me.a.up_peer(o);
+    up.flag = __shfl_up(me.flag, o, WAVE);
+    if (lane >= o) me = seg_combine(up, me);
+  }
+  SegItem<P> prev;
+  prev.a = me.a.up_peer(1);
+  prev.flag = __shfl_up(me.flag, 1, WAVE);
+  if (lane == WAVE - 1) s_wtot[w] = me;
+  __syncthreads();
+  SegItem<P> wpre = seg_empty<P>();
+  for (int k = 0; k < w; ++k) wpre = seg_combine(wpre, s_wtot[k]);
+  SegItem<P> tot = seg_empty<P>();
+#pragma unroll
+  for (int k = 0; k < NT / WAVE; ++k) tot = seg_combine(tot, s_wtot[k]);
+  total = tot;
+  return lane == 0 ? wpre : seg_combine(wpre, prev);
 }
 
 // Device view of one engine.
@@ -594,11 +648,11 @@ __global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
 template <class P>
 __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
-  __shared__ int32_t s_end[MP_NV + 1];
-  __shared__ int32_t s_base[MP_NV + 1];
-  __shared__ int32_t s_vtx[MP_NV + 1];
-  __shared__ int32_t s_slot[MP_NV + 1];
-  __shared__ SegItem<P> s_seg[NT];
+  __shared__ int32_t s_end[HUB_SEG_MAX];
+  __shared__ int32_t s_base[HUB_SEG_MAX];
+  __shared__ int32_t s_vtx[HUB_SEG_MAX];
+  __shared__ int32_t s_slot[HUB_SEG_MAX];
+  __shared__ SegItem<P> s_wtot[NT / WAVE];
 
   const Ctl *c = d.ctl;
   const int A = c->n_active[p];
@@ -681,25 +735,12 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
         acc = Acc<P>::empty();
       }
     }
-    // block-wide segmented scan of (end seen, partial)
+    // workgroup-wide segmented scan of (end seen, partial)
     SegItem<P> me;
     me.a = acc;
     me.flag = nend > 0;
-    for (int o = 1; o < NT; o <<= 1) {
-      s_seg[tid] = me;
-      __syncthreads();
-      if (tid >= o) me = seg_combine(s_seg[tid - o], me);
-      __syncthreads();
-    }
-    s_seg[tid] = me;  // inclusive
-    __syncthreads();
-    SegItem<P> carry;
-    if (tid > 0) {
-      carry = s_seg[tid - 1];
-    } else {
-      carry.a = Acc<P>::empty();
-      carry.flag = 0;
-    }
+    SegItem<P> total;
+    const SegItem<P> carry = block_seg_scan(me, total, s_wtot);
     if (nend > 0) {
       Acc<P> first = carry.a;
       first.merge(head);
@@ -707,10 +748,10 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
       if (!carry.flag) d.part_first[t] = first;
     }
     if (tid == NT - 1) {
-      if (me.flag)
-        d.part_last[t] = me.a;
+      if (total.flag)
+        d.part_last[t] = total.a;
       else
-        d.part_first[t] = me.a;
+        d.part_first[t] = total.a;
     }
     __syncthreads();
   }
@@ -867,8 +908,8 @@ __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
 // Hub scatter: merge-path tiles over the scatter hubs' CSR out-edges.
 template <class P>
 __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
-  __shared__ int32_t s_end[MP_NV + 1];
-  __shared__ int32_t s_base[MP_NV + 1];
+  __shared__ int32_t s_end[HUB_SEG_MAX];
+  __shared__ int32_t s_base[HUB_SEG_MAX];
   const Ctl *c = d.ctl;
   const int A = c->n_active[p];
   if (A == 0) return;

@@ -39,7 +39,8 @@ class GraphView(C.Structure):
     _fields_ = [("n_vertices", C.c_int32), ("n_edges", C.c_int32),
                 ("csc_offsets", C.c_void_p), ("csc_srcs", C.c_void_p),
                 ("csr_offsets", C.c_void_p), ("csr_dsts", C.c_void_p),
-                ("csc_weights", C.c_void_p), ("device_bytes", C.c_size_t)]
+                ("csc_weights", C.c_void_p), ("csr_weights", C.c_void_p),
+                ("device_bytes", C.c_size_t)]
 
 
 class RunStats(C.Structure):
@@ -346,3 +347,55 @@ class Engine:
         st = RunStats()
         _check(lib().gas_engine_stats(self._h, C.byref(st)))
         return st
+
+
+BATCH_BFS, BATCH_SSSP = 0, 1
+
+
+class Batch:
+    """Up to 64 BFS or SSSP queries sharing edge reads (gas_batch_t)."""
+
+    def __init__(self, graph: Graph, kind: int):
+        h = C.c_void_p()
+        _check(lib().gas_batch_create(graph.handle, C.c_int(kind), C.byref(h)))
+        self._h = h
+        self.graph = graph  # keeps the resident graph alive
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gas_batch_free(self._h)
+            self._h = None
+
+    def add(self, lanes, sources) -> None:
+        la, so = _i32(lanes), _i32(sources)
+        _check(lib().gas_batch_add(self._h, _ptr(la), _ptr(so), C.c_int32(len(la))))
+
+    def step(self, rounds: int = 1) -> None:
+        _check(lib().gas_batch_step(self._h, C.c_int32(rounds)))
+
+    def alive(self) -> int:
+        x = C.c_uint64(0)
+        _check(lib().gas_batch_alive(self._h, C.byref(x)))
+        return x.value
+
+    def used(self) -> int:
+        x = C.c_uint64(0)
+        _check(lib().gas_batch_used(self._h, C.byref(x)))
+        return x.value
+
+    def release(self, lanes_mask: int) -> None:
+        _check(lib().gas_batch_release(self._h, C.c_uint64(lanes_mask)))
+
+    def run(self) -> RunStats:
+        st = RunStats()
+        _check(lib().gas_batch_run(self._h, C.byref(st)))
+        return st
+
+    def result(self, lane: int) -> np.ndarray:
+        out = np.zeros(self.graph.n_vertices, np.int32)
+        _check(lib().gas_batch_result(self._h, C.c_int32(lane), _ptr(out)))
+        return out
+
+    def set_pull_divisor(self, divisor: int) -> None:
+        _check(lib().gas_batch_set_pull_divisor(self._h, C.c_int32(divisor)))

@@ -1,0 +1,134 @@
+// C ABI of the batched multi-source engine (gas_batch_*).
+#include <hip/hip_runtime.h>
+
+#include <memory>
+
+#include "gas.h"
+#include "gas/batch.hpp"
+#include "gas_internal.hpp"
+
+using namespace gas;
+
+struct gas_batch {
+  std::unique_ptr<BatchEngine> e;
+};
+
+namespace {
+
+template <class F>
+int guarded_b(F f) {
+  try {
+    clear_error();
+    f();
+    return GAS_OK;
+  } catch (const Error &x) {
+    return set_error(x.code ? x.code : GAS_ERR_HIP, "%s", x.what());
+  } catch (const std::bad_alloc &) {
+    return set_error(GAS_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &x) {
+    return set_error(GAS_ERR_STATE, "%s", x.what());
+  }
+}
+
+}  // namespace
+
+#define BATCH_OR_FAIL(b)                                           \
+  if (!(b) || !(b)->e)                                             \
+  return set_error(GAS_ERR_ARG, "%s: null batch", __func__)
+
+extern "C" {
+
+int gas_batch_create(gas_graph_t g, int kind, gas_batch_t *out) {
+  if (!g || !out || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_batch_create: bad arguments");
+  return guarded_b([&] {
+    std::unique_ptr<gas_batch> b(new gas_batch);
+    b->e.reset(new BatchEngine(g, kind == GAS_BATCH_BFS
+                                      ? BatchEngine::BFS_KIND
+                                      : BatchEngine::SSSP_KIND));
+    *out = b.release();
+  });
+}
+
+int gas_batch_add(gas_batch_t b, const int32_t *lanes, const int32_t *sources,
+                  int32_t n) {
+  BATCH_OR_FAIL(b);
+  if (n < 0 || (n > 0 && (!lanes || !sources)))
+    return set_error(GAS_ERR_ARG, "gas_batch_add: bad arguments");
+  return guarded_b([&] { b->e->addLanes(lanes, sources, n); });
+}
+
+int gas_batch_step(gas_batch_t b, int32_t rounds) {
+  BATCH_OR_FAIL(b);
+  return guarded_b([&] {
+    for (int i = 0; i < rounds; ++i) b->e->step();
+  });
+}
+
+int gas_batch_alive(gas_batch_t b, uint64_t *alive) {
+  BATCH_OR_FAIL(b);
+  if (!alive) return set_error(GAS_ERR_ARG, "gas_batch_alive: null");
+  return guarded_b([&] { *alive = b->e->lanesAlive(); });
+}
+
+int gas_batch_used(gas_batch_t b, uint64_t *used) {
+  BATCH_OR_FAIL(b);
+  if (!used) return set_error(GAS_ERR_ARG, "gas_batch_used: null");
+  *used = b->e->lanesUsed();
+  return GAS_OK;
+}
+
+int gas_batch_release(gas_batch_t b, uint64_t lanes) {
+  BATCH_OR_FAIL(b);
+  b->e->releaseLanes(lanes);
+  return GAS_OK;
+}
+
+int gas_batch_run(gas_batch_t b, gas_run_stats *stats) {
+  BATCH_OR_FAIL(b);
+  return guarded_b([&] {
+    gasrt_event_t a = nullptr, z = nullptr;
+    check(gasrt_event_create(&a), "event");
+    check(gasrt_event_create(&z), "event");
+    check(gasrt_event_record(a, b->e->stream()), "event");
+    const int rounds = b->e->runToCompletion();
+    check(gasrt_event_record(z, b->e->stream()), "event");
+    check(gasrt_event_synchronize(z), "event");
+    float ms = 0;
+    check(gasrt_event_elapsed_ms(a, z, &ms), "elapsed");
+    gasrt_event_destroy(a);
+    gasrt_event_destroy(z);
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->supersteps = rounds;
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_batch_result(gas_batch_t b, int32_t lane, int32_t *host_out) {
+  BATCH_OR_FAIL(b);
+  if (!host_out) return set_error(GAS_ERR_ARG, "gas_batch_result: null");
+  return guarded_b([&] { b->e->laneResult(lane, host_out); });
+}
+
+int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out) {
+  BATCH_OR_FAIL(b);
+  if (!dev_out) return set_error(GAS_ERR_ARG, "gas_batch_result_device: null");
+  return guarded_b([&] { b->e->laneResultDevice(lane, dev_out); });
+}
+
+int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor) {
+  BATCH_OR_FAIL(b);
+  if (divisor < 0)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_pull_divisor: need >= 0");
+  b->e->setPullDivisor(divisor);
+  return GAS_OK;
+}
+
+int gas_batch_free(gas_batch_t b) {
+  delete b;
+  return GAS_OK;
+}
+
+}  // extern "C"

@@ -37,7 +37,7 @@ struct gas_graph {
   std::vector<int32_t> h_csr_off, h_csr_dst, h_csr_eid;
   int32_t *csc_off = nullptr, *csc_src = nullptr;
   int32_t *csr_off = nullptr, *csr_dst = nullptr;
-  int32_t *csc_w = nullptr;
+  int32_t *csc_w = nullptr, *csr_w = nullptr;
   size_t device_bytes = 0;
   // Engines kept bound to this graph by the whole-run entry points
   // (gas_bfs, ...), one per program, released with the graph.

@@ -476,6 +476,7 @@ static void graph_release(gas_graph *g) {
   gasrt_free(g->csr_off);
   gasrt_free(g->csr_dst);
   gasrt_free(g->csc_w);
+  gasrt_free(g->csr_w);
 }
 
 int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
@@ -503,6 +504,8 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
     std::vector<int32_t> wc(g->nE);
     for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csc_eid[k]];
     up(g->csc_w, wc.data(), eb);
+    for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csr_eid[k]];
+    up(g->csr_w, wc.data(), eb);
   }
   if (rc) {
     graph_release(g);
@@ -522,6 +525,7 @@ int gas_graph_view_get(gas_graph_t g, gas_graph_view *v) {
   v->csr_offsets = g->csr_off;
   v->csr_dsts = g->csr_dst;
   v->csc_weights = g->csc_w;
+  v->csr_weights = g->csr_w;
   v->device_bytes = g->device_bytes;
   return GAS_OK;
 }
