@@ -214,6 +214,13 @@ int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats);
 int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
                  double *residual_out, gas_run_stats *stats);
 
+/* Personalised PageRank from one source, `iterations` all-active sweeps in
+ * FP32 with compensated summation: r_0 = 0.15 e_s, r_{k+1}(v) = 0.15 [v==s]
+ * + 0.85 sum_{u->v} r_k(u)/outdeg(u) (the PageRank program of
+ * pagerank.cu:27-73 with the teleport paid at the source only). */
+int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
+            float *rank_out, gas_run_stats *stats);
+
 /* ---- batched multi-source queries ------------------------------------------
  * Up to 64 BFS or SSSP queries on one resident graph, one per lane, sharing
  * every edge read (no counterpart in the reference, which runs one source
@@ -224,9 +231,15 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
  * SSSP distance (2147383647 unreached). */
 #define GAS_BATCH_BFS   0
 #define GAS_BATCH_SSSP  1
+#define GAS_BATCH_PPR   2   /* personalised PageRank, FP32 lanes */
 typedef struct gas_batch *gas_batch_t;
 
+/* kind GAS_BATCH_PPR runs 10 iterations per query; see gas_batch_create_ppr. */
 int gas_batch_create(gas_graph_t g, int kind, gas_batch_t *out);
+/* Personalised PageRank lanes with `iterations` sweeps each (the recurrence
+ * of gas_ppr); a lane's result must be read before the next step after it
+ * finishes. */
+int gas_batch_create_ppr(gas_graph_t g, int32_t iterations, gas_batch_t *out);
 /* Starts n queries: lanes[i] (0..63, free) runs from sources[i]. */
 int gas_batch_add(gas_batch_t b, const int32_t *lanes, const int32_t *sources,
                   int32_t n);
@@ -242,10 +255,46 @@ int gas_batch_run(gas_batch_t b, gas_run_stats *stats);
 /* Result of one lane: V int32 to host memory / to device memory. */
 int gas_batch_result(gas_batch_t b, int32_t lane, int32_t *host_out);
 int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out);
+/* PPR lanes: V float to host memory / to device memory. */
+int gas_batch_result_f32(gas_batch_t b, int32_t lane, float *host_out);
+int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out);
 /* Push-versus-pull switch of batched BFS: pull when the frontier's
  * out-edges times `divisor` reach |E| (default 1; 0 = never pull). */
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
 int gas_batch_free(gas_batch_t b);
+
+/* ---- continuous-batching query server -------------------------------------
+ * Keeps graphs resident and serves BFS / SSSP / personalised-PageRank
+ * queries (kinds GAS_BATCH_*) through one batch engine per (kind, graph),
+ * each on its own stream.  Queries are admitted into free lanes between
+ * rounds while their state fits the memory budget, so work of queries that
+ * arrive while others run overlaps on the device.  No counterpart in the
+ * reference. */
+typedef struct gas_server *gas_server_t;
+
+typedef struct {
+  double  elapsed_s;            /* first arrival to last completion window */
+  int64_t rounds;               /* host loop iterations that stepped work  */
+  int64_t admissions_deferred;  /* times a waiting query did not fit      */
+  int32_t max_concurrent;       /* most queries running at once           */
+  int32_t reserved;
+} gas_serve_stats;
+
+/* Per-query state bytes charged against the budget (lane state + result). */
+size_t gas_server_query_bytes(int kind, int64_t n_vertices);
+int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
+                      size_t memory_budget_bytes, int32_t ppr_iterations,
+                      gas_server_t *out);
+/* Serves n queries arriving at arrival_s[i] seconds (sorted, relative to
+ * the call) in real time.  latency_ms[i] receives arrival-to-result time;
+ * host_results[i], if non-NULL, receives the V int32 (BFS, SSSP) or float
+ * (PPR) result.  Fails with GAS_ERR_FULL, serving nothing, if some query can
+ * never fit the budget. */
+int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
+                      const int32_t *graph_ids, const int32_t *sources,
+                      const double *arrival_s, void *const *host_results,
+                      double *latency_ms, gas_serve_stats *stats);
+int gas_server_free(gas_server_t s);
 
 /* ---- the engine API, step by step --------------------------------------- */
 #define GAS_PROGRAM_BFS       0  /* VertexData int depth; no EdgeData      */

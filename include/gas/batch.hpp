@@ -106,7 +106,7 @@ __device__ inline int64_t block_sum64(int64_t x, int64_t *lds) {
 // ---- list construction -------------------------------------------------------
 // Pass 1 over contiguous chunks of vertices with mask array m: per chunk the
 // number of nonzero masks, their push items and out-edges, and the OR.
-__global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
+static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
 
 // Pass 2: ordered list of vertices with nonzero masks, item offsets and the
 // owner of every item; workgroup 0 publishes totals and the direction.
-__global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m) {
+static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m) {
 
 // ---- BFS ---------------------------------------------------------------------
 // Clears lanes `reset` (visited bits, frontier bits of parity p, depth rows).
-__global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
+static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
                                                    uint64_t reset) {
   const int stride = gridDim.x * NT;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += stride) {
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
 }
 
 // One thread per new lane: seed its source.
-__global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
+static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
                             const int32_t *sources, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -221,7 +221,7 @@ __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
 }
 
 // Push: one wave per item, one lane per out-edge.
-__global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
+static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
                                                   uint64_t used) {
   const BatchCtl *c = d.ctl;
   if (c->pull) return;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
 // Pull: every vertex missing some used lane ORs its in-neighbours'
 // frontier masks until all missing lanes are found (thread per vertex up to
 // 32 in-edges, wave per vertex above).
-__global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
+static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
                                                   uint64_t used) {
   const BatchCtl *c = d.ctl;
   if (!c->pull) return;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
 // New bits become visited and form the next frontier (parity p^1); depth
 // rows get round + 1 - lane_start.  Contiguous chunks, with the list
 // counters of the new frontier (pass 1 of kb_list).
-__global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
+static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
                                                       uint64_t used,
                                                       int round) {
   __shared__ int lds[NT / WAVE];
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
 }
 
 // Depth of lane s as int32 (-1 unreached), for the caller.
-__global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
+static __global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
                                                      int32_t *out) {
   const uint8_t *row = d.depth8 + (size_t)s * d.rowV;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
 }
 
 // ---- SSSP --------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void kb_sssp_reset(BatchDev d,
+static __global__ __launch_bounds__(NT) void kb_sssp_reset(BatchDev d,
                                                     uint64_t reset) {
   const int64_t total = (int64_t)d.nV * BW;
   const int64_t stride = (int64_t)gridDim.x * NT;
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(NT) void kb_sssp_reset(BatchDev d,
   }
 }
 
-__global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
+static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
                              const int32_t *sources, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -390,7 +390,7 @@ __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
 // marked changed in the destination whenever its candidate beats the value
 // it read: a superset of the lanes actually lowered (the read may be older
 // than a concurrent atomicMin), which can only cause a redundant push.
-__global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
+static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   const BatchCtl *c = d.ctl;
   const int n_items = c->n_items;
   const int lane = lane_id();
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
 }
 
 // chg <- chg_next, chg_next <- 0, with the list counters (pass 1 of kb_list).
-__global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
+static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
   }
 }
 
-__global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
+static __global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
                                                       int32_t *out) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
     out[v] = d.dist[(size_t)v * BW + s];
@@ -555,6 +555,7 @@ class BatchEngine {
 
   ~BatchEngine() {
     if (m_stream) gasrt_stream_synchronize(m_stream);
+    if (m_ev) gasrt_event_destroy(m_ev);
     gasrt_host_free(m_hctl);
     gasrt_stream_destroy(m_stream);
   }
@@ -632,6 +633,26 @@ class BatchEngine {
     ++m_round;
   }
 
+  // step() followed by an asynchronous snapshot of the control block and an
+  // event; poll() tells when both are done, aliveSnapshot() reads it.
+  void stepAsync() {
+    step();
+    snapshotAsync();
+  }
+  void snapshotAsync() {
+    if (!m_ev) check(gasrt_event_create(&m_ev), "event");
+    check(gasrt_memcpy_d2h_async(&m_hctl[1], m_ctl.p, sizeof(BatchCtl),
+                                 m_stream),
+          "ctl snapshot");
+    check(gasrt_event_record(m_ev, m_stream), "event");
+  }
+  bool poll() {
+    int done = 1;
+    if (m_ev) check(gasrt_event_query(m_ev, &done), "event query");
+    return done != 0;
+  }
+  uint64_t aliveSnapshot() const { return m_hctl[1].lanes_alive & m_used; }
+
   // Lanes whose query still has work (reads the device; synchronises).
   uint64_t lanesAlive() {
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(BatchCtl), m_stream),
@@ -682,7 +703,8 @@ class BatchEngine {
 
   Kind m_kind;
   gasrt_stream_t m_stream = nullptr;
-  BatchCtl *m_hctl = nullptr;
+  gasrt_event_t m_ev = nullptr;
+  BatchCtl *m_hctl = nullptr;  // [0] synchronous reads, [1] snapshots
   int32_t m_nV = 0, m_nE = 0;
   int m_par = 0;
   int m_round = 0;
@@ -696,6 +718,332 @@ class BatchEngine {
   DevBuf<uint64_t> m_lor, m_visited, m_front0, m_front1, m_next, m_chg,
       m_chg_next;
   DevBuf<uint8_t> m_depth;
+};
+
+// ---- personalised PageRank, 64 FP32 lanes ----------------------------------
+// r_0 = 0.15 e_s, r_{k+1}(v) = 0.15 [v == s] + 0.85 sum_{u->v} r_k(u)/outdeg(u)
+// per lane s.  Every vertex owns >= 1 fixed item of at most PPR_ITEM in-edges;
+// one wave per item walks them in CSC order, reading a source's row only for
+// the lanes whose mask says the row is nonzero, with a compensated sum per
+// lane.  Vertices with several items fold their partials in item order.
+// The additions a lane performs depend on that lane alone, so a query's
+// result is bit-identical alone and inside any mix.
+constexpr int PPR_ITEM = 256;
+constexpr int PPR_UNROLL = 8;
+
+struct PprDev {
+  int32_t nV, nE;
+  const int32_t *csc_off, *csc_src;
+  const int32_t *outdeg;          // [V]
+  const int32_t *itv, *ite0;      // item -> vertex, first in-edge
+  const int32_t *pslot;           // item -> partial slot, -1: single item
+  const int32_t *vitem0;          // [V+1] first item of each vertex
+  const int32_t *mv;              // vertices with several items
+  int32_t n_items, n_multi;
+  float *rank[2];                 // [V][BW]
+  uint64_t *mask[2];              // lanes with a nonzero rank
+  float *part_s, *part_c;         // [slots][BW]
+  const int32_t *src_lane;        // [BW]
+};
+
+__device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
+                                 float sum) {
+  const int lane = lane_id();
+  const bool on = (run >> lane) & 1ull;
+  const float tele = (on && d.src_lane[lane] == v) ? 0.15f : 0.0f;
+  const float val = on ? tele + 0.85f * sum : 0.0f;
+  d.rank[p ^ 1][(size_t)v * BW + lane] = val;
+  const uint64_t m = __ballot(val != 0.0f);
+  if (lane == 0) d.mask[p ^ 1][v] = m;
+}
+
+static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
+  const int lane = lane_id();
+  const int waves = gridDim.x * (NT / WAVE);
+  const float *rk = d.rank[p];
+  const uint64_t *mk = d.mask[p];
+  for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < d.n_items;
+       it += waves) {
+    const int v = d.itv[it];
+    const int e0 = d.ite0[it];
+    const int e1 = min(e0 + PPR_ITEM, d.csc_off[v + 1]);
+    float s = 0.f, c = 0.f;
+    for (int base = e0; base < e1; base += WAVE) {
+      const int k = base + lane;
+      int u_l = 0, od_l = 1;
+      uint64_t m_l = 0;
+      if (k < e1) {
+        u_l = d.csc_src[k];
+        m_l = mk[u_l] & run;
+        od_l = d.outdeg[u_l];
+      }
+      uint64_t nz = __ballot(m_l != 0);
+      while (nz) {
+        int u[PPR_UNROLL], od[PPR_UNROLL];
+        bool on[PPR_UNROLL];
+        float r[PPR_UNROLL];
+#pragma unroll
+        for (int x = 0; x < PPR_UNROLL; ++x) {
+          const int j = nz ? __ffsll((unsigned long long)nz) - 1 : 0;
+          const bool have = nz != 0;
+          nz &= nz - 1;
+          u[x] = __shfl(u_l, j, WAVE);
+          od[x] = __shfl(od_l, j, WAVE);
+          const uint64_t mj =
+              ((uint64_t)__shfl((uint32_t)(m_l >> 32), j, WAVE) << 32) |
+              __shfl((uint32_t)m_l, j, WAVE);
+          on[x] = have && ((mj >> lane) & 1ull);
+        }
+#pragma unroll
+        for (int x = 0; x < PPR_UNROLL; ++x)
+          r[x] = on[x] ? rk[(size_t)u[x] * BW + lane] : 0.f;
+#pragma unroll
+        for (int x = 0; x < PPR_UNROLL; ++x) {
+          if (!on[x]) continue;
+          const float y = r[x] / (float)od[x];
+          const float t = s + y;
+          const float bp = t - s;
+          c += (s - (t - bp)) + (y - bp);
+          s = t;
+        }
+      }
+    }
+    const int ps = d.pslot[it];
+    if (ps < 0) {
+      ppr_write(d, p, v, run, s + c);
+    } else {
+      d.part_s[(size_t)ps * BW + lane] = s;
+      d.part_c[(size_t)ps * BW + lane] = c;
+    }
+  }
+}
+
+static __global__ __launch_bounds__(NT) void kp_fix(PprDev d, int p, uint64_t run) {
+  const int lane = lane_id();
+  const int waves = gridDim.x * (NT / WAVE);
+  for (int i = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; i < d.n_multi;
+       i += waves) {
+    const int v = d.mv[i];
+    float s = 0.f, c = 0.f;
+    for (int it = d.vitem0[v]; it < d.vitem0[v + 1]; ++it) {
+      const int ps = d.pslot[it];
+      const float bs = d.part_s[(size_t)ps * BW + lane];
+      const float bc = d.part_c[(size_t)ps * BW + lane];
+      const float t = s + bs;
+      const float bp = t - s;
+      c = c + bc + ((s - (t - bp)) + (bs - bp));
+      s = t;
+    }
+    ppr_write(d, p, v, run, s + c);
+  }
+}
+
+static __global__ __launch_bounds__(NT) void kp_reset(PprDev d, int p,
+                                               uint64_t reset) {
+  const int64_t total = (int64_t)d.nV * BW;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < total;
+       i += stride) {
+    const int s = (int)(i & (BW - 1));
+    if ((reset >> s) & 1ull) d.rank[p][i] = 0.f;
+    if (s == 0) d.mask[p][i / BW] &= ~reset;
+  }
+}
+
+static __global__ void kp_seed(PprDev d, int p, const int32_t *lanes,
+                        const int32_t *sources, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = lanes[i], v = sources[i];
+  d.rank[p][(size_t)v * BW + s] = 0.15f;
+  atomicOr((unsigned long long *)&d.mask[p][v],
+           (unsigned long long)(1ull << s));
+}
+
+static __global__ __launch_bounds__(NT) void kp_extract(PprDev d, int p, int s,
+                                                 float *out) {
+  for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
+    out[v] = d.rank[p][(size_t)v * BW + s];
+}
+
+class PprBatch {
+ public:
+  PprBatch(gas_graph_t g, int iterations) : m_iters(iterations) {
+    if (iterations < 1) throw Error(GAS_ERR_ARG, "PPR needs >= 1 iteration");
+    check(gasrt_stream_create(&m_stream), "stream");
+    gas_graph_view gv;
+    check(gas_graph_view_get(g, &gv), "graph view");
+    m_nV = gv.n_vertices;
+    const size_t V = (size_t)m_nV;
+    std::vector<int32_t> cscoff(V + 1), csroff(V + 1);
+    check(gas_graph_host_csc(g, cscoff.data(), nullptr, nullptr), "csc");
+    check(gas_graph_host_csr(g, csroff.data(), nullptr, nullptr), "csr");
+    std::vector<int32_t> outdeg(V), itv, ite0, pslot, vitem0(V + 1), mv;
+    int32_t slots = 0;
+    for (size_t v = 0; v < V; ++v) {
+      outdeg[v] = csroff[v + 1] - csroff[v];
+      const int32_t deg = cscoff[v + 1] - cscoff[v];
+      const int32_t nit = deg == 0 ? 1 : (deg + PPR_ITEM - 1) / PPR_ITEM;
+      vitem0[v] = (int32_t)itv.size();
+      if (nit > 1) mv.push_back((int32_t)v);
+      for (int32_t k = 0; k < nit; ++k) {
+        itv.push_back((int32_t)v);
+        ite0.push_back(cscoff[v] + k * PPR_ITEM);
+        pslot.push_back(nit > 1 ? slots++ : -1);
+      }
+    }
+    vitem0[V] = (int32_t)itv.size();
+    m_outdeg.upload(outdeg, m_stream);
+    m_itv.upload(itv, m_stream);
+    m_ite0.upload(ite0, m_stream);
+    m_pslot.upload(pslot, m_stream);
+    m_vitem0.upload(vitem0, m_stream);
+    m_mv.upload(mv, m_stream);
+    m_rank0.alloc(V * BW);
+    m_rank1.alloc(V * BW);
+    m_mask0.alloc(V);
+    m_mask1.alloc(V);
+    m_ps.alloc((size_t)std::max(slots, 1) * BW);
+    m_pc.alloc((size_t)std::max(slots, 1) * BW);
+    m_src.alloc(BW);
+    m_lanes.alloc(BW);
+    m_sources.alloc(BW);
+    check(gasrt_memset(m_rank0.p, 0, V * BW * 4, m_stream), "memset");
+    check(gasrt_memset(m_rank1.p, 0, V * BW * 4, m_stream), "memset");
+    check(gasrt_memset(m_mask0.p, 0, V * 8, m_stream), "memset");
+    check(gasrt_memset(m_mask1.p, 0, V * 8, m_stream), "memset");
+    for (int i = 0; i < BW; ++i) m_hsrc[i] = -1;
+    check(gasrt_memcpy_h2d(m_src.p, m_hsrc, sizeof(m_hsrc), m_stream), "src");
+    PprDev &d = m_dev;
+    d.nV = m_nV;
+    d.nE = gv.n_edges;
+    d.csc_off = gv.csc_offsets;
+    d.csc_src = gv.csc_srcs;
+    d.outdeg = m_outdeg.p;
+    d.itv = m_itv.p;
+    d.ite0 = m_ite0.p;
+    d.pslot = m_pslot.p;
+    d.vitem0 = m_vitem0.p;
+    d.mv = m_mv.p;
+    d.n_items = (int32_t)itv.size();
+    d.n_multi = (int32_t)mv.size();
+    d.rank[0] = m_rank0.p;
+    d.rank[1] = m_rank1.p;
+    d.mask[0] = m_mask0.p;
+    d.mask[1] = m_mask1.p;
+    d.part_s = m_ps.p;
+    d.part_c = m_pc.p;
+    d.src_lane = m_src.p;
+    check(gasrt_stream_synchronize(m_stream), "sync");
+  }
+
+  ~PprBatch() {
+    if (m_stream) gasrt_stream_synchronize(m_stream);
+    if (m_ev) gasrt_event_destroy(m_ev);
+    gasrt_stream_destroy(m_stream);
+  }
+
+  gasrt_stream_t stream() const { return m_stream; }
+  uint64_t lanesUsed() const { return m_used; }
+  int iterations() const { return m_iters; }
+
+  void addLanes(const int32_t *lanes, const int32_t *sources, int n) {
+    if (n <= 0) return;
+    if (n > BW) throw Error(GAS_ERR_ARG, "more than 64 lanes");
+    uint64_t reset = 0;
+    for (int i = 0; i < n; ++i) {
+      if (lanes[i] < 0 || lanes[i] >= BW || sources[i] < 0 ||
+          sources[i] >= m_nV)
+        throw Error(GAS_ERR_ARG, "lane or source out of range");
+      const uint64_t bit = 1ull << lanes[i];
+      if ((m_used | reset) & bit) throw Error(GAS_ERR_STATE, "lane in use");
+      reset |= bit;
+      m_hsrc[lanes[i]] = sources[i];
+      m_done[lanes[i]] = 0;
+    }
+    check(gasrt_memcpy_h2d(m_src.p, m_hsrc, sizeof(m_hsrc), m_stream), "src");
+    check(gasrt_memcpy_h2d(m_lanes.p, lanes, 4 * (size_t)n, m_stream), "lanes");
+    check(gasrt_memcpy_h2d(m_sources.p, sources, 4 * (size_t)n, m_stream),
+          "sources");
+    launch(kp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, m_par, reset);
+    launch(kp_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
+    m_used |= reset;
+  }
+
+  // Lanes whose query has iterations left (host bookkeeping only).
+  uint64_t lanesAlive() const {
+    uint64_t a = 0;
+    for (int s = 0; s < BW; ++s)
+      if (((m_used >> s) & 1ull) && m_done[s] < m_iters) a |= 1ull << s;
+    return a;
+  }
+
+  void step() {
+    const uint64_t run = lanesAlive();
+    if (!run) return;
+    launch(kp_item, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, run);
+    if (m_dev.n_multi)
+      launch(kp_fix, SCAN_BLOCKS, m_stream, m_dev, m_par, run);
+    m_par ^= 1;
+    for (int s = 0; s < BW; ++s)
+      if ((run >> s) & 1ull) ++m_done[s];
+  }
+
+  void stepAsync() {
+    step();
+    if (!m_ev) check(gasrt_event_create(&m_ev), "event");
+    check(gasrt_event_record(m_ev, m_stream), "event");
+  }
+  bool poll() {
+    int done = 1;
+    if (m_ev) check(gasrt_event_query(m_ev, &done), "event query");
+    return done != 0;
+  }
+
+  // Finished lanes must be read out before the next step(): a step leaves
+  // the rows of lanes that are not running at zero.
+  void releaseLanes(uint64_t lanes) { m_used &= ~lanes; }
+
+  int runToCompletion() {
+    int rounds = 0;
+    while (lanesAlive()) {
+      step();
+      ++rounds;
+    }
+    return rounds;
+  }
+
+  void laneResultDevice(int lane, float *dev_out) {
+    if (lane < 0 || lane >= BW) throw Error(GAS_ERR_ARG, "lane out of range");
+    launch(kp_extract, gridFor(m_nV), m_stream, m_dev, m_par, lane, dev_out);
+  }
+
+  void laneResult(int lane, float *host_out) {
+    if (m_tmp.n < (size_t)m_nV) m_tmp.alloc((size_t)m_nV);
+    laneResultDevice(lane, m_tmp.p);
+    check(gasrt_memcpy_d2h(host_out, m_tmp.p, 4 * (size_t)m_nV, m_stream),
+          "lane result");
+  }
+
+  void sync() { check(gasrt_stream_synchronize(m_stream), "sync"); }
+
+ private:
+  static unsigned gridFor(int64_t n) {
+    return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + NT - 1) / NT));
+  }
+  int m_iters;
+  gasrt_stream_t m_stream = nullptr;
+  gasrt_event_t m_ev = nullptr;
+  int32_t m_nV = 0;
+  int m_par = 0;
+  uint64_t m_used = 0;
+  int32_t m_hsrc[BW];
+  int m_done[BW] = {};
+  PprDev m_dev{};
+  DevBuf<int32_t> m_outdeg, m_itv, m_ite0, m_pslot, m_vitem0, m_mv, m_src,
+      m_lanes, m_sources;
+  DevBuf<float> m_rank0, m_rank1, m_ps, m_pc, m_tmp;
+  DevBuf<uint64_t> m_mask0, m_mask1;
 };
 
 }  // namespace gas

@@ -112,6 +112,39 @@ struct PageRank {
                                           const VertexData *, EdgeData *) {}
 };
 
+// Personalised PageRank from one source: the PageRank program above with
+// the teleport term 0.15 paid only at the source,
+//   r_0 = 0.15 e_s,  r_{k+1}(v) = 0.15 [v == s] + 0.85 sum_{u->v} r_k(u)/outdeg(u),
+// run as a fixed number of all-active sweeps.  No counterpart in the
+// reference; it is the one-query-at-a-time baseline of the batched PPR.
+struct PersonalizedPageRank {
+  static constexpr bool compensatedSum = true;
+  struct VertexData {
+    float rank;
+    int numOutEdges;
+    int isSource;
+  };
+  struct EdgeData {};
+  typedef float GatherResult;
+  static constexpr float gatherZero = 0.0f;
+
+  __host__ __device__ static float gatherMap(const VertexData *,
+                                             const VertexData *src,
+                                             const EdgeData *) {
+    return src->rank / src->numOutEdges;
+  }
+  __host__ __device__ static float gatherReduce(const float &left,
+                                                const float &right) {
+    return left + right;
+  }
+  __host__ __device__ static bool apply(VertexData *v, const float &sum) {
+    v->rank = (v->isSource ? 0.15f : 0.0f) + 0.85f * sum;
+    return false;
+  }
+  __host__ __device__ static void scatter(const VertexData *,
+                                          const VertexData *, EdgeData *) {}
+};
+
 // connected_component.cu:25-63
 struct CC {
   typedef int VertexData;

@@ -114,3 +114,47 @@ def test_batch_errors(gpu):
     nw = gas.Graph(gas.Edges.from_arrays(3, [0, 1], [1, 2]))
     with pytest.raises(gas.GasError):
         gas.Batch(nw, gas.BATCH_SSSP)  # no weights
+
+
+def rel_l1(x, ref):
+    return float(np.abs(x.astype(np.float64) - ref).sum() / np.abs(ref).sum())
+
+
+def test_ppr_plain_and_batched_vs_fp64(gpu):
+    el = workloads.rmat(14, symmetrize=True, weights=True)
+    s, d, _ = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    src = _sources(np.random.default_rng(9), el, 64)
+    src[1] = 0  # hub source
+    # plain engine, one query
+    r, _ = g.ppr(int(src[0]), 10)
+    ref = oracle.check_ppr_fp64(n, s, d, int(src[0]), 10)
+    assert rel_l1(r, ref) <= 1e-5
+    # batched, 64 lanes
+    b = gas.Batch(g, gas.BATCH_PPR, ppr_iterations=10)
+    b.add(np.arange(64), src)
+    st = b.run()
+    assert st.supersteps == 10
+    for lane in (0, 1, 17, 63):
+        ref = oracle.check_ppr_fp64(n, s, d, int(src[lane]), 10)
+        assert rel_l1(b.result(lane), ref) <= 1e-5, lane
+
+
+def test_ppr_alone_equals_mixed_bitwise(gpu):
+    el = workloads.road(128, 128, seed=8)
+    g = gas.Graph(el)
+    src = _sources(np.random.default_rng(4), el, 20)
+    b1 = gas.Batch(g, gas.BATCH_PPR, ppr_iterations=10)
+    b1.add([3], [src[0]])
+    b1.run()
+    alone = b1.result(3)
+    b = gas.Batch(g, gas.BATCH_PPR, ppr_iterations=10)
+    b.add(np.arange(10), src[5:15])
+    b.step(4)
+    b.add([11], [src[0]])
+    while True:
+        b.step(1)
+        if not (b.alive() >> 11) & 1:
+            break
+    assert np.array_equal(b.result(11), alone)

@@ -1,0 +1,59 @@
+"""GPU tests of the continuous-batching server: every served result equals
+the query run alone (bit-exact for BFS/SSSP, identical for PPR) and the CPU
+oracle; admission waits when memory is short and refuses queries that can
+never fit."""
+import numpy as np
+import pytest
+
+import oracle
+import vertexapi2_amd as gas
+from vertexapi2_amd import serve, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone(gpu):
+    road = gas.Graph(workloads.road(96, 96, seed=2))
+    rmat = gas.Graph(workloads.rmat(12, symmetrize=True, weights=True))
+    graphs = [road, rmat]
+    edges = [workloads.road(96, 96, seed=2), workloads.rmat(12, symmetrize=True, weights=True)]
+    stream = serve.make_stream(rate_qps=3000, n=150, graphs=graphs, seed=11)
+    srv = gas.Server(graphs, memory_budget=1 << 34)
+    keep = list(range(0, 150, 3))
+    out, lat, results = serve.serve(srv, stream, keep=keep)
+    assert out["queries"] == 150 and np.all(lat > 0)
+    assert out["max_concurrent"] > 1  # queries overlapped
+    for q in keep:
+        k, gi, s = stream["kinds"][q], stream["graph_ids"][q], int(stream["sources"][q])
+        src, dst, w = edges[gi].arrays()
+        n = graphs[gi].n_vertices
+        if k == gas.BATCH_BFS:
+            assert np.array_equal(results[q], oracle.check_bfs(n, src, dst, s)), q
+        elif k == gas.BATCH_SSSP:
+            assert np.array_equal(results[q], oracle.check_dijkstra(n, src, dst, w, s)), q
+        else:
+            b = gas.Batch(graphs[gi], gas.BATCH_PPR)
+            b.add([0], [s])
+            b.run()
+            assert np.array_equal(results[q], b.result(0)), q  # alone == mixed
+            ref = oracle.check_ppr_fp64(n, src, dst, s, 10)
+            assert np.abs(results[q] - ref).sum() <= 1e-5 * np.abs(ref).sum()
+
+
+def test_admission_by_memory(gpu):
+    road = gas.Graph(workloads.road(64, 64, seed=3))
+    el = workloads.road(64, 64, seed=3)
+    V = road.n_vertices
+    budget = 2 * gas.query_bytes(gas.BATCH_SSSP, V)  # two SSSP queries at once
+    srv = gas.Server([road], memory_budget=budget)
+    stream = serve.make_stream(rate_qps=100000, n=12, graphs=[road], mix=(0, 1, 0), seed=4)
+    out, lat, results = serve.serve(srv, stream, keep=list(range(12)))
+    assert out["max_concurrent"] <= 2
+    assert out["admissions_deferred"] > 0
+    s, d, w = el.arrays()
+    for q in range(12):
+        assert np.array_equal(results[q], oracle.check_dijkstra(V, s, d, w, int(stream["sources"][q])))
+    tiny = gas.Server([road], memory_budget=gas.query_bytes(gas.BATCH_SSSP, V) - 1)
+    with pytest.raises(gas.GasError) as ex:
+        serve.serve(tiny, stream)
+    assert ex.value.code == 6  # GAS_ERR_FULL

@@ -270,6 +270,13 @@ class Graph:
         _check(lib().gas_cc(self._h, _ptr(out), C.byref(st)))
         return out, st
 
+    def ppr(self, source: int, iterations: int = 10):
+        out = np.zeros(self.n_vertices, np.float32)
+        st = RunStats()
+        _check(lib().gas_ppr(self._h, C.c_int32(source), C.c_int32(iterations),
+                             _ptr(out, C.c_float), C.byref(st)))
+        return out, st
+
     def pagerank(self, iterations: int = 20):
         out = np.zeros(self.n_vertices, np.float32)
         res = np.zeros(max(iterations, 1), np.float64)
@@ -349,15 +356,19 @@ class Engine:
         return st
 
 
-BATCH_BFS, BATCH_SSSP = 0, 1
+BATCH_BFS, BATCH_SSSP, BATCH_PPR = 0, 1, 2
 
 
 class Batch:
     """Up to 64 BFS or SSSP queries sharing edge reads (gas_batch_t)."""
 
-    def __init__(self, graph: Graph, kind: int):
+    def __init__(self, graph: Graph, kind: int, ppr_iterations: int = 10):
         h = C.c_void_p()
-        _check(lib().gas_batch_create(graph.handle, C.c_int(kind), C.byref(h)))
+        if kind == BATCH_PPR:
+            _check(lib().gas_batch_create_ppr(graph.handle, C.c_int32(ppr_iterations),
+                                              C.byref(h)))
+        else:
+            _check(lib().gas_batch_create(graph.handle, C.c_int(kind), C.byref(h)))
         self._h = h
         self.graph = graph  # keeps the resident graph alive
         self.kind = kind
@@ -393,9 +404,69 @@ class Batch:
         return st
 
     def result(self, lane: int) -> np.ndarray:
+        if self.kind == BATCH_PPR:
+            out = np.zeros(self.graph.n_vertices, np.float32)
+            _check(lib().gas_batch_result_f32(self._h, C.c_int32(lane),
+                                              _ptr(out, C.c_float)))
+            return out
         out = np.zeros(self.graph.n_vertices, np.int32)
         _check(lib().gas_batch_result(self._h, C.c_int32(lane), _ptr(out)))
         return out
 
     def set_pull_divisor(self, divisor: int) -> None:
         _check(lib().gas_batch_set_pull_divisor(self._h, C.c_int32(divisor)))
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("elapsed_s", C.c_double), ("rounds", C.c_int64),
+                ("admissions_deferred", C.c_int64), ("max_concurrent", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+def query_bytes(kind: int, n_vertices: int) -> int:
+    f = lib().gas_server_query_bytes
+    f.restype = C.c_size_t
+    return int(f(C.c_int(kind), C.c_int64(n_vertices)))
+
+
+class Server:
+    """Continuous-batching server over resident graphs (gas_server_t)."""
+
+    def __init__(self, graphs, memory_budget: int, ppr_iterations: int = 10):
+        self.graphs = list(graphs)
+        arr = (C.c_void_p * len(self.graphs))(*[g.handle.value for g in self.graphs])
+        h = C.c_void_p()
+        _check(lib().gas_server_create(arr, C.c_int32(len(self.graphs)),
+                                       C.c_size_t(memory_budget),
+                                       C.c_int32(ppr_iterations), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gas_server_free(self._h)
+            self._h = None
+
+    def replay(self, kinds, graph_ids, sources, arrivals_s, keep=None):
+        """Serves the stream in real time; returns (latency_ms, results, stats)
+        where results maps query index -> array for the indices in `keep`."""
+        k, gi, so = _i32(kinds), _i32(graph_ids), _i32(sources)
+        ar = np.ascontiguousarray(arrivals_s, dtype=np.float64)
+        n = len(k)
+        lat = np.zeros(n, np.float64)
+        results = {}
+        ptrs = None
+        if keep:
+            ptrs = (C.c_void_p * n)()
+            for q in keep:
+                dt = np.float32 if k[q] == BATCH_PPR else np.int32
+                results[q] = np.zeros(self.graphs[gi[q]].n_vertices, dt)
+                ptrs[q] = results[q].ctypes.data
+        st = ServeStats()
+        _check(lib().gas_server_replay(self._h, C.c_int32(n), _ptr(k), _ptr(gi), _ptr(so),
+                                       ar.ctypes.data_as(C.POINTER(C.c_double)), ptrs,
+                                       lat.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(st)))
+        return lat, results, st

@@ -324,6 +324,37 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
   });
 }
 
+int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
+            float *rank_out, gas_run_stats *stats) {
+  if (!g || !rank_out || source < 0 || source >= g->nV || iterations < 0)
+    return set_error(GAS_ERR_ARG, "gas_ppr: bad arguments");
+  return guarded([&] {
+    typedef PersonalizedPageRank PPR;
+    std::vector<PPR::VertexData> vd(g->nV);
+    for (int32_t v = 0; v < g->nV; ++v) {
+      vd[v].rank = v == source ? 0.15f : 0.0f;
+      vd[v].numOutEdges = g->h_csr_off[v + 1] - g->h_csr_off[v];
+      vd[v].isSource = v == source;
+    }
+    auto &e = cached_engine<PPR>(g, 4, vd.data());
+    e.resetStats();
+    RunTimer tm(e.stream());
+    for (int it = 0; it < iterations; ++it) {
+      e.setActive(0, g->nV);
+      e.gatherApply(true);
+    }
+    double ms = tm.stop();
+    e.getResults();
+    for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[v].rank;
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+      stats->supersteps = iterations;
+      stats->gathered_edges = (int64_t)iterations * g->nE;
+    }
+  });
+}
+
 // ---- step-by-step engine API ------------------------------------------------
 
 struct gas_engine {

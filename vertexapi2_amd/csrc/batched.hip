@@ -1,6 +1,8 @@
-// C ABI of the batched multi-source engine (gas_batch_*).
+// C ABI of the batched multi-source engines (gas_batch_*): BFS / SSSP lanes
+// (BatchEngine) and personalised PageRank lanes (PprBatch).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <memory>
 
 #include "gas.h"
@@ -10,7 +12,8 @@
 using namespace gas;
 
 struct gas_batch {
-  std::unique_ptr<BatchEngine> e;
+  std::unique_ptr<BatchEngine> e;   // BFS / SSSP
+  std::unique_ptr<PprBatch> pp;     // PPR
 };
 
 namespace {
@@ -33,12 +36,13 @@ int guarded_b(F f) {
 }  // namespace
 
 #define BATCH_OR_FAIL(b)                                           \
-  if (!(b) || !(b)->e)                                             \
+  if (!(b) || (!(b)->e && !(b)->pp))                               \
   return set_error(GAS_ERR_ARG, "%s: null batch", __func__)
 
 extern "C" {
 
 int gas_batch_create(gas_graph_t g, int kind, gas_batch_t *out) {
+  if (kind == GAS_BATCH_PPR) return gas_batch_create_ppr(g, 10, out);
   if (!g || !out || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
     return set_error(GAS_ERR_ARG, "gas_batch_create: bad arguments");
   return guarded_b([&] {
@@ -50,49 +54,76 @@ int gas_batch_create(gas_graph_t g, int kind, gas_batch_t *out) {
   });
 }
 
+int gas_batch_create_ppr(gas_graph_t g, int32_t iterations, gas_batch_t *out) {
+  if (!g || !out || iterations < 1)
+    return set_error(GAS_ERR_ARG, "gas_batch_create_ppr: bad arguments");
+  return guarded_b([&] {
+    std::unique_ptr<gas_batch> b(new gas_batch);
+    b->pp.reset(new PprBatch(g, iterations));
+    *out = b.release();
+  });
+}
+
 int gas_batch_add(gas_batch_t b, const int32_t *lanes, const int32_t *sources,
                   int32_t n) {
   BATCH_OR_FAIL(b);
   if (n < 0 || (n > 0 && (!lanes || !sources)))
     return set_error(GAS_ERR_ARG, "gas_batch_add: bad arguments");
-  return guarded_b([&] { b->e->addLanes(lanes, sources, n); });
+  return guarded_b([&] {
+    if (b->e)
+      b->e->addLanes(lanes, sources, n);
+    else
+      b->pp->addLanes(lanes, sources, n);
+  });
 }
 
 int gas_batch_step(gas_batch_t b, int32_t rounds) {
   BATCH_OR_FAIL(b);
   return guarded_b([&] {
-    for (int i = 0; i < rounds; ++i) b->e->step();
+    for (int i = 0; i < rounds; ++i) {
+      if (b->e)
+        b->e->step();
+      else
+        b->pp->step();
+    }
   });
 }
 
 int gas_batch_alive(gas_batch_t b, uint64_t *alive) {
   BATCH_OR_FAIL(b);
   if (!alive) return set_error(GAS_ERR_ARG, "gas_batch_alive: null");
-  return guarded_b([&] { *alive = b->e->lanesAlive(); });
+  return guarded_b([&] {
+    *alive = b->e ? b->e->lanesAlive() : b->pp->lanesAlive();
+  });
 }
 
 int gas_batch_used(gas_batch_t b, uint64_t *used) {
   BATCH_OR_FAIL(b);
   if (!used) return set_error(GAS_ERR_ARG, "gas_batch_used: null");
-  *used = b->e->lanesUsed();
+  *used = b->e ? b->e->lanesUsed() : b->pp->lanesUsed();
   return GAS_OK;
 }
 
 int gas_batch_release(gas_batch_t b, uint64_t lanes) {
   BATCH_OR_FAIL(b);
-  b->e->releaseLanes(lanes);
+  if (b->e)
+    b->e->releaseLanes(lanes);
+  else
+    b->pp->releaseLanes(lanes);
   return GAS_OK;
 }
 
 int gas_batch_run(gas_batch_t b, gas_run_stats *stats) {
   BATCH_OR_FAIL(b);
   return guarded_b([&] {
+    gasrt_stream_t s = b->e ? b->e->stream() : b->pp->stream();
     gasrt_event_t a = nullptr, z = nullptr;
     check(gasrt_event_create(&a), "event");
     check(gasrt_event_create(&z), "event");
-    check(gasrt_event_record(a, b->e->stream()), "event");
-    const int rounds = b->e->runToCompletion();
-    check(gasrt_event_record(z, b->e->stream()), "event");
+    check(gasrt_event_record(a, s), "event");
+    const int rounds =
+        b->e ? b->e->runToCompletion() : b->pp->runToCompletion();
+    check(gasrt_event_record(z, s), "event");
     check(gasrt_event_synchronize(z), "event");
     float ms = 0;
     check(gasrt_event_elapsed_ms(a, z, &ms), "elapsed");
@@ -108,21 +139,37 @@ int gas_batch_run(gas_batch_t b, gas_run_stats *stats) {
 
 int gas_batch_result(gas_batch_t b, int32_t lane, int32_t *host_out) {
   BATCH_OR_FAIL(b);
-  if (!host_out) return set_error(GAS_ERR_ARG, "gas_batch_result: null");
+  if (!host_out || !b->e)
+    return set_error(GAS_ERR_ARG, "gas_batch_result: null or PPR batch");
   return guarded_b([&] { b->e->laneResult(lane, host_out); });
 }
 
 int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out) {
   BATCH_OR_FAIL(b);
-  if (!dev_out) return set_error(GAS_ERR_ARG, "gas_batch_result_device: null");
+  if (!dev_out || !b->e)
+    return set_error(GAS_ERR_ARG, "gas_batch_result_device: null or PPR batch");
   return guarded_b([&] { b->e->laneResultDevice(lane, dev_out); });
+}
+
+int gas_batch_result_f32(gas_batch_t b, int32_t lane, float *host_out) {
+  BATCH_OR_FAIL(b);
+  if (!host_out || !b->pp)
+    return set_error(GAS_ERR_ARG, "gas_batch_result_f32: null or not PPR");
+  return guarded_b([&] { b->pp->laneResult(lane, host_out); });
+}
+
+int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out) {
+  BATCH_OR_FAIL(b);
+  if (!dev_out || !b->pp)
+    return set_error(GAS_ERR_ARG, "gas_batch_result_f32_device: bad batch");
+  return guarded_b([&] { b->pp->laneResultDevice(lane, dev_out); });
 }
 
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor) {
   BATCH_OR_FAIL(b);
   if (divisor < 0)
     return set_error(GAS_ERR_ARG, "gas_batch_set_pull_divisor: need >= 0");
-  b->e->setPullDivisor(divisor);
+  if (b->e) b->e->setPullDivisor(divisor);
   return GAS_OK;
 }
 

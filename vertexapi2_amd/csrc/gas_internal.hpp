@@ -41,8 +41,8 @@ struct gas_graph {
   size_t device_bytes = 0;
   // Engines kept bound to this graph by the whole-run entry points
   // (gas_bfs, ...), one per program, released with the graph.
-  void *engine_cache[4] = {nullptr, nullptr, nullptr, nullptr};
-  void (*engine_release[4])(void *) = {nullptr, nullptr, nullptr, nullptr};
+  void *engine_cache[8] = {};
+  void (*engine_release[8])(void *) = {};
 };
 
 namespace gas {

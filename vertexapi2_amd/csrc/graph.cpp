@@ -468,7 +468,7 @@ int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,
 }
 
 static void graph_release(gas_graph *g) {
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i)
     if (g->engine_cache[i] && g->engine_release[i])
       g->engine_release[i](g->engine_cache[i]);
   gasrt_free(g->csc_off);

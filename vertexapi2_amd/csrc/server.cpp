@@ -1,0 +1,298 @@
+// Continuous-batching query server over graphs resident in HBM.
+//
+// One batch engine per (query kind, graph) -- BFS and SSSP lanes
+// (BatchEngine), personalised PageRank lanes (PprBatch) -- each on its own
+// stream, so rounds of different engines overlap on the device.  The host
+// loop (gas_server_replay) admits arrived queries into free lanes of their
+// engine while the per-query state memory fits the budget, adds them between
+// rounds of the engines already running, steps every engine that has work,
+// and retires lanes whose query finished: the lane's result is extracted
+// into a per-lane buffer in HBM (and optionally copied to the caller), the
+// lane and its memory are released, and the next waiting query can take it.
+// A query's latency is the time from its arrival to its result being in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "gas.h"
+#include "gas/batch.hpp"
+#include "gas_internal.hpp"
+
+using namespace gas;
+
+namespace {
+
+double now_s() {
+  using C = std::chrono::steady_clock;
+  static const C::time_point t0 = C::now();
+  return std::chrono::duration<double>(C::now() - t0).count();
+}
+
+size_t query_bytes(int kind, int64_t V) {
+  // lane state + the lane's result buffer
+  switch (kind) {
+    case GAS_BATCH_BFS: return (size_t)(V * 1 + V * 3 / 8 + V * 4);
+    case GAS_BATCH_SSSP: return (size_t)(V * 4 + V / 4 + V * 4);
+    default: return (size_t)(V * 8 + V / 4 + V * 4);  // PPR
+  }
+}
+
+struct Engine {
+  int kind = 0, graph = 0;
+  int64_t V = 0;
+  std::unique_ptr<BatchEngine> be;
+  std::unique_ptr<PprBatch> pe;
+  uint64_t running = 0;             // lanes with an unfinished query
+  bool inflight = false;            // a round is queued on the stream
+  int lane_query[BW];
+  std::vector<int32_t> add_lanes, add_src;
+  DevBuf<int32_t> res_i[BW];
+  DevBuf<float> res_f[BW];
+  gasrt_stream_t stream() const { return be ? be->stream() : pe->stream(); }
+};
+
+}  // namespace
+
+struct gas_server {
+  std::vector<gas_graph_t> graphs;
+  std::vector<int64_t> nV;
+  size_t budget = 0, reserved = 0;
+  int ppr_iterations = 10;
+  std::vector<std::unique_ptr<Engine>> engines;  // [kind * n_graphs + graph]
+
+  Engine &engine(int kind, int graph) {
+    auto &slot = engines[(size_t)kind * graphs.size() + graph];
+    if (!slot) {
+      std::unique_ptr<Engine> e(new Engine);
+      e->kind = kind;
+      e->graph = graph;
+      e->V = nV[graph];
+      if (kind == GAS_BATCH_PPR)
+        e->pe.reset(new PprBatch(graphs[graph], ppr_iterations));
+      else
+        e->be.reset(new BatchEngine(graphs[graph],
+                                    kind == GAS_BATCH_BFS
+                                        ? BatchEngine::BFS_KIND
+                                        : BatchEngine::SSSP_KIND));
+      for (auto &q : e->lane_query) q = -1;
+      slot = std::move(e);
+    }
+    return *slot;
+  }
+};
+
+namespace {
+
+template <class F>
+int guarded_s(F f) {
+  try {
+    clear_error();
+    f();
+    return GAS_OK;
+  } catch (const Error &x) {
+    return set_error(x.code ? x.code : GAS_ERR_HIP, "%s", x.what());
+  } catch (const std::bad_alloc &) {
+    return set_error(GAS_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &x) {
+    return set_error(GAS_ERR_STATE, "%s", x.what());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t gas_server_query_bytes(int kind, int64_t n_vertices) {
+  return query_bytes(kind, n_vertices);
+}
+
+int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
+                      size_t memory_budget_bytes, int32_t ppr_iterations,
+                      gas_server_t *out) {
+  if (!graphs || n_graphs < 1 || !out || ppr_iterations < 1)
+    return set_error(GAS_ERR_ARG, "gas_server_create: bad arguments");
+  return guarded_s([&] {
+    std::unique_ptr<gas_server> s(new gas_server);
+    for (int i = 0; i < n_graphs; ++i) {
+      if (!graphs[i]) throw Error(GAS_ERR_ARG, "null graph");
+      gas_graph_view gv;
+      check(gas_graph_view_get(graphs[i], &gv), "graph view");
+      s->graphs.push_back(graphs[i]);
+      s->nV.push_back(gv.n_vertices);
+    }
+    s->budget = memory_budget_bytes;
+    s->ppr_iterations = ppr_iterations;
+    s->engines.resize(3 * (size_t)n_graphs);
+    *out = s.release();
+  });
+}
+
+int gas_server_free(gas_server_t s) {
+  delete s;
+  return GAS_OK;
+}
+
+int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
+                      const int32_t *graph_ids, const int32_t *sources,
+                      const double *arrival_s, void *const *host_results,
+                      double *latency_ms, gas_serve_stats *stats) {
+  if (!s || n < 0 || (n > 0 && (!kinds || !graph_ids || !sources ||
+                                !arrival_s || !latency_ms)))
+    return set_error(GAS_ERR_ARG, "gas_server_replay: bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (kinds[i] < 0 || kinds[i] > 2 || graph_ids[i] < 0 ||
+        graph_ids[i] >= (int)s->graphs.size() || sources[i] < 0 ||
+        sources[i] >= s->nV[graph_ids[i]])
+      return set_error(GAS_ERR_ARG, "gas_server_replay: query %d invalid", i);
+    if (query_bytes(kinds[i], s->nV[graph_ids[i]]) > s->budget)
+      return set_error(GAS_ERR_FULL,
+                       "gas_server_replay: query %d needs %zu bytes, budget "
+                       "is %zu",
+                       i, query_bytes(kinds[i], s->nV[graph_ids[i]]),
+                       s->budget);
+    if (i && arrival_s[i] < arrival_s[i - 1])
+      return set_error(GAS_ERR_ARG, "gas_server_replay: arrivals not sorted");
+  }
+  return guarded_s([&] {
+    std::deque<int> waiting;
+    int next = 0, done = 0;
+    int64_t rounds = 0, admissions_deferred = 0;
+    int max_running = 0;
+    const double t0 = now_s();
+    auto running_total = [&] {
+      int r = 0;
+      for (auto &ep : s->engines)
+        if (ep) r += __builtin_popcountll(ep->running);
+      return r;
+    };
+    while (done < n) {
+      bool progressed = false;
+      // 1. engines whose round finished: retire the queries that ended
+      for (auto &ep : s->engines) {
+        if (!ep || !ep->inflight) continue;
+        Engine &e = *ep;
+        if (!(e.be ? e.be->poll() : e.pe->poll())) continue;
+        e.inflight = false;
+        progressed = true;
+        const uint64_t alive =
+            e.be ? e.be->aliveSnapshot() : e.pe->lanesAlive();
+        const uint64_t fin_all = e.running & ~alive;
+        if (!fin_all) continue;
+        for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
+          const int lane = __builtin_ctzll(fin);
+          if (e.be) {
+            if (e.res_i[lane].n < (size_t)e.V) e.res_i[lane].alloc((size_t)e.V);
+            e.be->laneResultDevice(lane, e.res_i[lane].p);
+          } else {
+            if (e.res_f[lane].n < (size_t)e.V) e.res_f[lane].alloc((size_t)e.V);
+            e.pe->laneResultDevice(lane, e.res_f[lane].p);
+          }
+        }
+        check(gasrt_stream_synchronize(e.stream()), "sync");
+        const double tf = now_s() - t0;
+        for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
+          const int lane = __builtin_ctzll(fin);
+          const int q = e.lane_query[lane];
+          latency_ms[q] = (tf - arrival_s[q]) * 1e3;
+          if (host_results && host_results[q]) {
+            void *src = e.be ? (void *)e.res_i[lane].p : (void *)e.res_f[lane].p;
+            check(gasrt_memcpy_d2h(host_results[q], src, 4 * (size_t)e.V,
+                                   e.stream()),
+                  "result copy");
+          }
+          s->reserved -= query_bytes(e.kind, e.V);
+          e.lane_query[lane] = -1;
+          e.running &= ~(1ull << lane);
+          if (e.be)
+            e.be->releaseLanes(1ull << lane);
+          else
+            e.pe->releaseLanes(1ull << lane);
+          ++done;
+        }
+      }
+      // 2. arrivals and admission (first come first served per engine; a
+      // query that does not fit does not block other engines), only into
+      // engines between rounds
+      const double t = now_s() - t0;
+      while (next < n && arrival_s[next] <= t) waiting.push_back(next++);
+      bool blocked[3 * 64] = {};
+      for (auto it = waiting.begin(); it != waiting.end();) {
+        const int q = *it;
+        const int key = kinds[q] * (int)s->graphs.size() + graph_ids[q];
+        if (key < 3 * 64 && blocked[key]) {
+          ++it;
+          continue;
+        }
+        Engine &e = s->engine(kinds[q], graph_ids[q]);
+        const size_t need = query_bytes(kinds[q], s->nV[graph_ids[q]]);
+        uint64_t busy = e.be ? e.be->lanesUsed() : e.pe->lanesUsed();
+        for (int32_t l : e.add_lanes) busy |= 1ull << l;
+        if (e.inflight || busy == ~0ull || s->reserved + need > s->budget) {
+          if (key < 3 * 64) blocked[key] = true;
+          if (!e.inflight) ++admissions_deferred;
+          ++it;
+          continue;
+        }
+        const int lane = __builtin_ctzll(~busy);
+        e.add_lanes.push_back(lane);
+        e.add_src.push_back(sources[q]);
+        e.lane_query[lane] = q;
+        s->reserved += need;
+        it = waiting.erase(it);
+      }
+      // 3. start the next round of every idle engine with work
+      for (auto &ep : s->engines) {
+        if (!ep || ep->inflight) continue;
+        Engine &e = *ep;
+        if (!e.add_lanes.empty()) {
+          if (e.be)
+            e.be->addLanes(e.add_lanes.data(), e.add_src.data(),
+                           (int)e.add_lanes.size());
+          else
+            e.pe->addLanes(e.add_lanes.data(), e.add_src.data(),
+                           (int)e.add_lanes.size());
+          for (int32_t l : e.add_lanes) e.running |= 1ull << l;
+          e.add_lanes.clear();
+          e.add_src.clear();
+        }
+        if (e.running) {
+          if (e.be)
+            e.be->stepAsync();
+          else
+            e.pe->stepAsync();
+          e.inflight = true;
+          ++rounds;
+          progressed = true;
+        }
+      }
+      max_running = std::max(max_running, running_total());
+      if (!progressed) {
+        bool any_inflight = false;
+        for (auto &ep : s->engines) any_inflight |= ep && ep->inflight;
+        if (!any_inflight && waiting.empty() && next < n) {
+          const double wait = arrival_s[next] - (now_s() - t0);
+          if (wait > 0.0005)
+            std::this_thread::sleep_for(
+                std::chrono::duration<double>(wait * 0.8));
+        } else {
+          std::this_thread::yield();
+        }
+      }
+    }
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->elapsed_s = now_s() - t0;
+      stats->rounds = rounds;
+      stats->admissions_deferred = admissions_deferred;
+      stats->max_concurrent = max_running;
+    }
+  });
+}
+
+}  // extern "C"
