@@ -164,6 +164,21 @@ int gas_graph_permute_to_csc(gas_graph_t g, const void *in, void *out,
 int gas_graph_free(gas_graph_t g);
 
 /* ---- run statistics ----------------------------------------------------- */
+/* Kernel slots of gas_run_stats.kernel_ms / kernel_launches. */
+enum {
+  GAS_K_GATHER = 0,    /* k_gather_vp: thread and wave64 gather tiers      */
+  GAS_K_APPLY = 1,     /* k_apply: apply + fused thread/wave activation    */
+  GAS_K_SCATTER = 2,   /* k_scatter_vp: stepwise scatterActivate           */
+  GAS_K_HSCAN1 = 3,    /* k_hscan1: hub degree scan, pass 1                */
+  GAS_K_HSCAN2 = 4,    /* k_hscan2: hub degree scan, pass 2                */
+  GAS_K_HGATHER = 5,   /* k_hgather: merge-path gather of hub in-edges     */
+  GAS_K_HFIX = 6,      /* k_hfix: fold of hub tile partials                */
+  GAS_K_HSCATTER = 7,  /* k_hscatter: merge-path activation over hub edges */
+  GAS_K_COMPACT1 = 8,  /* k_compact1: frontier count per block             */
+  GAS_K_COMPACT2 = 9,  /* k_compact2: ordered frontier write               */
+  GAS_K_SLOTS = 12     /* slots 10-11 unused                              */
+};
+
 typedef struct {
   int32_t supersteps;        /* BSP supersteps executed                    */
   int32_t reserved;
@@ -171,14 +186,18 @@ typedef struct {
   int64_t scattered_edges;   /* sum over supersteps of out-edges scattered */
   int64_t active_total;      /* sum over supersteps of |active|            */
   double  ms;                /* device time of the run (HIP events)        */
-  double  gather_ms;         /* device time in the gather kernel           */
-  double  scatter_ms;        /* device time in the scatter kernel          */
-  int32_t gather_launches;
-  int32_t scatter_launches;
+  /* The fields below are filled only while gas_set_kernel_timing is on. */
+  double  gather_ms;         /* gather phase: k_gather_vp + hub gather     */
+  double  scatter_ms;        /* k_apply / k_scatter_vp + hub scatter       */
+  int32_t gather_launches;   /* k_gather_vp launches                       */
+  int32_t scatter_launches;  /* k_apply + k_scatter_vp launches            */
+  double  kernel_ms[GAS_K_SLOTS];        /* per kernel, summed over launches */
+  int32_t kernel_launches[GAS_K_SLOTS];
 } gas_run_stats;
 
-/* Set to nonzero to time every gather / scatter launch with HIP events
- * (adds host synchronisation; used for per-kernel roofline figures). */
+/* Set to nonzero to bracket every engine kernel launch with HIP events on
+ * the engine's stream (adds event packets between launches; used for the
+ * per-kernel roofline figures). */
 int gas_set_kernel_timing(int enable);
 
 /* Degree tiers of the engine the whole-run entry points keep for (graph,
@@ -203,6 +222,14 @@ int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
  * unreached vertices keep gatherZero = 2147383647. */
 int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
              gas_run_stats *stats);
+/* The same two runs with the result left in HBM: the initial vertex state
+ * is written on the device (no upload) and the result copied device to
+ * device into depth_dev / dist_dev (n_vertices int32 each).  stats->ms
+ * covers initialisation, the run and the copy. */
+int gas_bfs_device(gas_graph_t g, int32_t source, int32_t *depth_dev,
+                   gas_run_stats *stats);
+int gas_sssp_device(gas_graph_t g, int32_t source, int32_t *dist_dev,
+                    gas_run_stats *stats);
 /* Connected components by min-label propagation over in-edges
  * (connected_component.cu:25-81); label[v] initialised to v. */
 int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats);
@@ -220,6 +247,10 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
  * pagerank.cu:27-73 with the teleport paid at the source only). */
 int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
             float *rank_out, gas_run_stats *stats);
+/* The same with the ranks left in HBM (rank_dev: n_vertices floats); the
+ * initial state is written on the device. */
+int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
+                   float *rank_dev, gas_run_stats *stats);
 
 /* ---- batched multi-source queries ------------------------------------------
  * Up to 64 BFS or SSSP queries on one resident graph, one per lane, sharing

@@ -195,6 +195,17 @@ class GASEngineGPU {
           "resetVertexData");
   }
 
+  // Sets every vertex to `value` and vertex `at` to `one` on the device
+  // (at < 0: no exception); unbinds any host vertex-data array.
+  void fillVertexData(const VertexData &value, Int at, const VertexData &one) {
+    if (at >= m_nV) throw Error(GAS_ERR_ARG, "fillVertexData: vertex out of range");
+    m_vd_host = nullptr;
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)m_nV + NT - 1) / NT));
+    launch(k_fill_vdata<Program>, grid, m_stream, m_dev, value, (int32_t)at,
+           one);
+  }
+
   void setActive(Int vertexStart, Int vertexEnd) {
     if (vertexStart < 0 || vertexEnd > m_nV || vertexEnd < vertexStart)
       throw Error(GAS_ERR_ARG, "setActive: range outside [0, nVertices)");
@@ -215,16 +226,14 @@ class GASEngineGPU {
   void gatherApply(bool haveGather = true) {
     if (haveGather) {
       gatherPhase();
-      launch(k_apply<Program, true, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      klaunch(GAS_K_APPLY, 1, k_apply<Program, true, false>, SCAN_BLOCKS);
     } else {
-      launch(k_apply<Program, false, false>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      klaunch(GAS_K_APPLY, 1, k_apply<Program, false, false>, SCAN_BLOCKS);
     }
   }
 
   void scatterActivate(bool /*haveScatter*/ = true) {
-    timedBegin();
-    launch(k_scatter_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-    timedEnd(1);
+    klaunch(GAS_K_SCATTER, 1, k_scatter_vp<Program>, SCAN_BLOCKS);
     endSuperstep();
   }
 
@@ -236,12 +245,10 @@ class GASEngineGPU {
   // wave tiers of the activation fused into the apply kernel.
   void superstep(bool haveGather) {
     if (haveGather) gatherPhase();
-    timedBegin();
     if (haveGather)
-      launch(k_apply<Program, true, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      klaunch(GAS_K_APPLY, 1, k_apply<Program, true, true>, SCAN_BLOCKS);
     else
-      launch(k_apply<Program, false, true>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-    timedEnd(1);
+      klaunch(GAS_K_APPLY, 1, k_apply<Program, false, true>, SCAN_BLOCKS);
     endSuperstep();
   }
 
@@ -363,12 +370,16 @@ class GASEngineGPU {
   }
 
   void resetStats() {
+    collectTiming();
     Ctl c = readCtl();
     c.stat_gathered = c.stat_scattered = c.stat_active = 0;
     c.stat_supersteps = 0;
     check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
     m_gather_ms = m_scatter_ms = 0;
-    m_gather_launches = m_scatter_launches = 0;
+    for (int i = 0; i < GAS_K_SLOTS; ++i) {
+      m_kernel_ms[i] = 0;
+      m_kernel_launches[i] = 0;
+    }
   }
 
   void stats(gas_run_stats *st) {
@@ -381,8 +392,13 @@ class GASEngineGPU {
     st->active_total = c.stat_active;
     st->gather_ms = m_gather_ms;
     st->scatter_ms = m_scatter_ms;
-    st->gather_launches = m_gather_launches;
-    st->scatter_launches = m_scatter_launches;
+    st->gather_launches = m_kernel_launches[GAS_K_GATHER];
+    st->scatter_launches =
+        m_kernel_launches[GAS_K_APPLY] + m_kernel_launches[GAS_K_SCATTER];
+    for (int i = 0; i < GAS_K_SLOTS; ++i) {
+      st->kernel_ms[i] = m_kernel_ms[i];
+      st->kernel_launches[i] = m_kernel_launches[i];
+    }
   }
 
   gasrt_stream_t stream() const { return m_stream; }
@@ -401,28 +417,24 @@ class GASEngineGPU {
   bool outHubs() const { return m_max_out > m_st2; }
 
   void gatherPhase() {
-    timedBegin();
-    launch(k_gather_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    klaunch(GAS_K_GATHER, 0, k_gather_vp<Program>, SCAN_BLOCKS);
     if (inHubs()) {
-      launch(k_hscan1<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hscan2<Program, 0>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hgather<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hfix<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+      klaunch(GAS_K_HSCAN1, 0, k_hscan1<Program, 0>, SCAN_BLOCKS);
+      klaunch(GAS_K_HSCAN2, 0, k_hscan2<Program, 0>, SCAN_BLOCKS);
+      klaunch(GAS_K_HGATHER, 0, k_hgather<Program>, MP_BLOCKS);
+      klaunch(GAS_K_HFIX, 0, k_hfix<Program>, SCAN_BLOCKS);
     }
-    timedEnd(0);
   }
 
   // Hub scatter, compaction of the activation flags, next parity.
   void endSuperstep() {
     if (outHubs()) {
-      timedBegin();
-      launch(k_hscan1<Program, 1>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hscan2<Program, 1>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-      launch(k_hscatter<Program>, MP_BLOCKS, m_stream, m_dev, m_par);
-      timedEnd(1);
+      klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
+      klaunch(GAS_K_HSCAN2, 1, k_hscan2<Program, 1>, SCAN_BLOCKS);
+      klaunch(GAS_K_HSCATTER, 1, k_hscatter<Program>, MP_BLOCKS);
     }
-    launch(k_compact1<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-    launch(k_compact2<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
+    klaunch(GAS_K_COMPACT1, 2, k_compact1<Program>, SCAN_BLOCKS);
+    klaunch(GAS_K_COMPACT2, 2, k_compact2<Program>, SCAN_BLOCKS);
     m_par ^= 1;
   }
 
@@ -552,23 +564,29 @@ class GASEngineGPU {
     buildHubs();
   }
 
-  void timedBegin() {
-    if (!kernel_timing_flag()) return;
-    if (m_tpos == m_timing.size()) {
-      gasrt_event_t a, b;
-      check(gasrt_event_create(&a), "event");
-      check(gasrt_event_create(&b), "event");
-      m_timing.push_back({a, b});
-      m_tkind.push_back(0);
+  // Launch of an engine kernel taking (Dev, parity); with kernel timing on,
+  // bracketed by events and booked under slot `id` and phase `phase`
+  // (0 gather, 1 apply / scatter, 2 other).
+  template <class K>
+  void klaunch(int id, int phase, K k, unsigned grid) {
+    const bool timed = kernel_timing_flag() != 0;
+    if (timed) {
+      if (m_tpos == m_timing.size()) {
+        gasrt_event_t a, b;
+        check(gasrt_event_create(&a), "event");
+        check(gasrt_event_create(&b), "event");
+        m_timing.push_back({a, b});
+        m_tkind.push_back(0);
+      }
+      check(gasrt_event_record(m_timing[m_tpos].first, m_stream), "event");
     }
-    check(gasrt_event_record(m_timing[m_tpos].first, m_stream), "event");
-  }
-  void timedEnd(int kind) {
-    if (!kernel_timing_flag()) return;
-    check(gasrt_event_record(m_timing[m_tpos].second, m_stream), "event");
-    m_tkind[m_tpos] = kind;
-    ++m_tpos;
-    if (m_tpos >= 4096) collectTiming();
+    launch(k, grid, m_stream, m_dev, m_par);
+    if (timed) {
+      check(gasrt_event_record(m_timing[m_tpos].second, m_stream), "event");
+      m_tkind[m_tpos] = id * 4 + phase;
+      ++m_tpos;
+      if (m_tpos >= 4096) collectTiming();
+    }
   }
   void collectTiming() {
     if (m_tpos == 0) return;
@@ -578,13 +596,11 @@ class GASEngineGPU {
       check(gasrt_event_elapsed_ms(m_timing[i].first, m_timing[i].second,
                                    &ms),
             "elapsed");
-      if (m_tkind[i] == 0) {
-        m_gather_ms += ms;
-        ++m_gather_launches;
-      } else {
-        m_scatter_ms += ms;
-        ++m_scatter_launches;
-      }
+      const int id = m_tkind[i] / 4, phase = m_tkind[i] % 4;
+      m_kernel_ms[id] += ms;
+      ++m_kernel_launches[id];
+      if (phase == 0) m_gather_ms += ms;
+      if (phase == 1) m_scatter_ms += ms;
     }
     m_tpos = 0;
   }
@@ -616,7 +632,8 @@ class GASEngineGPU {
   std::vector<int> m_tkind;
   size_t m_tpos = 0;
   double m_gather_ms = 0, m_scatter_ms = 0;
-  int m_gather_launches = 0, m_scatter_launches = 0;
+  double m_kernel_ms[GAS_K_SLOTS] = {};
+  int m_kernel_launches[GAS_K_SLOTS] = {};
 };
 
 }  // namespace gas

@@ -8,7 +8,7 @@
 // size in n_active[p]; the superstep builds the next list in queue p^1.  The
 // host passes p as a kernel argument.
 //
-// Work on an active vertex is split by degree (north-star (a)):
+// Work on an active vertex is split by degree:
 //   degree <= t1        one thread walks the vertex's edges;
 //   t1 < degree <= t2   one wave64 walks them 64 at a time (coalesced) and
 //                       reduces with cross-lane shuffles;
@@ -16,21 +16,21 @@
 //                       are cut into equal merge-path tiles of MP_NV items
 //                       staged through LDS, with a segmented reduce and a
 //                       fold of the per-tile partials of segments that
-//                       cross tiles (north-star (b)).
+//                       cross tiles.
 // The host knows the graph's maximum in/out degree and launches the hub
 // kernels only when some vertex exceeds t2.
 //
 //   k_gather_vp          thread/wave tiers of the gather; appends hubs
 //   k_hscan1/2, k_hgather, k_hfix   hub gather (skipped without hubs)
-//   k_apply              Program::apply (north-star (c)); fused with the
+//   k_apply              Program::apply; fused with the
 //                        thread/wave tiers of the scatter inside run(), or
 //                        recording the return flag for scatterActivate()
 //   k_scatter_vp         scatter tiers when apply was not fused
 //   k_sscan1/2, k_hscatter          hub scatter (skipped without hubs)
 //   k_compact1/2         ordered compaction of the byte activation flags
-//                        into the next active list (north-star (e))
+//                        into the next active list
 //
-// Activation (north-star (d)) writes one byte per destination with a plain
+// Activation writes one byte per destination with a plain
 // store: every writer stores the same value, so no atomics are needed, and
 // the compaction yields the next list in ascending vertex order, as
 // refgas.h:201-210 does.  Wave size is 64.  No MFMA: there is no dense
@@ -950,7 +950,7 @@ __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
   }
 }
 
-// Ordered compaction of the activation flags (north-star (e); replaces
+// Ordered compaction of the activation flags (replaces
 // scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157): pass 1 counts
 // set flags per chunk of 32-bit flag words and ends the superstep's
 // bookkeeping; pass 2 writes the ids of every chunk in ascending order at
@@ -1015,6 +1015,17 @@ __global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
     carry += tot;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) c->n_active[p ^ 1] = total;
+}
+
+// Vertex-data initialisation on the device: `value` everywhere, `one` at
+// vertex `at` (none if at < 0).  Replaces an upload of the whole array when
+// the initial state is uniform, as it is for BFS and SSSP.
+template <class P>
+__global__ void k_fill_vdata(Dev<P> d, typename P::VertexData value,
+                             int32_t at, typename P::VertexData one) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT)
+    d.vdata[v] = v == at ? one : value;
 }
 
 // Superstep counter write (host-side setIteration).

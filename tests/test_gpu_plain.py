@@ -16,7 +16,7 @@ from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-PR_REL_L1 = 1e-5     # north-star tolerance vs FP64
+PR_REL_L1 = 1e-5     # PageRank tolerance: relative L1 vs FP64
 
 
 

@@ -48,10 +48,23 @@ class RunStats(C.Structure):
                 ("gathered_edges", C.c_int64), ("scattered_edges", C.c_int64),
                 ("active_total", C.c_int64), ("ms", C.c_double),
                 ("gather_ms", C.c_double), ("scatter_ms", C.c_double),
-                ("gather_launches", C.c_int32), ("scatter_launches", C.c_int32)]
+                ("gather_launches", C.c_int32), ("scatter_launches", C.c_int32),
+                ("kernel_ms", C.c_double * 12), ("kernel_launches", C.c_int32 * 12)]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        out = {f: getattr(self, f) for f, _ in self._fields_
+               if f not in ("reserved", "kernel_ms", "kernel_launches")}
+        out["kernels"] = self.kernels()
+        return out
+
+    def kernels(self) -> dict:
+        """{kernel name: (summed ms, launches)} of the timed launches."""
+        return {name: (self.kernel_ms[i], self.kernel_launches[i])
+                for i, name in enumerate(KERNEL_NAMES) if self.kernel_launches[i]}
+
+
+KERNEL_NAMES = ["k_gather_vp", "k_apply", "k_scatter_vp", "k_hscan1", "k_hscan2",
+                "k_hgather", "k_hfix", "k_hscatter", "k_compact1", "k_compact2"]
 
 
 _lib = None
@@ -100,6 +113,35 @@ def device_info(device: int = 0) -> dict:
 
 def set_kernel_timing(enable: bool) -> None:
     _check(lib().gas_set_kernel_timing(1 if enable else 0))
+
+
+class DeviceBuffer:
+    """Device memory from gasrt_malloc (freed with the object)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().gasrt_malloc(C.byref(p), C.c_size_t(self.nbytes)))
+        self.ptr = p.value
+
+    def to_numpy(self, dtype=np.int32) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        _check(lib().gasrt_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                      C.c_size_t(out.nbytes), None))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.gasrt_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer) -> None:
+    """Synchronous device-to-device copy of min(dst, src) bytes."""
+    n = min(dst.nbytes, src.nbytes)
+    _check(lib().gasrt_memcpy_d2d(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), C.c_size_t(n),
+                                  None))
+    _check(lib().gasrt_stream_synchronize(None))
 
 
 class Edges:
@@ -263,6 +305,28 @@ class Graph:
         st = RunStats()
         _check(lib().gas_sssp(self._h, C.c_int32(source), _ptr(out), C.byref(st)))
         return out, st
+
+    def bfs_device(self, source: int, out_ptr: int):
+        """BFS with the depths written to device memory at out_ptr
+        (n_vertices int32); returns the run statistics."""
+        st = RunStats()
+        _check(lib().gas_bfs_device(self._h, C.c_int32(source), C.c_void_p(out_ptr),
+                                    C.byref(st)))
+        return st
+
+    def sssp_device(self, source: int, out_ptr: int):
+        """SSSP with the distances written to device memory at out_ptr."""
+        st = RunStats()
+        _check(lib().gas_sssp_device(self._h, C.c_int32(source), C.c_void_p(out_ptr),
+                                     C.byref(st)))
+        return st
+
+    def ppr_device(self, source: int, out_ptr: int, iterations: int = 10):
+        """Personalised PageRank with the ranks written to device memory."""
+        st = RunStats()
+        _check(lib().gas_ppr_device(self._h, C.c_int32(source), C.c_int32(iterations),
+                                    C.c_void_p(out_ptr), C.byref(st)))
+        return st
 
     def cc(self):
         out = np.zeros(self.n_vertices, np.int32)

@@ -97,7 +97,8 @@ static GASEngineGPU<P> &cached_engine(gas_graph_t g, int slot,
       delete static_cast<GASEngineGPU<P> *>(p);
     };
   } else {
-    static_cast<GASEngineGPU<P> *>(g->engine_cache[slot])->resetVertexData(vd);
+    if (vd)
+      static_cast<GASEngineGPU<P> *>(g->engine_cache[slot])->resetVertexData(vd);
   }
   return *static_cast<GASEngineGPU<P> *>(g->engine_cache[slot]);
 }
@@ -123,6 +124,25 @@ struct RunTimer {
     gasrt_event_destroy(b);
   }
 };
+
+// Personalised PageRank initial state on the device: rank 0.15 and the
+// source flag at `source`, 0 elsewhere, out-degree from the CSR offsets.
+__global__ void k_ppr_init(Dev<PersonalizedPageRank> d, int32_t source) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT) {
+    PersonalizedPageRank::VertexData x;
+    x.rank = v == source ? 0.15f : 0.0f;
+    x.numOutEdges = d.csr_off[v + 1] - d.csr_off[v];
+    x.isSource = v == source;
+    d.vdata[v] = x;
+  }
+}
+
+__global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, float *out) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT)
+    out[v] = d.vdata[v].rank;
+}
 
 }  // namespace gas
 
@@ -233,6 +253,29 @@ int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
   });
 }
 
+int gas_bfs_device(gas_graph_t g, int32_t source, int32_t *depth_dev,
+                   gas_run_stats *stats) {
+  if (!g || !depth_dev || source < 0 || source >= g->nV)
+    return set_error(GAS_ERR_ARG, "gas_bfs_device: bad arguments");
+  return guarded([&] {
+    auto &e = cached_engine<BFS>(g, GAS_PROGRAM_BFS, nullptr);
+    e.resetStats();
+    RunTimer tm(e.stream());
+    e.fillVertexData(BFS::VertexData{-1}, -1, BFS::VertexData{-1});
+    e.setActive(source, source + 1);
+    e.setIteration(0);
+    e.runUntilEmpty(false, false);
+    check(gasrt_memcpy_d2d(depth_dev, e.deviceVertexData(),
+                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
+          "result");
+    double ms = tm.stop();
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
 int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
              gas_run_stats *stats) {
   if (!g || !dist_out || source < 0 || source >= g->nV)
@@ -251,6 +294,31 @@ int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
     double ms = tm.stop();
     e.getResults();
     std::copy(vd.begin(), vd.end(), dist_out);
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_sssp_device(gas_graph_t g, int32_t source, int32_t *dist_dev,
+                    gas_run_stats *stats) {
+  if (!g || !dist_dev || source < 0 || source >= g->nV)
+    return set_error(GAS_ERR_ARG, "gas_sssp_device: bad arguments");
+  if (!g->csc_w && g->nE > 0)
+    return set_error(GAS_ERR_ARG, "gas_sssp_device: graph has no edge weights");
+  return guarded([&] {
+    auto &e = cached_engine<SSSP>(g, GAS_PROGRAM_SSSP, nullptr);
+    e.resetStats();
+    RunTimer tm(e.stream());
+    e.fillVertexData(SSSP::gatherZero, source, 0);
+    e.setActive(0, g->nV);
+    e.setIteration(0);
+    e.runUntilEmpty(true, true);
+    check(gasrt_memcpy_d2d(dist_dev, e.deviceVertexData(),
+                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
+          "result");
+    double ms = tm.stop();
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
@@ -346,6 +414,33 @@ int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[v].rank;
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+      stats->supersteps = iterations;
+      stats->gathered_edges = (int64_t)iterations * g->nE;
+    }
+  });
+}
+
+int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
+                   float *rank_dev, gas_run_stats *stats) {
+  if (!g || !rank_dev || source < 0 || source >= g->nV || iterations < 0)
+    return set_error(GAS_ERR_ARG, "gas_ppr_device: bad arguments");
+  return guarded([&] {
+    typedef PersonalizedPageRank PPR;
+    auto &e = cached_engine<PPR>(g, 4, nullptr);
+    e.resetStats();
+    RunTimer tm(e.stream());
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
+    launch(k_ppr_init, grid, e.stream(), e.dev(), source);
+    for (int it = 0; it < iterations; ++it) {
+      e.setActive(0, g->nV);
+      e.gatherApply(true);
+    }
+    launch(k_ppr_rank_out, grid, e.stream(), e.dev(), rank_dev);
+    double ms = tm.stop();
     if (stats) {
       e.stats(stats);
       stats->ms = ms;

@@ -3,7 +3,8 @@ and the plain engine answering the same stream one query at a time.
 
     stream = make_stream(rate_qps=2000, n=1000, graphs=[road, rmat], seed=5)
     served = serve(server, stream)            # continuous batching
-    plain = serve_plain(graphs, stream)       # one query at a time, FIFO
+    svc = plain_service_times(graphs, stream) # one query at a time
+    lat, end = fifo_latencies(stream["arrivals"], svc)
 """
 from __future__ import annotations
 
@@ -57,20 +58,23 @@ def serve(server, stream, keep=None):
 
 
 def plain_service_times(graphs, stream, ppr_iterations: int = 10):
-    """Seconds each query takes on the plain engine (gas_bfs / gas_sssp /
-    gas_ppr: initial state upload, run, result download), measured one
-    query at a time."""
+    """Seconds each query takes on the plain engine answering one query at a
+    time with its result left in HBM (gas_bfs_device / gas_sssp_device /
+    gas_ppr_device: device-side initial state, run, device copy of the
+    result), measured back to back on the host clock."""
+    from . import DeviceBuffer
+    bufs = [DeviceBuffer(4 * g.n_vertices) for g in graphs]
     svc = np.zeros(len(stream["kinds"]))
     for i, (k, gi, s) in enumerate(zip(stream["kinds"], stream["graph_ids"],
                                        stream["sources"])):
-        g = graphs[gi]
+        g, buf = graphs[gi], bufs[gi]
         t = time.perf_counter()
         if k == BATCH_BFS:
-            g.bfs(int(s))
+            g.bfs_device(int(s), buf.ptr)
         elif k == BATCH_SSSP:
-            g.sssp(int(s))
+            g.sssp_device(int(s), buf.ptr)
         else:
-            g.ppr(int(s), ppr_iterations)
+            g.ppr_device(int(s), buf.ptr, ppr_iterations)
         svc[i] = time.perf_counter() - t
     return svc
 
