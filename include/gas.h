@@ -59,6 +59,11 @@ int gasrt_malloc(void **ptr, size_t bytes);                 /* gpuAlloc   */
 int gasrt_free(void *ptr);                                  /* gpuFree    */
 int gasrt_host_alloc(void **ptr, size_t bytes);             /* pinned     */
 int gasrt_host_free(void *ptr);
+/* Pinned host memory that kernels can write (coherent, mapped), with the
+ * device address to hand to kernels; freed with gasrt_host_free.  No
+ * counterpart in the reference beyond its mapped host scalar
+ * (gpugas.h:120-122). */
+int gasrt_host_alloc_mapped(void **host_ptr, void **dev_ptr, size_t bytes);
 int gasrt_memcpy_h2d(void *dst, const void *src, size_t bytes,
                      gasrt_stream_t stream);                /* copyToGPU  */
 int gasrt_memcpy_d2h(void *dst, const void *src, size_t bytes,
@@ -73,6 +78,8 @@ int gasrt_memset(void *dst, int value, size_t bytes, gasrt_stream_t stream);
 int gasrt_stream_create(gasrt_stream_t *stream);
 int gasrt_stream_destroy(gasrt_stream_t stream);
 int gasrt_stream_synchronize(gasrt_stream_t stream);        /* SYNC_CHECK */
+/* *idle = 1 when all work queued on the stream has finished. */
+int gasrt_stream_query(gasrt_stream_t stream, int *idle);
 int gasrt_event_create(gasrt_event_t *event);
 int gasrt_event_destroy(gasrt_event_t event);
 int gasrt_event_record(gasrt_event_t event, gasrt_stream_t stream);
@@ -208,6 +215,14 @@ int gas_set_kernel_timing(int enable);
  * (gpugas.h:421-513). */
 int gas_graph_set_tiers(gas_graph_t g, int program, const int32_t *tiers);
 int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers);
+/* Frontier representation of whole runs: a superstep's scatter builds the
+ * next frontier as a list of ids (claimed with an atomic OR, appended, work
+ * proportional to the frontier) while the last reported frontier has
+ * |A| * divisor < V, and as activation bytes scanned in O(V) otherwise.
+ * divisor 0 = always activation bytes; default 64 for graphs of at least
+ * 2^24 vertices, 0 below.  Replaces the reference's dense / sparse scatter
+ * choice (gpugas.h:669-751). */
+int gas_graph_set_frontier_switch(gas_graph_t g, int program, int32_t divisor);
 /* Autotuner: times candidate tiers on this graph for the program (gather
  * and scatter phases separately, vertex data untouched) and keeps the
  * fastest; tiers_out (may be NULL) receives them. */

@@ -128,10 +128,12 @@ class GASEngineGPU {
 
   GASEngineGPU() {
     check(gasrt_stream_create(&m_stream), "gasrt_stream_create");
-    check(gasrt_host_alloc((void **)&m_hctl, sizeof(Ctl) * kRing),
-          "gasrt_host_alloc");
-    for (int i = 0; i < kRing; ++i)
-      check(gasrt_event_create(&m_ring_ev[i]), "gasrt_event_create");
+    check(gasrt_host_alloc((void **)&m_hctl, sizeof(Ctl)), "gasrt_host_alloc");
+    void *dev_status = nullptr;
+    check(gasrt_host_alloc_mapped((void **)&m_status, &dev_status,
+                                  sizeof(int64_t) * STATUS_SLOTS),
+          "gasrt_host_alloc_mapped");
+    m_dev.status = static_cast<int64_t *>(dev_status);
   }
 
   ~GASEngineGPU() {
@@ -140,8 +142,8 @@ class GASEngineGPU {
       gasrt_event_destroy(pr.first);
       gasrt_event_destroy(pr.second);
     }
-    for (int i = 0; i < kRing; ++i) gasrt_event_destroy(m_ring_ev[i]);
     gasrt_host_free(m_hctl);
+    gasrt_host_free((void *)m_status);
     if (m_own_graph) gas_graph_free(m_own_graph);
     gasrt_stream_destroy(m_stream);
   }
@@ -213,17 +215,24 @@ class GASEngineGPU {
         std::max<int64_t>(m_dev.nfwords, (int64_t)vertexEnd - vertexStart);
     const unsigned grid =
         (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (work + NT - 1) / NT));
+    m_dev.seq = (int32_t)m_seq;
     launch(k_set_active<Program>, grid, m_stream, m_dev, m_par, vertexStart,
            vertexEnd);
+    m_in = (vertexStart == 0 && vertexEnd == m_nV) ? IN_ALL : IN_LIST;
+    m_prev_in = IN_LIST;  // nothing of an earlier superstep left to report
+    m_first = true;
   }
 
   Int countActive() {
+    if (m_in == IN_BITMAP)
+      throw Error(GAS_ERR_STATE, "countActive: frontier held as a bitmap");
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(Ctl), m_stream),
           "countActive");
-    return m_hctl[0].n_active[m_par];
+    return m_hctl->ring[m_seq & 3].n_active;
   }
 
   void gatherApply(bool haveGather = true) {
+    m_out_list = 0;
     if (haveGather) {
       gatherPhase();
       klaunch(GAS_K_APPLY, 1, k_apply<Program, true, false>, SCAN_BLOCKS);
@@ -232,9 +241,12 @@ class GASEngineGPU {
     }
   }
 
+  // Scatter of the step-by-step API: plain activation bytes, then the
+  // ordered compaction, so the next frontier is an ascending list.
   void scatterActivate(bool /*haveScatter*/ = true) {
+    m_out_list = 0;
     klaunch(GAS_K_SCATTER, 1, k_scatter_vp<Program>, SCAN_BLOCKS);
-    endSuperstep();
+    endSuperstep(true);
   }
 
   Int nextIter() { return countActive(); }
@@ -242,44 +254,80 @@ class GASEngineGPU {
   void run() { runUntilEmpty(true, true); }
 
   // One whole superstep: gatherApply + scatterActivate, with the thread and
-  // wave tiers of the activation fused into the apply kernel.
-  void superstep(bool haveGather) {
+  // wave tiers of the activation fused into the apply kernel.  outList:
+  // the scatter builds the next frontier as a list (claims + appends),
+  // otherwise as activation bytes only (bitmap).
+  void superstep(bool haveGather, bool outList) {
+    m_out_list = outList ? 1 : 0;
     if (haveGather) gatherPhase();
     if (haveGather)
       klaunch(GAS_K_APPLY, 1, k_apply<Program, true, true>, SCAN_BLOCKS);
     else
       klaunch(GAS_K_APPLY, 1, k_apply<Program, false, true>, SCAN_BLOCKS);
-    endSuperstep();
+    endSuperstep(false);
   }
 
-  // Supersteps until the frontier is empty, checking the device count a
-  // few supersteps behind so the host never idles the GPU.
+  // Supersteps until the frontier is empty.  The host enqueues supersteps
+  // without waiting; every superstep's frontier size reaches the mapped
+  // status ring (list and all inputs from their own first kernel, bitmap
+  // inputs from the next superstep's), and the host consumes the reports in
+  // order, staying at most kDepth + 1 supersteps ahead of the last one.  The
+  // scatter of each superstep builds a list when the last reported frontier
+  // is sparse (|A| * switch < V), a bitmap otherwise.  The run ends at the
+  // first reported empty frontier; supersteps enqueued after it are no-ops.
   void runUntilEmpty(bool haveGather, bool /*haveScatter*/) {
-    if (countActive() == 0) return;
-    std::deque<int> inflight;
-    int j = 0;
-    for (;;) {
-      superstep(haveGather);
-      const int slot = j % kRing;
-      m_ring_par[slot] = m_par;
-      check(gasrt_memcpy_d2h_async(&m_hctl[slot], m_ctl.p, sizeof(Ctl),
-                                   m_stream),
-            "count readback");
-      check(gasrt_event_record(m_ring_ev[slot], m_stream), "event");
-      inflight.push_back(j++);
-      if ((int)inflight.size() >= kDepth) {
-        int o = inflight.front();
-        inflight.pop_front();
-        check(gasrt_event_synchronize(m_ring_ev[o % kRing]), "event sync");
-        if (m_hctl[o % kRing].n_active[m_ring_par[o % kRing]] == 0) break;
+    const int64_t a0 = countActive();
+    if (a0 == 0) return;
+    for (int i = 0; i < STATUS_SLOTS; ++i)
+      __atomic_store_n(m_status + i, (int64_t)0, __ATOMIC_RELEASE);
+    const int64_t s0 = m_seq;
+    const int it0 = m_iter;
+    int64_t next = s0, last_a = a0, empty_at = -1;
+    // consumes every report that has arrived, in superstep order
+    auto poll = [&]() {
+      bool got = false;
+      while (empty_at < 0) {
+        const int64_t v = __atomic_load_n(m_status + (next & (STATUS_SLOTS - 1)),
+                                          __ATOMIC_ACQUIRE);
+        if ((v >> 32) != next + 1) break;
+        last_a = v & 0xffffffffll;
+        if (last_a == 0) empty_at = next;
+        ++next;
+        got = true;
+      }
+      return got;
+    };
+    while (empty_at < 0) {
+      superstep(haveGather, m_switch > 0 &&
+                                last_a * (int64_t)m_switch < (int64_t)m_nV);
+      poll();
+      for (int64_t spin = 0; empty_at < 0 && m_seq - next > kDepth + 1; ++spin) {
+        if (poll()) continue;
+        if ((spin & 255) == 255) {
+          int idle = 0;
+          check(gasrt_stream_query(m_stream, &idle), "stream query");
+          if (idle && !poll())
+            throw Error(GAS_ERR_STATE, "superstep status was not reported");
+        }
+        __builtin_ia32_pause();
       }
     }
     check(gasrt_stream_synchronize(m_stream), "sync");
+    const int64_t real = empty_at - s0;  // supersteps with a non-empty frontier
+    m_supersteps -= (m_seq - s0) - real;
+    m_iter = it0 + (int)real;
+    setActive(0, 0);
   }
 
-  void setIteration(int it) {
-    launch(k_set_iteration<Program>, 1, m_stream, m_dev, it);
+  void setIteration(int it) { m_iter = it; }
+
+  // Frontier density switch: the scatter builds a list while
+  // |A| * divisor < V, a bitmap otherwise; 0 = always a bitmap.
+  void setSwitchDivisor(int divisor) {
+    if (divisor < 0) throw Error(GAS_ERR_ARG, "switch divisor must be >= 0");
+    m_switch = divisor;
   }
+  int switchDivisor() const { return m_switch; }
 
   // Degree tiers of the gather (g) and the scatter (s):
   // thread <= t1 < wave <= t2 < merge-path hub.
@@ -320,8 +368,13 @@ class GASEngineGPU {
         if (gather) {
           gatherPhase();
         } else {
-          launch(k_scatter_vp<Program>, SCAN_BLOCKS, m_stream, m_dev, m_par);
-          endSuperstep();
+          m_out_list = 0;
+          klaunch(GAS_K_SCATTER, 1, k_scatter_vp<Program>, SCAN_BLOCKS);
+          if (outHubs()) {
+            klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
+            klaunch(GAS_K_HSCAN2, 1, k_hscan2<Program, 1>, SCAN_BLOCKS);
+            klaunch(GAS_K_HSCATTER, 1, k_hscatter<Program>, MP_BLOCKS);
+          }
         }
         check(gasrt_event_record(b, m_stream), "event");
         check(gasrt_event_synchronize(b), "event");
@@ -366,15 +419,15 @@ class GASEngineGPU {
   Ctl readCtl() {
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(Ctl), m_stream),
           "readCtl");
-    return m_hctl[0];
+    return *m_hctl;
   }
 
   void resetStats() {
     collectTiming();
-    Ctl c = readCtl();
-    c.stat_gathered = c.stat_scattered = c.stat_active = 0;
-    c.stat_supersteps = 0;
-    check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
+    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * 3 * SCAN_BLOCKS,
+                       m_stream),
+          "memset");
+    m_supersteps = 0;
     m_gather_ms = m_scatter_ms = 0;
     for (int i = 0; i < GAS_K_SLOTS; ++i) {
       m_kernel_ms[i] = 0;
@@ -384,12 +437,18 @@ class GASEngineGPU {
 
   void stats(gas_run_stats *st) {
     collectTiming();
-    Ctl c = readCtl();
-    st->supersteps = (int32_t)c.stat_supersteps;
+    std::vector<int64_t> acc((size_t)3 * SCAN_BLOCKS);
+    check(gasrt_memcpy_d2h(acc.data(), m_stat.p, sizeof(int64_t) * acc.size(),
+                           m_stream),
+          "stats");
+    int64_t tot[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k)
+      for (int b = 0; b < SCAN_BLOCKS; ++b) tot[k] += acc[(size_t)k * SCAN_BLOCKS + b];
+    st->supersteps = (int32_t)m_supersteps;
     st->reserved = 0;
-    st->gathered_edges = c.stat_gathered;
-    st->scattered_edges = c.stat_scattered;
-    st->active_total = c.stat_active;
+    st->gathered_edges = tot[0];
+    st->scattered_edges = tot[1];
+    st->active_total = tot[2];
     st->gather_ms = m_gather_ms;
     st->scatter_ms = m_scatter_ms;
     st->gather_launches = m_kernel_launches[GAS_K_GATHER];
@@ -410,7 +469,6 @@ class GASEngineGPU {
   const Dev<Program> &dev() const { return m_dev; }
 
  private:
-  static constexpr int kRing = 8;
   static constexpr int kDepth = 3;
 
   bool inHubs() const { return m_max_in > m_gt2; }
@@ -426,16 +484,34 @@ class GASEngineGPU {
     }
   }
 
-  // Hub scatter, compaction of the activation flags, next parity.
-  void endSuperstep() {
+  // Hub scatter, optional ordered compaction, and the move to the next
+  // superstep: parity, counter-ring index, input mode of the next frontier.
+  void endSuperstep(bool compact) {
     if (outHubs()) {
       klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
       klaunch(GAS_K_HSCAN2, 1, k_hscan2<Program, 1>, SCAN_BLOCKS);
       klaunch(GAS_K_HSCATTER, 1, k_hscatter<Program>, MP_BLOCKS);
     }
-    klaunch(GAS_K_COMPACT1, 2, k_compact1<Program>, SCAN_BLOCKS);
-    klaunch(GAS_K_COMPACT2, 2, k_compact2<Program>, SCAN_BLOCKS);
+    if (compact) {
+      klaunch(GAS_K_COMPACT1, 2, k_compact1<Program>, SCAN_BLOCKS);
+      klaunch(GAS_K_COMPACT2, 2, k_compact2<Program>, SCAN_BLOCKS);
+    }
+    m_prev_in = m_in;
+    m_in = (compact || m_out_list) ? IN_LIST : IN_BITMAP;
     m_par ^= 1;
+    ++m_seq;
+    ++m_iter;
+    ++m_supersteps;
+    m_first = true;
+  }
+
+  // What the first kernel of the current superstep reports.
+  int pubKind() const {
+    int k = 0;
+    if (m_in == IN_LIST) k |= PUB_OWN_LIST;
+    if (m_in == IN_ALL) k |= PUB_OWN_ALL;
+    if (m_prev_in == IN_BITMAP) k |= PUB_PREV_BITMAP;
+    return k;
   }
 
   // Gather hubs of an all-active superstep: vertices with in-degree > t2,
@@ -509,7 +585,8 @@ class GASEngineGPU {
     m_ctl.alloc(1);
     m_q0.alloc(V);
     m_q1.alloc(V);
-    m_flags.alloc((size_t)nfwords * 4);
+    m_fl0.alloc((size_t)nfwords * 4);
+    m_fl1.alloc((size_t)nfwords * 4);
     m_rflag.alloc(V);
     m_gtmp.alloc(V);
     m_ghub.alloc(V);
@@ -520,11 +597,16 @@ class GASEngineGPU {
     m_shpart.alloc(tiles);
     m_bsum.alloc(SCAN_BLOCKS);
     m_stat.alloc(3 * SCAN_BLOCKS);
+    m_bcnt.alloc(2 * SCAN_BLOCKS);
     m_rpart.alloc(SCAN_BLOCKS);
     m_pfirst.alloc(tiles);
     m_plast.alloc(tiles);
-    check(gasrt_memset(m_flags.p, 0, (size_t)nfwords * 4, m_stream), "memset");
-    check(gasrt_memset(m_stat.p, 0, sizeof(int32_t) * 3 * SCAN_BLOCKS,
+    check(gasrt_memset(m_fl0.p, 0, (size_t)nfwords * 4, m_stream), "memset");
+    check(gasrt_memset(m_fl1.p, 0, (size_t)nfwords * 4, m_stream), "memset");
+    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * 3 * SCAN_BLOCKS,
+                       m_stream),
+          "memset");
+    check(gasrt_memset(m_bcnt.p, 0, sizeof(int32_t) * 2 * SCAN_BLOCKS,
                        m_stream),
           "memset");
     check(gasrt_memset(m_rpart.p, 0, sizeof(double) * SCAN_BLOCKS, m_stream),
@@ -545,7 +627,8 @@ class GASEngineGPU {
     d.ctl = m_ctl.p;
     d.q[0] = m_q0.p;
     d.q[1] = m_q1.p;
-    d.flags = m_flags.p;
+    d.fl[0] = m_fl0.p;
+    d.fl[1] = m_fl1.p;
     d.nfwords = nfwords;
     d.rflag = m_rflag.p;
     d.gtmp = m_gtmp.p;
@@ -556,12 +639,20 @@ class GASEngineGPU {
     d.shoff = m_shoff.p;
     d.shpart = m_shpart.p;
     d.bsum = m_bsum.p;
-    d.stat_blk = m_stat.p;
+    d.bcnt = m_bcnt.p;
+    d.stat_acc = m_stat.p;
     d.rpart = m_rpart.p;
     d.part_first = m_pfirst.p;
     d.part_last = m_plast.p;
     m_par = 0;
+    m_in = m_prev_in = IN_LIST;
+    // Below 2^24 vertices the bitmap scan (V bytes per frontier kernel,
+    // staged through LDS) is cheaper than claiming destinations with
+    // atomics even for one-vertex frontiers; above, sparse frontiers use
+    // the list.
+    m_switch = m_nV >= (1 << 24) ? 64 : 0;
     buildHubs();
+    setActive(0, 0);
   }
 
   // Launch of an engine kernel taking (Dev, parity); with kernel timing on,
@@ -580,6 +671,13 @@ class GASEngineGPU {
       }
       check(gasrt_event_record(m_timing[m_tpos].first, m_stream), "event");
     }
+    m_dev.seq = (int32_t)m_seq;
+    m_dev.iter = m_iter;
+    m_dev.in_mode = m_in;
+    m_dev.out_list = m_out_list;
+    m_dev.first = m_first ? 1 : 0;
+    m_dev.pub = m_first ? pubKind() : 0;
+    m_first = false;
     launch(k, grid, m_stream, m_dev, m_par);
     if (timed) {
       check(gasrt_event_record(m_timing[m_tpos].second, m_stream), "event");
@@ -606,9 +704,15 @@ class GASEngineGPU {
   }
 
   gasrt_stream_t m_stream = nullptr;
-  Ctl *m_hctl = nullptr;  // pinned ring of control-block snapshots
-  gasrt_event_t m_ring_ev[kRing] = {};
-  int m_ring_par[kRing] = {};
+  Ctl *m_hctl = nullptr;            // pinned copy of the control block
+  int64_t *m_status = nullptr;      // mapped status ring (STATUS_SLOTS)
+  int64_t m_seq = 0;                // superstep number
+  int m_iter = 0;                   // program-visible superstep counter
+  int64_t m_supersteps = 0;         // supersteps with work since resetStats
+  int m_in = IN_LIST, m_prev_in = IN_LIST;  // input mode: this / last superstep
+  int m_out_list = 0;               // scatter of this superstep builds a list
+  bool m_first = true;              // next launch is the superstep's first
+  int m_switch = 0;                 // density switch divisor (0: bitmap only)
   gas_graph_t m_graph = nullptr;
   gas_graph_t m_own_graph = nullptr;
   Int m_nV = 0, m_nE = 0;
@@ -622,8 +726,9 @@ class GASEngineGPU {
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;
   DevBuf<int32_t> m_q0, m_q1, m_ghub, m_ghoff, m_ghpart, m_shub, m_shoff,
-      m_shpart, m_bsum, m_stat, m_ahub, m_ahoff, m_ahpart;
-  DevBuf<uint8_t> m_flags, m_rflag;
+      m_shpart, m_bsum, m_bcnt, m_ahub, m_ahoff, m_ahpart;
+  DevBuf<int64_t> m_stat;
+  DevBuf<uint8_t> m_fl0, m_fl1, m_rflag;
   DevBuf<double> m_rpart;
   DevBuf<GatherResult> m_gtmp;
   DevBuf<Acc<Program>> m_pfirst, m_plast;

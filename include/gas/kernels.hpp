@@ -1,12 +1,25 @@
 // gfx950 kernels of the GAS superstep, templated on the vertex program.
 //
 // One superstep of the reference engine (gpugas.h:421-751, semantics of
-// refgas.h:143-210) runs here as a fixed sequence of kernels whose sizes
-// all come from a device-resident control block, so the host never waits
-// for a frontier size.  Supersteps alternate between two parities p = 0, 1:
-// the active list of a superstep with parity p lives in queue p and its
-// size in n_active[p]; the superstep builds the next list in queue p^1.  The
-// host passes p as a kernel argument.
+// refgas.h:143-210) runs here as a short fixed sequence of kernels whose
+// sizes all come from device memory, so the host never waits for a frontier
+// size before enqueuing the next superstep.  Supersteps alternate parity
+// p = 0, 1 (input list q[p] / activation bytes fl[p], next ones q[p^1] /
+// fl[p^1]); per-superstep counters live in a ring of four entries indexed by
+// the superstep number, so no kernel reads a counter the same launch writes
+// and no kernel needs to know it is the last workgroup to finish.
+//
+// The frontier of a superstep is one of
+//   all      every vertex (setActive(0, V));
+//   list     the ids in q[p] -- built by the previous superstep's scatter,
+//            which claims each destination with an atomicOr on its
+//            activation byte and appends the winners (O(|frontier|) work);
+//   bitmap   the activation bytes fl[p] -- written by the previous
+//            superstep's scatter with plain stores (no atomics), scanned
+//            in O(V).
+// The host picks, per superstep, whether the scatter builds a list or a
+// bitmap from the last frontier size the kernels reported (sparse: list;
+// dense: bitmap), the switch driven by frontier density.
 //
 // Work on an active vertex is split by degree:
 //   degree <= t1        one thread walks the vertex's edges;
@@ -22,19 +35,19 @@
 //
 //   k_gather_vp          thread/wave tiers of the gather; appends hubs
 //   k_hscan1/2, k_hgather, k_hfix   hub gather (skipped without hubs)
-//   k_apply              Program::apply; fused with the
-//                        thread/wave tiers of the scatter inside run(), or
-//                        recording the return flag for scatterActivate()
+//   k_apply              Program::apply; fused with the thread/wave tiers
+//                        of the scatter inside run(), or recording the
+//                        return flag for scatterActivate()
 //   k_scatter_vp         scatter tiers when apply was not fused
-//   k_sscan1/2, k_hscatter          hub scatter (skipped without hubs)
-//   k_compact1/2         ordered compaction of the byte activation flags
-//                        into the next active list
+//   k_hscan1/2, k_hscatter          hub scatter (skipped without hubs)
+//   k_compact1/2         ordered compaction of activation bytes into a list
+//                        (the step-by-step API, whose next frontier is in
+//                        ascending vertex order as refgas.h:201-210)
 //
-// Activation writes one byte per destination with a plain
-// store: every writer stores the same value, so no atomics are needed, and
-// the compaction yields the next list in ascending vertex order, as
-// refgas.h:201-210 does.  Wave size is 64.  No MFMA: there is no dense
-// contraction on this path.
+// The first kernel of a superstep reports a frontier size to a host-mapped
+// status word; the host reads it a few supersteps behind and stops once a
+// reported frontier is empty (later supersteps are no-ops).  Wave size is
+// 64.  No MFMA: there is no dense contraction on this path.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,20 +84,44 @@ __host__ inline int iteration() { return host_superstep(); }
 __device__ inline int iteration() { return s_superstep; }
 
 // ---------------------------------------------------------------------------
+// Per-superstep counters, in a ring of four entries indexed by the
+// superstep number s & 3.  Superstep s reads its own entry (list size, hub
+// lists), appends the next active list into entry (s + 1) & 3, and its first
+// kernel clears entry (s + 2) & 3 -- the one superstep s + 1 appends into,
+// last used by superstep s - 2.  No kernel ever clears an entry another
+// kernel of the same launch reads or writes.
+struct StepCounts {
+  int32_t n_active;    // |A| of a list input (appended by superstep s - 1)
+  int32_t n_ghub;      // gather hubs appended by this superstep
+  int32_t ghub_items;  // their summed in-degree
+  int32_t n_shub;      // scatter hubs appended by this superstep
+  int32_t shub_items;  // their summed out-degree
+  int32_t pad[3];
+};
+constexpr int STEP_COUNT_WORDS = sizeof(StepCounts) / 4;
+
 // Control block, one per engine, in device memory.
 struct Ctl {
-  int32_t n_active[2];    // |A| of the superstep of each parity
-  int32_t all_active[2];  // 1: A is implicitly [0, V)
-  int32_t iteration;      // supersteps completed (BFS level)
-  int32_t n_ghub;         // gather hubs appended this superstep
-  int32_t ghub_items;     // their summed in-degree
-  int32_t n_shub;         // scatter hubs appended this superstep
-  int32_t shub_items;     // their summed out-degree
-  int32_t pad[3];
-  int64_t stat_gathered;  // totals over the run
-  int64_t stat_scattered;
-  int64_t stat_active;
-  int64_t stat_supersteps;
+  StepCounts ring[4];
+};
+
+// How a superstep finds its active vertices.
+enum : int32_t {
+  IN_ALL = 0,     // every vertex, slot i is vertex i
+  IN_LIST = 1,    // slots index the list q[p] of ring[s & 3].n_active ids
+  IN_BITMAP = 2,  // slot i is vertex i, active iff its activation byte is set
+};
+
+// What the first kernel of superstep s reports to the host status ring:
+// slot r & (STATUS_SLOTS - 1) receives ((r + 1) << 32) | |A(r)| for the
+// reported superstep r.  Every superstep is reported exactly once: a list
+// or all input by itself, a bitmap input by the next superstep (its size is
+// the sum of per-workgroup counts, known only after its apply kernel).
+constexpr int STATUS_SLOTS = 16;
+enum : int32_t {
+  PUB_OWN_LIST = 1,     // (s, list size)
+  PUB_OWN_ALL = 2,      // (s, V)
+  PUB_PREV_BITMAP = 4,  // (s - 1, sum of the per-workgroup counts of s - 1)
 };
 
 // Optional program trait: `static constexpr bool compensatedSum = true`
@@ -275,22 +312,31 @@ struct Dev {
   VD *vdata;
   Ctl *ctl;
   int32_t *q[2];     // active lists by parity
-  uint8_t *flags;    // activation byte per vertex (V rounded up to 4)
-  int32_t nfwords;   // flags as 32-bit words
+  uint8_t *fl[2];    // activation bytes by parity (V rounded up to 4)
+  int32_t nfwords;   // activation bytes as 32-bit words
   uint8_t *rflag;    // apply's return per active slot (unfused path)
   GR *gtmp;          // gather result per active slot
   int32_t gt1, gt2;  // gather degree tiers (thread <= gt1 < wave <= gt2 < hub)
   int32_t st1, st2;  // scatter degree tiers
-  // gather hubs of the current superstep (queue mode): active slots
+  // gather hubs of the current superstep (list / bitmap input): slots
   int32_t *ghub, *ghoff, *ghpart;
   // gather hubs when every vertex is active (fixed per graph): vertex ids
   const int32_t *ahub, *ahoff, *ahpart;
   int32_t n_ahub, ahub_items;
-  // scatter hubs of the current superstep: active slots
+  // scatter hubs of the current superstep: slots
   int32_t *shub, *shoff, *shpart;
   int32_t *bsum;     // per-chunk sums of the two-pass scans
-  int32_t *stat_blk; // per-workgroup counters: gathered | scattered
+  int32_t *bcnt;     // [2][SCAN_BLOCKS] active slots per workgroup, bitmap input
+  int64_t *stat_acc; // [3][SCAN_BLOCKS] per-workgroup totals: gathered, scattered, active
   double *rpart;     // per-chunk FP64 residual sums
+  int64_t *status;   // host-mapped status ring of STATUS_SLOTS words (PUB_*)
+  // set by the host for every launch
+  int32_t seq;       // superstep number (ring index seq & 3)
+  int32_t iter;      // the program-visible superstep counter
+  int32_t in_mode;   // IN_*
+  int32_t out_list;  // 1: scatter claims and appends the next list; 0: bytes only
+  int32_t first;     // 1: first kernel of the superstep (clears, publishes)
+  int32_t pub;       // PUB_* bits
   Acc<P> *part_first, *part_last;  // per-tile partials of crossing segments
 };
 
@@ -434,53 +480,180 @@ __device__ inline void block_append(bool take, int slot, int32_t *list,
 }
 
 // ---------------------------------------------------------------------------
+// Frontier of the current superstep.
+template <class P>
+__device__ inline int frontier_size(const Dev<P> &d) {
+  return d.in_mode == IN_LIST ? d.ctl->ring[d.seq & 3].n_active : d.nV;
+}
+
+// Vertex of slot i (< frontier_size) and whether it is active.
+template <class P>
+__device__ inline bool frontier_vertex(const Dev<P> &d, int p, int i, int &v) {
+  if (d.in_mode == IN_LIST) {
+    v = d.q[p][i];
+    return true;
+  }
+  v = i;
+  return d.in_mode == IN_ALL || d.fl[p][i] != 0;
+}
+
+// Vertex of a slot stored in a hub list.
+template <class P>
+__device__ inline int slot_vertex(const Dev<P> &d, int p, int slot) {
+  return d.in_mode == IN_LIST ? d.q[p][slot] : slot;
+}
+
+// Duties of the first kernel of a superstep, done by its workgroup 0: clear
+// ring entry (s + 2) & 3 and report frontier sizes to the host status ring
+// (system-scope stores into host-mapped memory; the host reads them while
+// later supersteps run).  Must be reached by every thread of the workgroup.
+template <class P>
+__device__ inline void superstep_prologue(const Dev<P> &d, int p, int *lds) {
+  if (!d.first || blockIdx.x != 0) return;
+  Ctl *c = d.ctl;
+  if (threadIdx.x < STEP_COUNT_WORDS)
+    reinterpret_cast<int32_t *>(&c->ring[(d.seq + 2) & 3])[threadIdx.x] = 0;
+  if (d.pub & PUB_PREV_BITMAP) {
+    int t = 0;
+    for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT)
+      t += d.bcnt[(p ^ 1) * SCAN_BLOCKS + i];
+    t = block_sum(t, lds);
+    const int64_t r = d.seq - 1;
+    if (threadIdx.x == 0)
+      __hip_atomic_store(d.status + (r & (STATUS_SLOTS - 1)),
+                         ((r + 1) << 32) | (int64_t)(uint32_t)t,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  if ((d.pub & (PUB_OWN_LIST | PUB_OWN_ALL)) && threadIdx.x == 0) {
+    const int64_t r = d.seq;
+    const int a = (d.pub & PUB_OWN_LIST) ? c->ring[d.seq & 3].n_active : d.nV;
+    __hip_atomic_store(d.status + (r & (STATUS_SLOTS - 1)),
+                       ((r + 1) << 32) | (int64_t)(uint32_t)a,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Claims vertex u for the next list: sets its activation byte and returns
+// true for exactly one of the threads that activate u in this superstep.
+__device__ inline bool claim_vertex(uint8_t *fl, int u) {
+  if (fl[u]) return false;  // already set (bytes only go 0 -> 1 here)
+  const uint32_t bit = 1u << ((u & 3) * 8);
+  const uint32_t old = atomicOr(reinterpret_cast<uint32_t *>(fl) + (u >> 2), bit);
+  return (old & bit) == 0;
+}
+
+// Per-workgroup run totals (0 gathered, 1 scattered, 2 active): each
+// workgroup adds into its own slot, so no atomics.
+template <class P>
+__device__ inline void add_stat(const Dev<P> &d, int which, int64_t x) {
+  if (threadIdx.x == 0 && x) d.stat_acc[which * SCAN_BLOCKS + blockIdx.x] += x;
+}
+
+// Calls body(valid, slot, vertex) for every active slot of the superstep,
+// in rounds of NT that every thread of the workgroup reaches (body may use
+// workgroup barriers).  All / list inputs: slot i = round * stride + thread.
+// Bitmap input: the workgroup reads its chunk of activation words
+// (BM_WORDS at a time, 4 slots per word), compacts the active vertex ids
+// into LDS in ascending order and processes them NT per round, so a sparse
+// bitmap costs one read of its words plus one round; with clear = true the
+// words read are zeroed (the last reader of this superstep's bitmap).
+constexpr int BM_WORDS = 2 * NT;
+template <class P, class Body>
+__device__ inline void for_each_active(const Dev<P> &d, int p, bool clear,
+                                       int32_t *s_ids, int *lds, Body body) {
+  if (d.in_mode != IN_BITMAP) {
+    const int A = frontier_size(d);
+    const int stride = gridDim.x * NT;
+    const int rounds = (A + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+      const int i = r * stride + blockIdx.x * NT + threadIdx.x;
+      int v = 0;
+      const bool valid = i < A && frontier_vertex(d, p, i, v);
+      body(valid, i, v);
+    }
+    return;
+  }
+  uint32_t *fw = reinterpret_cast<uint32_t *>(d.fl[p]);
+  int wb, we;
+  scan_chunk(d.nfwords, blockIdx.x, wb, we);
+  for (int w0 = wb; w0 < we; w0 += BM_WORDS) {
+    uint32_t wd[BM_WORDS / NT];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < BM_WORDS / NT; ++k) {
+      const int wi = w0 + k * NT + threadIdx.x;
+      wd[k] = wi < we ? fw[wi] : 0u;
+      cnt += __popc(wd[k]);
+    }
+    int tot;
+    int pos = block_excl_scan(cnt, tot, lds);
+#pragma unroll
+    for (int k = 0; k < BM_WORDS / NT; ++k) {
+      if (wd[k]) {
+        const int wi = w0 + k * NT + threadIdx.x;
+        if (clear) fw[wi] = 0u;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if ((wd[k] >> (8 * b)) & 0xffu) s_ids[pos++] = wi * 4 + b;
+      }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < tot; j0 += NT) {
+      const int j = j0 + threadIdx.x;
+      const bool valid = j < tot;
+      const int v = valid ? s_ids[j] : 0;
+      body(valid, v, v);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Frontier initialisation: setActive(start, end) (gpugas.h:342-349); clears
-// the activation flags and the hub counters.
+// the activation bytes of both parities and the counter ring, and lists
+// [start, end) unless it is every vertex.
 template <class P>
 __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
                                                    int end) {
-  Ctl *c = d.ctl;
   const bool all = (start == 0 && end == d.nV);
   const int n = end - start;
   int32_t *q = d.q[p];
   const int stride = gridDim.x * NT;
-  uint32_t *fw = reinterpret_cast<uint32_t *>(d.flags);
-  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nfwords; i += stride)
-    fw[i] = 0u;
+  uint32_t *f0 = reinterpret_cast<uint32_t *>(d.fl[0]);
+  uint32_t *f1 = reinterpret_cast<uint32_t *>(d.fl[1]);
+  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nfwords; i += stride) {
+    f0[i] = 0u;
+    f1[i] = 0u;
+  }
   if (!all)
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += stride)
       q[i] = start + i;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->n_active[p] = n;
-    c->all_active[p] = all ? 1 : 0;
-    c->n_ghub = 0;
-    c->n_shub = 0;
+  if (blockIdx.x == 0) {
+    int32_t *ring = reinterpret_cast<int32_t *>(d.ctl->ring);
+    if (threadIdx.x < 4 * STEP_COUNT_WORDS) ring[threadIdx.x] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) d.ctl->ring[d.seq & 3].n_active = n;
   }
 }
 
 // Thread and wave tiers of the gather; hub slots are appended to ghub
-// (queue mode) or skipped (all active: the fixed hub list covers them).
+// (list / bitmap input) or skipped (all active: the fixed hub list covers
+// them).
 template <class P>
 __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   __shared__ int32_t s_buf[NT];
+  __shared__ int32_t s_ids[BM_WORDS * 4];
   __shared__ int s_cnt, s_base;
   __shared__ int lds[NT / WAVE];
+  superstep_prologue(d, p, lds);
   Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  const bool all = c->all_active[p] != 0;
-  const int32_t *q = d.q[p];
-  const int stride = gridDim.x * NT;
-  const int rounds = (A + stride - 1) / stride;
+  const bool all = d.in_mode == IN_ALL;
   const int lane = lane_id();
   int gathered = 0;
-  for (int r = 0; r < rounds; ++r) {
-    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
-    const bool valid = i < A;
-    int v = 0, o = 0, deg = 0;
+  for_each_active(d, p, false, s_ids, lds, [&](bool valid, int i, int v) {
+    int o = 0, deg = 0;
     if (valid) {
-      v = all ? i : q[i];
       o = d.csc_off[v];
       deg = d.csc_off[v + 1] - o;
       gathered += deg;
@@ -512,6 +685,7 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
       const int vj = __shfl(v, j, WAVE);
       const int oj = __shfl(o, j, WAVE);
       const int dj = __shfl(deg, j, WAVE);
+      const int ij = __shfl(i, j, WAVE);
       Acc<P> acc = Acc<P>::empty();
       for (int base = 0; base < dj; base += WAVE * MID_UNROLL) {
         int src[MID_UNROLL];
@@ -534,12 +708,14 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
       }
 #pragma unroll
       for (int x = WAVE / 2; x > 0; x >>= 1) acc.merge(acc.xor_peer(x));
-      if (lane == j) d.gtmp[i] = acc.finish();
+      if (lane == 0) d.gtmp[ij] = acc.finish();
     }
-    if (!all) block_append(hub, i, d.ghub, &c->n_ghub, s_buf, &s_cnt, &s_base);
-  }
+    if (!all)
+      block_append(hub, i, d.ghub, &c->ring[d.seq & 3].n_ghub, s_buf, &s_cnt,
+                   &s_base);
+  });
   gathered = block_sum(gathered, lds);
-  if (threadIdx.x == 0) d.stat_blk[blockIdx.x] = gathered;
+  add_stat(d, 0, gathered);
 }
 
 // Hub list of a superstep: all-active supersteps use the fixed list.
@@ -550,21 +726,21 @@ struct HubList {
 };
 
 template <class P>
-__device__ inline HubList<P> gather_hubs(const Dev<P> &d, const Ctl *c,
-                                         bool all) {
+__device__ inline HubList<P> gather_hubs(const Dev<P> &d) {
   HubList<P> h;
-  if (all) {
+  if (d.in_mode == IN_ALL) {
     h.list = d.ahub;
     h.off = d.ahoff;
     h.part = d.ahpart;
     h.n = d.n_ahub;
     h.items = d.ahub_items;
   } else {
+    const StepCounts &sc = d.ctl->ring[d.seq & 3];
     h.list = d.ghub;
     h.off = d.ghoff;
     h.part = d.ghpart;
-    h.n = c->n_ghub;
-    h.items = c->ghub_items;
+    h.n = sc.n_ghub;
+    h.items = sc.ghub_items;
   }
   return h;
 }
@@ -574,20 +750,16 @@ __device__ inline HubList<P> gather_hubs(const Dev<P> &d, const Ctl *c,
 template <class P, int Which>
 __global__ __launch_bounds__(NT) void k_hscan1(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  const Ctl *c = d.ctl;
-  if (c->n_active[p] == 0) return;
-  const bool all = c->all_active[p] != 0;
-  if (Which == 0 && all) return;  // fixed hub list, scanned on the host
-  const int n = Which == 0 ? c->n_ghub : c->n_shub;
+  if (Which == 0 && d.in_mode == IN_ALL) return;  // fixed list, scanned on the host
+  const StepCounts &sc = d.ctl->ring[d.seq & 3];
+  const int n = Which == 0 ? sc.n_ghub : sc.n_shub;
   const int32_t *list = Which == 0 ? d.ghub : d.shub;
   const int32_t *gofs = Which == 0 ? d.csc_off : d.csr_off;
-  const int32_t *q = d.q[p];
   int b, e;
   scan_chunk(n, blockIdx.x, b, e);
   int sum = 0;
   for (int i = b + threadIdx.x; i < e; i += NT) {
-    const int slot = list[i];
-    const int v = all ? slot : q[slot];
+    const int v = slot_vertex(d, p, list[i]);
     sum += gofs[v + 1] - gofs[v];
   }
   sum = block_sum(sum, lds);
@@ -598,17 +770,14 @@ __global__ __launch_bounds__(NT) void k_hscan1(Dev<P> d, int p) {
 template <class P, int Which>
 __global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  Ctl *c = d.ctl;
-  if (c->n_active[p] == 0) return;
-  const bool all = c->all_active[p] != 0;
-  if (Which == 0 && all) return;
-  const int n = Which == 0 ? c->n_ghub : c->n_shub;
+  if (Which == 0 && d.in_mode == IN_ALL) return;
+  StepCounts &sc = d.ctl->ring[d.seq & 3];
+  const int n = Which == 0 ? sc.n_ghub : sc.n_shub;
   if (n == 0) return;
   const int32_t *list = Which == 0 ? d.ghub : d.shub;
   int32_t *off = Which == 0 ? d.ghoff : d.shoff;
   int32_t *part = Which == 0 ? d.ghpart : d.shpart;
   const int32_t *gofs = Which == 0 ? d.csc_off : d.csr_off;
-  const int32_t *q = d.q[p];
   int b, e, pre, total;
   scan_chunk(n, blockIdx.x, b, e);
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
@@ -617,8 +786,7 @@ __global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
     const int i = base + threadIdx.x;
     int deg = 0;
     if (i < e) {
-      const int slot = list[i];
-      const int v = all ? slot : q[slot];
+      const int v = slot_vertex(d, p, list[i]);
       deg = gofs[v + 1] - gofs[v];
     }
     int tot;
@@ -633,9 +801,9 @@ __global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
     off[n] = total;
     part[(total + n + MP_NV - 1) / MP_NV] = n;
     if (Which == 0)
-      c->ghub_items = total;
+      sc.ghub_items = total;
     else
-      c->shub_items = total;
+      sc.shub_items = total;
   }
 }
 
@@ -654,13 +822,10 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
   __shared__ int32_t s_slot[HUB_SEG_MAX];
   __shared__ SegItem<P> s_wtot[NT / WAVE];
 
-  const Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  const bool all = c->all_active[p] != 0;
-  const HubList<P> h = gather_hubs(d, c, all);
+  if (frontier_size(d) == 0) return;
+  const bool all = d.in_mode == IN_ALL;
+  const HubList<P> h = gather_hubs(d);
   if (h.n == 0) return;
-  const int32_t *q = d.q[p];
   const int N = h.items + h.n;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
@@ -674,7 +839,7 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
     for (int k = tid; k < nseg; k += NT) {
       const int s = s0 + k;
       const int slot = h.list[s];
-      const int v = all ? slot : q[slot];
+      const int v = all ? slot : slot_vertex(d, p, slot);
       s_vtx[k] = v;
       s_slot[k] = slot;
       s_base[k] = d.csc_off[v] - h.off[s];
@@ -760,10 +925,8 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
 // Folds the tile partials of hubs whose edges cross tiles.
 template <class P>
 __global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
-  const Ctl *c = d.ctl;
-  if (c->n_active[p] == 0) return;
-  const bool all = c->all_active[p] != 0;
-  const HubList<P> h = gather_hubs(d, c, all);
+  if (frontier_size(d) == 0) return;
+  const HubList<P> h = gather_hubs(d);
   for (int s = blockIdx.x * NT + threadIdx.x; s < h.n; s += gridDim.x * NT) {
     const int st = h.off[s] + s;
     const int en = h.off[s + 1] + s;
@@ -786,19 +949,51 @@ struct has_residual<P, std::void_t<decltype(P::residual(
     : std::true_type {};
 
 // Thread and wave tiers of the scatter for the slots whose apply returned
-// true (`ret`); hubs go to the scatter hub list.  Reached by every thread.
+// true (`ret`); hubs go to the scatter hub list.  With d.out_list the
+// destinations are claimed (claim_vertex) and the claimed ones appended to
+// the next list q[p^1] -- one atomicAdd per workgroup for the thread tier
+// and per wave for the wave tier; otherwise the activation byte is a plain
+// store (every writer stores the same value).  Reached by every thread.
 template <class P>
-__device__ inline int scatter_tiers(const Dev<P> &d, Ctl *c, bool ret,
-                                    int slot, int v, int32_t *s_buf,
-                                    int *s_cnt, int *s_base) {
+__device__ inline int scatter_tiers(const Dev<P> &d, int p, bool ret, int slot,
+                                    int v, int32_t *s_buf, int *s_cnt,
+                                    int *s_base, int *lds) {
+  uint8_t *fl = d.fl[p ^ 1];
+  int32_t *qn = d.q[p ^ 1];
+  int32_t *nn = &d.ctl->ring[(d.seq + 1) & 3].n_active;
   int o = 0, deg = 0;
   if (ret) {
     o = d.csr_off[v];
     deg = d.csr_off[v + 1] - o;
   }
-  if (ret && deg <= d.st1) {
+  const bool small = ret && deg <= d.st1;
+  if (d.out_list) {
+    int dst[T1_MAX];
+#pragma unroll
+    for (int k = 0; k < T1_MAX; ++k)
+      dst[k] = (small && k < deg) ? d.csr_dst[o + k] : -1;
+    int nc = 0;
+#pragma unroll
+    for (int k = 0; k < T1_MAX; ++k) {
+      if (dst[k] >= 0 && claim_vertex(fl, dst[k]))
+        ++nc;
+      else
+        dst[k] = -1;
+    }
+    int tot;
+    int pos = block_excl_scan(nc, tot, lds);
+    if (threadIdx.x == 0 && tot) *s_base = atomicAdd(nn, tot);
+    __syncthreads();
+    if (nc) {
+      pos += *s_base;
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        if (dst[k] >= 0) qn[pos++] = dst[k];
+    }
+    __syncthreads();
+  } else if (small) {
 #pragma unroll 4
-    for (int k = 0; k < deg; ++k) d.flags[d.csr_dst[o + k]] = 1;
+    for (int k = 0; k < deg; ++k) fl[d.csr_dst[o + k]] = 1;
   }
   const bool mid = ret && deg > d.st1 && deg <= d.st2;
   uint64_t m = __ballot(mid);
@@ -808,50 +1003,53 @@ __device__ inline int scatter_tiers(const Dev<P> &d, Ctl *c, bool ret,
     m &= m - 1;
     const int oj = __shfl(o, j, WAVE);
     const int dj = __shfl(deg, j, WAVE);
-    for (int k = lane; k < dj; k += WAVE) d.flags[d.csr_dst[oj + k]] = 1;
+    for (int base = 0; base < dj; base += WAVE) {
+      const int k = base + lane;
+      const int u = k < dj ? d.csr_dst[oj + k] : 0;
+      if (d.out_list) {
+        const bool got = k < dj && claim_vertex(fl, u);
+        const uint64_t gm = __ballot(got);
+        if (gm) {
+          int at = 0;
+          if (lane == 0) at = atomicAdd(nn, __popcll(gm));
+          at = __shfl(at, 0, WAVE);
+          if (got) qn[at + lanemask_lt_count(gm)] = u;
+        }
+      } else if (k < dj) {
+        fl[u] = 1;
+      }
+    }
   }
-  block_append(ret && deg > d.st2, slot, d.shub, &c->n_shub, s_buf, s_cnt,
-               s_base);
+  block_append(ret && deg > d.st2, slot, d.shub,
+               &d.ctl->ring[d.seq & 3].n_shub, s_buf, s_cnt, s_base);
   return ret ? deg : 0;
 }
 
 // Apply (kApply, gpugas_kernels.cuh:139-151).  Fuse = true also runs the
 // scatter tiers (the whole superstep is being executed, so scatterActivate
-// follows); otherwise the return flag is kept for k_scatter_vp.  Resets the
-// next parity's list size, which k_compact2 fills.
+// follows); otherwise the return flag is kept for k_scatter_vp.  Clears the
+// activation bytes of its own input (so they are zero when the superstep
+// after next writes them) and, for a bitmap input, counts its active slots
+// per workgroup for the next superstep's report.
 template <class P, bool HaveGather, bool Fuse>
 __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
   typedef typename P::VertexData VD;
   __shared__ int32_t s_buf[NT];
+  __shared__ int32_t s_ids[BM_WORDS * 4];
   __shared__ int s_cnt, s_base;
   __shared__ int lds[NT / WAVE];
   __shared__ double ldsd[NT / WAVE];
-  Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  // The next parity starts empty even when this superstep is: otherwise a
-  // superstep enqueued after the frontier emptied would re-run the list
-  // left in the other queue two supersteps ago.
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->n_active[p ^ 1] = 0;
-    c->all_active[p ^ 1] = 0;
-  }
-  if (A == 0) return;
-  if (threadIdx.x == 0) s_superstep = c->iteration;
+  superstep_prologue(d, p, lds);
+  if (threadIdx.x == 0) s_superstep = d.iter;
   __syncthreads();
-  const bool all = c->all_active[p] != 0;
-  const int32_t *q = d.q[p];
-  const int stride = gridDim.x * NT;
-  const int rounds = (A + stride - 1) / stride;
-  int scattered = 0;
+  int scattered = 0, active = 0;
   double res = 0.0;
-  for (int r = 0; r < rounds; ++r) {
-    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
-    const bool valid = i < A;
-    int v = 0;
+  for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) {
     bool ret = false;
     if (valid) {
-      v = all ? i : q[i];
+      ++active;
+      if (d.in_mode == IN_LIST) d.fl[p][v] = 0;
       GR g;
       if (HaveGather)
         g = d.gtmp[i];
@@ -865,15 +1063,17 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
         ret = P::apply(d.vdata + v, g);
       }
     }
-    if (Fuse) {
-      scattered += scatter_tiers(d, c, ret, i, v, s_buf, &s_cnt, &s_base);
-    } else if (valid) {
+    if (Fuse)
+      scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
+    else if (valid)
       d.rflag[i] = ret ? 1 : 0;
-    }
-  }
-  if (!HaveGather && threadIdx.x == 0) d.stat_blk[blockIdx.x] = 0;
+  });
+  active = block_sum(active, lds);
+  if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
+    d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
+  add_stat(d, 2, active);
   scattered = block_sum(scattered, lds);
-  if (threadIdx.x == 0) d.stat_blk[SCAN_BLOCKS + blockIdx.x] = scattered;
+  add_stat(d, 1, scattered);
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
     if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
@@ -884,25 +1084,17 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
 template <class P>
 __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
   __shared__ int32_t s_buf[NT];
+  __shared__ int32_t s_ids[BM_WORDS * 4];
   __shared__ int s_cnt, s_base;
   __shared__ int lds[NT / WAVE];
-  Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  const bool all = c->all_active[p] != 0;
-  const int32_t *q = d.q[p];
-  const int stride = gridDim.x * NT;
-  const int rounds = (A + stride - 1) / stride;
+  superstep_prologue(d, p, lds);
   int scattered = 0;
-  for (int r = 0; r < rounds; ++r) {
-    const int i = r * stride + blockIdx.x * NT + threadIdx.x;
-    const bool valid = i < A;
+  for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) {
     const bool ret = valid && d.rflag[i] != 0;
-    const int v = valid ? (all ? i : q[i]) : 0;
-    scattered += scatter_tiers(d, c, ret, i, v, s_buf, &s_cnt, &s_base);
-  }
+    scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
+  });
   scattered = block_sum(scattered, lds);
-  if (threadIdx.x == 0) d.stat_blk[SCAN_BLOCKS + blockIdx.x] = scattered;
+  add_stat(d, 1, scattered);
 }
 
 // Hub scatter: merge-path tiles over the scatter hubs' CSR out-edges.
@@ -910,16 +1102,16 @@ template <class P>
 __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
   __shared__ int32_t s_end[HUB_SEG_MAX];
   __shared__ int32_t s_base[HUB_SEG_MAX];
-  const Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  const int n = c->n_shub;
+  const StepCounts &sc = d.ctl->ring[d.seq & 3];
+  const int n = sc.n_shub;
   if (n == 0) return;
-  const bool all = c->all_active[p] != 0;
-  const int32_t *q = d.q[p];
-  const int N = c->shub_items + n;
+  uint8_t *fl = d.fl[p ^ 1];
+  int32_t *qn = d.q[p ^ 1];
+  int32_t *nn = &d.ctl->ring[(d.seq + 1) & 3].n_active;
+  const int N = sc.shub_items + n;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
+  const int lane = lane_id();
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
     const int d1 = min(d0 + MP_NV, N);
@@ -928,8 +1120,7 @@ __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
     const int nseg = min(s1, n - 1) - s0 + 1;
     for (int k = tid; k < nseg; k += NT) {
       const int s = s0 + k;
-      const int slot = d.shub[s];
-      const int v = all ? slot : q[slot];
+      const int v = slot_vertex(d, p, d.shub[s]);
       s_base[k] = d.csr_off[v] - d.shoff[s];
       s_end[k] = d.shoff[s + 1] + s;
     }
@@ -939,63 +1130,52 @@ __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
       const int pos = p0 + it;
+      int u = -1;
       if (pos < d1) {
         if (s_end[k] == pos)
           ++k;
         else
-          d.flags[d.csr_dst[s_base[k] + (pos - s0 - k)]] = 1;
+          u = d.csr_dst[s_base[k] + (pos - s0 - k)];
+      }
+      if (d.out_list) {
+        const bool got = u >= 0 && claim_vertex(fl, u);
+        const uint64_t gm = __ballot(got);
+        if (gm) {
+          int at = 0;
+          if (lane == 0) at = atomicAdd(nn, __popcll(gm));
+          at = __shfl(at, 0, WAVE);
+          if (got) qn[at + lanemask_lt_count(gm)] = u;
+        }
+      } else if (u >= 0) {
+        fl[u] = 1;
       }
     }
     __syncthreads();
   }
 }
 
-// Ordered compaction of the activation flags (replaces
-// scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157): pass 1 counts
-// set flags per chunk of 32-bit flag words and ends the superstep's
-// bookkeeping; pass 2 writes the ids of every chunk in ascending order at
-// its prefix and clears the words.
+// Ordered compaction of the activation bytes of the next parity into the
+// next list (replaces scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157),
+// used by the step-by-step API so its frontier is in ascending vertex order
+// as in refgas.h:201-210.  Pass 1 counts set bytes per chunk of 32-bit words;
+// pass 2 writes the ids of every chunk in ascending order at its prefix and
+// clears the words.
 template <class P>
 __global__ __launch_bounds__(NT) void k_compact1(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  Ctl *c = d.ctl;
-  const int A = c->n_active[p];
-  if (A == 0) return;
-  const uint32_t *fw = reinterpret_cast<const uint32_t *>(d.flags);
+  const uint32_t *fw = reinterpret_cast<const uint32_t *>(d.fl[p ^ 1]);
   int b, e;
   scan_chunk(d.nfwords, blockIdx.x, b, e);
   int cnt = 0;
   for (int i = b + threadIdx.x; i < e; i += NT) cnt += __popc(fw[i]);
   cnt = block_sum(cnt, lds);
   if (threadIdx.x == 0) d.bsum[blockIdx.x] = cnt;
-  if (blockIdx.x == 0) {
-    // superstep totals (all per-chunk counters were written by earlier
-    // kernels of this superstep)
-    int g = 0, s = 0;
-    for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) {
-      g += d.stat_blk[i];
-      s += d.stat_blk[SCAN_BLOCKS + i];
-    }
-    const int gi = block_sum(g, lds);
-    const int si = block_sum(s, lds);
-    if (threadIdx.x == 0) {
-      c->stat_gathered += gi;
-      c->stat_scattered += si;
-      c->stat_active += A;
-      c->stat_supersteps += 1;
-      c->iteration += 1;
-      c->n_ghub = 0;
-      c->n_shub = 0;
-    }
-  }
 }
 
 template <class P>
 __global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
-  Ctl *c = d.ctl;
-  if (c->n_active[p] == 0) return;
-  uint32_t *fw = reinterpret_cast<uint32_t *>(d.flags);
+  uint32_t *fw = reinterpret_cast<uint32_t *>(d.fl[p ^ 1]);
   int32_t *qn = d.q[p ^ 1];
   int b, e, pre, total;
   scan_chunk(d.nfwords, blockIdx.x, b, e);
@@ -1014,7 +1194,8 @@ __global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
     }
     carry += tot;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) c->n_active[p ^ 1] = total;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    d.ctl->ring[(d.seq + 1) & 3].n_active = total;
 }
 
 // Vertex-data initialisation on the device: `value` everywhere, `one` at
@@ -1026,12 +1207,6 @@ __global__ void k_fill_vdata(Dev<P> d, typename P::VertexData value,
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
     d.vdata[v] = v == at ? one : value;
-}
-
-// Superstep counter write (host-side setIteration).
-template <class P>
-__global__ void k_set_iteration(Dev<P> d, int it) {
-  if (threadIdx.x == 0) d.ctl->iteration = it;
 }
 
 }  // namespace gas

@@ -287,6 +287,11 @@ class Graph:
         _check(lib().gas_graph_get_tiers(self._h, C.c_int(program), _ptr(t)))
         return t.tolist()
 
+    def set_frontier_switch(self, program: int, divisor: int) -> None:
+        """List frontier while |A| * divisor < V, bitmap otherwise."""
+        _check(lib().gas_graph_set_frontier_switch(self._h, C.c_int(program),
+                                                   C.c_int32(divisor)))
+
     def autotune(self, program: int):
         """Times candidate degree tiers for `program` on this graph and keeps
         the fastest; returns [gather t1, gather t2, scatter t1, scatter t2]."""

@@ -210,6 +210,14 @@ int gas_graph_set_tiers(gas_graph_t g, int program, const int32_t *tiers) {
   });
 }
 
+int gas_graph_set_frontier_switch(gas_graph_t g, int program,
+                                  int32_t divisor) {
+  if (divisor < 0)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_frontier_switch: need >= 0");
+  return with_engine(g, program,
+                     [&](auto &e) { e.setSwitchDivisor(divisor); });
+}
+
 int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers) {
   if (!tiers) return set_error(GAS_ERR_ARG, "gas_graph_get_tiers: null");
   return with_engine(g, program, [&](auto &e) {

@@ -98,6 +98,23 @@ int gasrt_host_alloc(void **ptr, size_t bytes) {
                     "hipHostMalloc");
 }
 
+int gasrt_host_alloc_mapped(void **host_ptr, void **dev_ptr, size_t bytes) {
+  if (!host_ptr || !dev_ptr || bytes == 0)
+    return set_error(GAS_ERR_ARG, "gasrt_host_alloc_mapped: bad arguments");
+  *host_ptr = *dev_ptr = nullptr;
+  int rc = hip_status(
+      hipHostMalloc(host_ptr, bytes, hipHostMallocMapped | hipHostMallocCoherent),
+      "hipHostMalloc(mapped, coherent)");
+  if (rc != GAS_OK) return rc;
+  rc = hip_status(hipHostGetDevicePointer(dev_ptr, *host_ptr, 0),
+                  "hipHostGetDevicePointer");
+  if (rc != GAS_OK) {
+    hipHostFree(*host_ptr);
+    *host_ptr = nullptr;
+  }
+  return rc;
+}
+
 int gasrt_host_free(void *ptr) {
   if (!ptr) return GAS_OK;
   return hip_status(hipHostFree(ptr), "hipHostFree");
@@ -163,6 +180,17 @@ int gasrt_stream_destroy(gasrt_stream_t stream) {
 
 int gasrt_stream_synchronize(gasrt_stream_t stream) {
   return hip_status(hipStreamSynchronize(S(stream)), "hipStreamSynchronize");
+}
+
+int gasrt_stream_query(gasrt_stream_t stream, int *idle) {
+  if (!idle) return set_error(GAS_ERR_ARG, "gasrt_stream_query: null");
+  hipError_t e = hipStreamQuery(S(stream));
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return GAS_OK;
+  }
+  *idle = 1;
+  return hip_status(e, "hipStreamQuery");
 }
 
 int gasrt_event_create(gasrt_event_t *event) {
