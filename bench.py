@@ -77,6 +77,8 @@ def reached_edges(values, csr_off, unreached):
 #   k_hgather     per hub in-edge: 4 + edge value + source state; per hub:
 #                 8 + 4
 #   k_compact1    V (activation bytes); k_compact2: V + 4 per listed id
+#   k_fused       per active vertex: 8 (offsets) + state read + state write;
+#                 per in-edge as k_gather_vp; per scattered edge 5
 def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
     """{kernel: total algorithmic bytes over the run}.  hub_split =
     (hub_vertices, hub_edges) per sweep for all-active runs, splitting the
@@ -86,6 +88,11 @@ def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
     vd, ed = VD[app], ED.get(app, 0)
     A, Eg, Es = st.active_total, st.gathered_edges, st.scattered_edges
     out = {}
+    ks = st.kernels()
+    if have_gather and "k_fused" in ks and "k_gather_vp" not in ks:
+        # every superstep ran as one launch: gather + apply + scatter
+        out["k_fused"] = A * (8 + 2 * vd) + Eg * (4 + ed + vd) + Es * 5
+        return out
     if have_gather:
         if hub_split is not None:
             sweeps = st.supersteps

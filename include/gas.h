@@ -183,7 +183,8 @@ enum {
   GAS_K_HSCATTER = 7,  /* k_hscatter: merge-path activation over hub edges */
   GAS_K_COMPACT1 = 8,  /* k_compact1: frontier count per block             */
   GAS_K_COMPACT2 = 9,  /* k_compact2: ordered frontier write               */
-  GAS_K_SLOTS = 12     /* slots 10-11 unused                              */
+  GAS_K_FUSED = 10,    /* k_fused: gather + apply + scatter in one launch  */
+  GAS_K_SLOTS = 12     /* slot 11 unused                                  */
 };
 
 typedef struct {

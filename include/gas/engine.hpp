@@ -180,7 +180,7 @@ class GASEngineGPU {
   // gpugas.h:329-338
   void getResults() {
     if (m_vd_host)
-      check(gasrt_memcpy_d2h(m_vd_host, m_vdata.p,
+      check(gasrt_memcpy_d2h(m_vd_host, m_dev.vdata,
                              sizeof(VertexData) * (size_t)m_nV, m_stream),
             "getResults");
   }
@@ -192,9 +192,10 @@ class GASEngineGPU {
   // Uploads the caller's vertex data again (reuse of a bound graph).
   void resetVertexData(VertexData *vertexData) {
     m_vd_host = vertexData;
-    check(gasrt_memcpy_h2d(m_vdata.p, vertexData,
+    check(gasrt_memcpy_h2d(m_dev.vdata, vertexData,
                            sizeof(VertexData) * (size_t)m_nV, m_stream),
           "resetVertexData");
+    m_other_stale = true;
   }
 
   // Sets every vertex to `value` and vertex `at` to `one` on the device
@@ -206,6 +207,7 @@ class GASEngineGPU {
         2048, std::max<int64_t>(1, ((int64_t)m_nV + NT - 1) / NT));
     launch(k_fill_vdata<Program>, grid, m_stream, m_dev, value, (int32_t)at,
            one);
+    m_other_stale = true;
   }
 
   void setActive(Int vertexStart, Int vertexEnd) {
@@ -233,6 +235,7 @@ class GASEngineGPU {
 
   void gatherApply(bool haveGather = true) {
     m_out_list = 0;
+    m_other_stale = true;
     if (haveGather) {
       gatherPhase();
       klaunch(GAS_K_APPLY, 1, k_apply<Program, true, false>, SCAN_BLOCKS);
@@ -259,6 +262,19 @@ class GASEngineGPU {
   // otherwise as activation bytes only (bitmap).
   void superstep(bool haveGather, bool outList) {
     m_out_list = outList ? 1 : 0;
+    if (haveGather && !inHubs() && m_in != IN_LIST) {
+      // gather, apply and scatter in one launch, state double-buffered
+      if (m_other_stale)
+        check(gasrt_memcpy_d2d(m_dev.vnext, m_dev.vdata,
+                               sizeof(VertexData) * (size_t)m_nV, m_stream),
+              "state copy");
+      m_other_stale = false;
+      klaunch(GAS_K_FUSED, 0, k_fused<Program>, SCAN_BLOCKS);
+      std::swap(m_dev.vdata, m_dev.vnext);
+      endSuperstep(false);
+      return;
+    }
+    m_other_stale = true;
     if (haveGather) gatherPhase();
     if (haveGather)
       klaunch(GAS_K_APPLY, 1, k_apply<Program, true, true>, SCAN_BLOCKS);
@@ -465,7 +481,7 @@ class GASEngineGPU {
   Int numEdges() const { return m_nE; }
   int maxInDegree() const { return m_max_in; }
   int maxOutDegree() const { return m_max_out; }
-  VertexData *deviceVertexData() { return m_vdata.p; }
+  VertexData *deviceVertexData() { return m_dev.vdata; }
   const Dev<Program> &dev() const { return m_dev; }
 
  private:
@@ -549,6 +565,7 @@ class GASEngineGPU {
     m_nE = gv.n_edges;
     m_vd_host = vertexData;
     m_vdata.alloc((size_t)m_nV);
+    m_vdata2.alloc((size_t)m_nV);
     if (vertexData)
       check(gasrt_memcpy_h2d(m_vdata.p, vertexData,
                              sizeof(VertexData) * (size_t)m_nV, m_stream),
@@ -587,6 +604,8 @@ class GASEngineGPU {
     m_q1.alloc(V);
     m_fl0.alloc((size_t)nfwords * 4);
     m_fl1.alloc((size_t)nfwords * 4);
+    m_chg0.alloc((size_t)nfwords * 4);
+    m_chg1.alloc((size_t)nfwords * 4);
     m_rflag.alloc(V);
     m_gtmp.alloc(V);
     m_ghub.alloc(V);
@@ -624,6 +643,10 @@ class GASEngineGPU {
     d.csr_dst = gv.csr_dsts;
     d.edata = ed_dev;
     d.vdata = m_vdata.p;
+    d.vnext = m_vdata2.p;
+    d.chg[0] = m_chg0.p;
+    d.chg[1] = m_chg1.p;
+    m_other_stale = true;
     d.ctl = m_ctl.p;
     d.q[0] = m_q0.p;
     d.q[1] = m_q1.p;
@@ -713,6 +736,7 @@ class GASEngineGPU {
   int m_out_list = 0;               // scatter of this superstep builds a list
   bool m_first = true;              // next launch is the superstep's first
   int m_switch = 0;                 // density switch divisor (0: bitmap only)
+  bool m_other_stale = true;        // vnext does not mirror vdata
   gas_graph_t m_graph = nullptr;
   gas_graph_t m_own_graph = nullptr;
   Int m_nV = 0, m_nE = 0;
@@ -722,13 +746,13 @@ class GASEngineGPU {
   std::vector<int32_t> m_h_csc_off;
   int m_par = 0;  // parity of the next superstep
   VertexData *m_vd_host = nullptr;
-  DevBuf<VertexData> m_vdata;
+  DevBuf<VertexData> m_vdata, m_vdata2;  // state, double-buffered by k_fused
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;
   DevBuf<int32_t> m_q0, m_q1, m_ghub, m_ghoff, m_ghpart, m_shub, m_shoff,
       m_shpart, m_bsum, m_bcnt, m_ahub, m_ahoff, m_ahpart;
   DevBuf<int64_t> m_stat;
-  DevBuf<uint8_t> m_fl0, m_fl1, m_rflag;
+  DevBuf<uint8_t> m_fl0, m_fl1, m_chg0, m_chg1, m_rflag;
   DevBuf<double> m_rpart;
   DevBuf<GatherResult> m_gtmp;
   DevBuf<Acc<Program>> m_pfirst, m_plast;

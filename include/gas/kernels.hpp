@@ -309,7 +309,9 @@ struct Dev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst;
   const ED *edata;  // CSC order
-  VD *vdata;
+  VD *vdata;        // current vertex state (read by gathers)
+  VD *vnext;        // the other state buffer, written by k_fused's apply
+  uint8_t *chg[2];  // k_fused: apply returned true, by parity (V rounded to 4)
   Ctl *ctl;
   int32_t *q[2];     // active lists by parity
   uint8_t *fl[2];    // activation bytes by parity (V rounded up to 4)
@@ -559,7 +561,7 @@ __device__ inline void add_stat(const Dev<P> &d, int which, int64_t x) {
 // words read are zeroed (the last reader of this superstep's bitmap).
 constexpr int BM_WORDS = 2 * NT;
 template <class P, class Body>
-__device__ inline void for_each_active(const Dev<P> &d, int p, bool clear,
+__device__ __forceinline__ void for_each_active(const Dev<P> &d, int p, bool clear,
                                        int32_t *s_ids, int *lds, Body body) {
   if (d.in_mode != IN_BITMAP) {
     const int A = frontier_size(d);
@@ -621,9 +623,13 @@ __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
   const int stride = gridDim.x * NT;
   uint32_t *f0 = reinterpret_cast<uint32_t *>(d.fl[0]);
   uint32_t *f1 = reinterpret_cast<uint32_t *>(d.fl[1]);
+  uint32_t *c0 = reinterpret_cast<uint32_t *>(d.chg[0]);
+  uint32_t *c1 = reinterpret_cast<uint32_t *>(d.chg[1]);
   for (int i = blockIdx.x * NT + threadIdx.x; i < d.nfwords; i += stride) {
     f0[i] = 0u;
     f1[i] = 0u;
+    c0[i] = 0u;
+    c1[i] = 0u;
   }
   if (!all)
     for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += stride)
@@ -651,7 +657,7 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
   const bool all = d.in_mode == IN_ALL;
   const int lane = lane_id();
   int gathered = 0;
-  for_each_active(d, p, false, s_ids, lds, [&](bool valid, int i, int v) {
+  for_each_active(d, p, false, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     int o = 0, deg = 0;
     if (valid) {
       o = d.csc_off[v];
@@ -955,7 +961,7 @@ struct has_residual<P, std::void_t<decltype(P::residual(
 // and per wave for the wave tier; otherwise the activation byte is a plain
 // store (every writer stores the same value).  Reached by every thread.
 template <class P>
-__device__ inline int scatter_tiers(const Dev<P> &d, int p, bool ret, int slot,
+__device__ __forceinline__ int scatter_tiers(const Dev<P> &d, int p, bool ret, int slot,
                                     int v, int32_t *s_buf, int *s_cnt,
                                     int *s_base, int *lds) {
   uint8_t *fl = d.fl[p ^ 1];
@@ -1045,7 +1051,7 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   __syncthreads();
   int scattered = 0, active = 0;
   double res = 0.0;
-  for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) {
+  for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     bool ret = false;
     if (valid) {
       ++active;
@@ -1080,6 +1086,131 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   }
 }
 
+// A whole superstep in one launch for graphs without gather hubs, on an
+// all or bitmap frontier: gather (thread / wave tiers) from the current
+// state buffer, apply into the other buffer, scatter tiers.  Reading one
+// buffer and writing the other keeps the reference's bulk-synchronous
+// semantics (every gather of a superstep sees the state before any apply,
+// refgas.h:143-168) without a kernel boundary between gather and apply.
+// The other buffer is complete except for vertices changed in the previous
+// superstep that are not active now: the workgroup owning their activation
+// words copies them across (chg[p^1]) before processing its frontier.
+template <class P>
+__global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
+  typedef typename P::GatherResult GR;
+  typedef typename P::VertexData VD;
+  __shared__ int32_t s_buf[NT];
+  __shared__ int32_t s_ids[BM_WORDS * 4];
+  __shared__ int s_cnt, s_base;
+  __shared__ int lds[NT / WAVE];
+  __shared__ double ldsd[NT / WAVE];
+  superstep_prologue(d, p, lds);
+  if (threadIdx.x == 0) s_superstep = d.iter;
+  // state of vertices changed last superstep and inactive now
+  {
+    const uint32_t *aw = reinterpret_cast<const uint32_t *>(d.fl[p]);
+    uint32_t *cw = reinterpret_cast<uint32_t *>(d.chg[p ^ 1]);
+    int wb, we;
+    scan_chunk(d.nfwords, blockIdx.x, wb, we);
+    for (int wi = wb + threadIdx.x; wi < we; wi += NT) {
+      const uint32_t c = cw[wi];
+      if (!c) continue;
+      cw[wi] = 0u;
+      const uint32_t a = d.in_mode == IN_ALL ? 0xffffffffu : aw[wi];
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (((c >> (8 * b)) & 0xffu) && !((a >> (8 * b)) & 0xffu))
+          d.vnext[wi * 4 + b] = d.vdata[wi * 4 + b];
+    }
+  }
+  __syncthreads();
+  uint8_t *chg = d.chg[p];
+  const int lane = lane_id();
+  int gathered = 0, scattered = 0, active = 0;
+  double res = 0.0;
+  for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
+    int o = 0, deg = 0;
+    if (valid) {
+      o = d.csc_off[v];
+      deg = d.csc_off[v + 1] - o;
+      gathered += deg;
+      ++active;
+    }
+    Acc<P> acc = Acc<P>::empty();
+    if (valid && deg <= d.gt1) {
+      int src[T1_MAX];
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        src[k] = (k < deg) ? d.csc_src[o + k] : 0;
+      GR val[T1_MAX];
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        if (k < deg)
+          val[k] = P::gatherMap(d.vdata + v, d.vdata + src[k], d.edata + o + k);
+#pragma unroll
+      for (int k = 0; k < T1_MAX; ++k)
+        if (k < deg) acc.add(val[k]);
+    }
+    uint64_t m = __ballot(valid && deg > d.gt1);
+    while (m) {
+      const int j = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int vj = __shfl(v, j, WAVE);
+      const int oj = __shfl(o, j, WAVE);
+      const int dj = __shfl(deg, j, WAVE);
+      Acc<P> w = Acc<P>::empty();
+      for (int base = 0; base < dj; base += WAVE * MID_UNROLL) {
+        int src[MID_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u) {
+          const int k = base + u * WAVE + lane;
+          src[u] = (k < dj) ? d.csc_src[oj + k] : 0;
+        }
+        GR val[MID_UNROLL];
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u) {
+          const int k = base + u * WAVE + lane;
+          if (k < dj)
+            val[u] = P::gatherMap(d.vdata + vj, d.vdata + src[u],
+                                  d.edata + oj + k);
+        }
+#pragma unroll
+        for (int u = 0; u < MID_UNROLL; ++u)
+          if (base + u * WAVE + lane < dj) w.add(val[u]);
+      }
+#pragma unroll
+      for (int x = WAVE / 2; x > 0; x >>= 1) w.merge(w.xor_peer(x));
+      if (lane == j) acc = w;
+    }
+    bool ret = false;
+    if (valid) {
+      VD x = d.vdata[v];
+      if constexpr (has_residual<P>::value) {
+        const VD before = x;
+        ret = P::apply(&x, acc.finish());
+        res += P::residual(before, x);
+      } else {
+        ret = P::apply(&x, acc.finish());
+      }
+      d.vnext[v] = x;
+      if (ret) chg[v] = 1;
+    }
+    scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
+  });
+  active = block_sum(active, lds);
+  if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
+    d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
+  add_stat(d, 2, active);
+  gathered = block_sum(gathered, lds);
+  add_stat(d, 0, gathered);
+  scattered = block_sum(scattered, lds);
+  add_stat(d, 1, scattered);
+  if constexpr (has_residual<P>::value) {
+    res = block_sum(res, ldsd);
+    if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
+  }
+}
+
 // Scatter tiers after an unfused apply (scatterActivate, gpugas.h:669-751).
 template <class P>
 __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
@@ -1089,7 +1220,7 @@ __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
   __shared__ int lds[NT / WAVE];
   superstep_prologue(d, p, lds);
   int scattered = 0;
-  for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) {
+  for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     const bool ret = valid && d.rflag[i] != 0;
     scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
   });

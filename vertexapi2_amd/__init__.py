@@ -64,7 +64,8 @@ class RunStats(C.Structure):
 
 
 KERNEL_NAMES = ["k_gather_vp", "k_apply", "k_scatter_vp", "k_hscan1", "k_hscan2",
-                "k_hgather", "k_hfix", "k_hscatter", "k_compact1", "k_compact2"]
+                "k_hgather", "k_hfix", "k_hscatter", "k_compact1", "k_compact2",
+                "k_fused"]
 
 
 _lib = None
