@@ -56,6 +56,10 @@ HBM_PEAK_GBS = 8000.0
 VD = {"bfs": 4, "sssp": 4, "cc": 4, "pagerank": 8, "ppr": 12}
 ED = {"sssp": 4}
 GR = 4
+# Bytes a gather reads per in-edge for the source: the state, or for the
+# source-only programs (PageRank, PPR) the per-vertex contribution
+# rank / outdeg (4 bytes, refreshed by apply).
+SRC = {"bfs": 4, "sssp": 4, "cc": 4, "pagerank": 4, "ppr": 4}
 
 
 def log(msg):
@@ -70,7 +74,7 @@ def reached_edges(values, csr_off, unreached):
 # ---------------------------------------------------------------------------
 # Algorithmic bytes per kernel (each datum counted once, lower bounds).
 #   k_gather_vp   per gathered vertex: 8 (CSC offset pair) + 4 (result);
-#                 per in-edge: 4 (source id) + edge value + source state
+#                 per in-edge: 4 (source id) + edge value + source read (SRC)
 #   k_apply       per active vertex: 4 (slot id) + 4 (gather result, when
 #                 there is a gather) + state; per scattered out-edge: 4
 #                 (destination id) + 1 (activation byte)
@@ -85,23 +89,23 @@ def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
     gather between k_gather_vp and k_hgather; without it every gathered
     edge is booked to k_gather_vp (graphs whose in-degrees stay under the
     hub tier)."""
-    vd, ed = VD[app], ED.get(app, 0)
+    vd, ed, sv = VD[app], ED.get(app, 0), SRC[app]
     A, Eg, Es = st.active_total, st.gathered_edges, st.scattered_edges
     out = {}
     ks = st.kernels()
     if have_gather and "k_fused" in ks and "k_gather_vp" not in ks:
         # every superstep ran as one launch: gather + apply + scatter
-        out["k_fused"] = A * (8 + 2 * vd) + Eg * (4 + ed + vd) + Es * 5
+        out["k_fused"] = A * (8 + 2 * vd) + Eg * (4 + ed + sv) + Es * 5
         return out
     if have_gather:
         if hub_split is not None:
             sweeps = st.supersteps
             hv, he = hub_split
             out["k_gather_vp"] = ((A - sweeps * hv) * (8 + GR) +
-                                  (Eg - sweeps * he) * (4 + ed + vd))
-            out["k_hgather"] = sweeps * (hv * (8 + GR) + he * (4 + ed + vd))
+                                  (Eg - sweeps * he) * (4 + ed + sv))
+            out["k_hgather"] = sweeps * (hv * (8 + GR) + he * (4 + ed + sv))
         else:
-            out["k_gather_vp"] = A * (8 + GR) + Eg * (4 + ed + vd)
+            out["k_gather_vp"] = A * (8 + GR) + Eg * (4 + ed + sv)
     out["k_apply"] = A * (4 + (GR if have_gather else 0) + vd) + Es * 5
     launches = st.kernels().get("k_compact1", (0, 0))[1]
     out["k_compact1"] = launches * n_vertices
@@ -376,6 +380,9 @@ def main():
     ap.add_argument("--extra", dest="extra", action="store_true", default=True)
     ap.add_argument("--no-extra", dest="extra", action="store_false")
     ap.add_argument("--no-cpu", dest="cpu", action="store_false", default=True)
+    ap.add_argument("--lines", default="headline,apps,batched,served",
+                    help="comma-separated subset of the extra lines to run "
+                         "(the headline always runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device copy of 256 MiB, whose FETCH_SIZE / "
                          "WRITE_SIZE readings scale the counters of the engine's kernels")
@@ -414,14 +421,19 @@ def main():
         line["cpu_baseline"] = cpu_baseline(el)
         log("cpu baseline done")
     del el
+    want = set(args.lines.split(","))
     if args.extra:
-        line["apps"] = app_lines(min(args.steps, 5))
-        road = gas.Graph(workloads.road())
-        g22 = gas.Graph(workloads.rmat(22, symmetrize=True, weights=True))
-        log("rmat22 built")
-        line["batched"] = batched_lines(g22)
-        log("batched done")
-        line["served"] = served_lines([road, g22])
+        if "apps" in want:
+            line["apps"] = app_lines(min(args.steps, 5))
+        if want & {"batched", "served"}:
+            road = gas.Graph(workloads.road())
+            g22 = gas.Graph(workloads.rmat(22, symmetrize=True, weights=True))
+            log("rmat22 built")
+            if "batched" in want:
+                line["batched"] = batched_lines(g22)
+                log("batched done")
+            if "served" in want:
+                line["served"] = served_lines([road, g22])
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         # one device-to-device copy of 256 MiB (the runtime's copy kernel)

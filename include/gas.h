@@ -299,14 +299,21 @@ int gas_batch_used(gas_batch_t b, uint64_t *used);
 int gas_batch_release(gas_batch_t b, uint64_t lanes);
 /* Rounds until every lane in use is finished; stats may be NULL. */
 int gas_batch_run(gas_batch_t b, gas_run_stats *stats);
-/* Result of one lane: V int32 to host memory / to device memory. */
+/* Result of one lane: V int32 to host memory / to device memory (complete
+ * when the call returns). */
 int gas_batch_result(gas_batch_t b, int32_t lane, int32_t *host_out);
 int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out);
+/* Results of every lane set in `lanes` in one pass over the lane state:
+ * lane s into dev_outs[s] (device, n_vertices int32 each; entries of lanes
+ * not in the mask are ignored).  BFS / SSSP batches. */
+int gas_batch_results_device(gas_batch_t b, uint64_t lanes,
+                             int32_t *const *dev_outs);
 /* PPR lanes: V float to host memory / to device memory. */
 int gas_batch_result_f32(gas_batch_t b, int32_t lane, float *host_out);
 int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out);
 /* Push-versus-pull switch of batched BFS: pull when the frontier's
- * out-edges times `divisor` reach |E| (default 1; 0 = never pull). */
+ * out-edges times `divisor` reach |E| (default 0 = never pull: with 64
+ * lanes a pull rarely stops early, measured slower on every R-MAT round). */
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
 int gas_batch_free(gas_batch_t b);
 

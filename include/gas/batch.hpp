@@ -5,8 +5,9 @@
 //   BFS   per-vertex uint64 masks: visited, frontier (by parity), next;
 //         depth stored as one byte per (lane, vertex) (255 = unreached),
 //         levels up to 254.  A round is either a push from the frontier list
-//         (items of up to ITEM_EDGES out-edges, one wave per item, one lane
-//         per edge, 64-bit atomicOr of the new bits) or a pull over every
+//         (items of up to 16 out-edges carrying the frontier mask, 4 items
+//         per wave and 8 in flight, one lane per edge, 64-bit atomicOr of the
+//         new bits) or a pull over every
 //         vertex still missing lanes (in-edges with early exit once all
 //         missing lanes are found); the direction follows the frontier's
 //         out-edge count.
@@ -21,10 +22,13 @@
 //         is the unique shortest-path distance of each lane, unreached lanes
 //         keep 2147383647 (sssp.cu:32-33).
 //
-// Every round ends with an ordered compaction (two passes, no atomics) of
-// the vertices whose mask is nonzero, which also cuts their out-edges into
-// items.  Lanes are reset and seeded between rounds, so queries can join a
-// running batch (continuous batching) and leave when their lane empties.
+// Every round ends with a deterministic compaction (two passes, no atomics
+// except one per hub) of the vertices whose mask is nonzero, which also cuts
+// their out-edges into items; vertices are dealt to workgroups in 64-vertex
+// slabs so heavy id ranges spread out, and a hub's items are written by a
+// whole workgroup.  Lanes are reset and seeded between rounds, so queries
+// can join a running batch (continuous batching) and leave when their lane
+// empties.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,7 +41,11 @@
 namespace gas {
 
 constexpr int BW = 64;            // lanes per batch (= wave size)
-constexpr int ITEM_EDGES = 64;    // out-edges per push item
+constexpr int BFS_ITEM_EDGES = 16;   // BFS push: out-edges per item, 4 items per wave
+constexpr int SSSP_ITEM_EDGES = 64;  // SSSP push: out-edges per item (one wave)
+constexpr int BFS_ITEMS_PER_TRIP = 8;  // BFS push: items in flight per wave
+constexpr int HUB_ITEMS = 256;    // a vertex with more items is written by
+                                  // a whole workgroup of kb_hub_items
 constexpr int SSSP_UNROLL = 8;    // edges per wave trip in the SSSP push
 constexpr int32_t SSSP_INF = INT32_MAX - 100000;  // sssp.cu:32-33
 
@@ -45,7 +53,7 @@ struct BatchCtl {
   int32_t n_list;       // vertices in the current list
   int32_t n_items;      // push items over the list
   int32_t pull;         // BFS: next round pulls
-  int32_t pad;
+  int32_t n_hubs;       // list vertices whose items kb_hub_items writes
   int64_t list_edges;   // summed out-degree of the list
   uint64_t lanes_alive; // OR of the masks of the current list
   int64_t stat_edges;   // edges examined so far
@@ -56,14 +64,22 @@ struct BatchDev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst, *csr_w;
   BatchCtl *ctl;
-  uint64_t *visited, *front[2], *next;  // BFS
+  uint64_t *vn;                         // BFS [V][2]: visited, next bits
+                                        // (one 16-byte pair per vertex)
+  uint64_t *front[2];                   // BFS frontier masks by parity
   uint8_t *depth8;                      // BFS [BW][rowV]
   int32_t rowV;                         // depth row stride (V rounded to 4)
   int32_t *lane_start;                  // BFS [BW]: round the lane began
   int32_t *dist;                        // SSSP [V][BW]
   uint64_t *chg, *chg_next;             // SSSP
-  int32_t *list, *ioff, *iown;          // current list, item offsets/owners
-  int32_t *bsum;                        // [2][SCAN_BLOCKS] count | items
+  int32_t *list;                        // current list (slab order)
+  int4 *item;                           // push items: BFS {e0, e1, mask};
+                                        // SSSP {u, e0, e1, 0}
+  int32_t item_edges;                   // out-edges per item
+  int4 *hub;                            // vertices with > HUB_ITEMS items:
+                                        // {v, first item, e0, e1}
+  uint64_t *hub_mask;                   // their masks
+  int32_t *bsum;                        // [3][SCAN_BLOCKS] count | items | hubs
   int64_t *esum;                        // [SCAN_BLOCKS] out-edges
   uint64_t *lor;                        // [SCAN_BLOCKS] OR of masks
   int32_t pull_div;                     // pull when list_edges*div >= E
@@ -104,32 +120,52 @@ __device__ inline int64_t block_sum64(int64_t x, int64_t *lds) {
 }
 
 // ---- list construction -------------------------------------------------------
+// Vertices of a workgroup in the list passes: 64-vertex slabs dealt
+// round-robin to the SCAN_BLOCKS workgroups, NT / 64 slabs per round, so
+// runs of heavy vertices (R-MAT's low ids) spread over many workgroups.
+// Every pass that counts for kb_list walks the same assignment.
+constexpr int SLAB = 64;
+__device__ inline int slab_rounds(int nV) {
+  const int nslabs = (nV + SLAB - 1) / SLAB;
+  const int mine = (nslabs - (int)blockIdx.x + SCAN_BLOCKS - 1) / SCAN_BLOCKS;
+  return (mine * SLAB + NT - 1) / NT;
+}
+__device__ inline int slab_vertex(int r) {
+  const int idx = r * NT + (int)threadIdx.x;
+  return ((int)blockIdx.x + (idx / SLAB) * SCAN_BLOCKS) * SLAB + idx % SLAB;
+}
+
 // Pass 1 over contiguous chunks of vertices with mask array m: per chunk the
 // number of nonzero masks, their push items and out-edges, and the OR.
 static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
-  int b, e;
-  scan_chunk(d.nV, blockIdx.x, b, e);
-  int cnt = 0, items = 0;
+  int cnt = 0, items = 0, hubs = 0;
   int64_t edges = 0;
   uint64_t any = 0;
-  for (int v = b + threadIdx.x; v < e; v += NT) {
+  const int nr = slab_rounds(d.nV);
+  for (int r = 0; r < nr; ++r) {
+    const int v = slab_vertex(r);
+    if (v >= d.nV) continue;
     const uint64_t x = m[v];
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      items += nit;
+      hubs += nit > HUB_ITEMS;
       edges += deg;
       any |= x;
     }
   }
   cnt = block_sum(cnt, lds);
   items = block_sum(items, lds);
+  hubs = block_sum(hubs, lds);
   edges = block_sum64(edges, lds64);
   any = block_or(any, ldsu);
   if (threadIdx.x == 0) {
+    d.bsum[2 * SCAN_BLOCKS + blockIdx.x] = hubs;
     d.bsum[blockIdx.x] = cnt;
     d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
     d.esum[blockIdx.x] = edges;
@@ -143,9 +179,11 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
-  int b, e, pre, total, ipre, itotal;
-  scan_chunk(d.nV, blockIdx.x, b, e);
+  __shared__ int32_t s_ipos[NT], s_jpos[NT + 1], s_v[NT], s_o[NT], s_o1[NT];
+  __shared__ uint64_t s_m[NT];
+  int pre, total, ipre, itotal, hpre, htotal;
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
+  chunk_prefix(d.bsum + 2 * SCAN_BLOCKS, blockIdx.x, hpre, htotal, lds);
   chunk_prefix(d.bsum + SCAN_BLOCKS, blockIdx.x, ipre, itotal, lds);
   // every workgroup derives the direction from the edge totals; a round
   // that pulls needs no list
@@ -153,25 +191,62 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
   for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) ed += d.esum[i];
   ed = block_sum64(ed, lds64);
   const bool pull = d.pull_div > 0 && ed * (int64_t)d.pull_div >= (int64_t)d.nE;
-  int carry = pre, icarry = ipre;
-  for (int base = pull ? e : b; base < e; base += NT) {
-    const int v = base + threadIdx.x;
-    const bool on = v < e && m[v] != 0;
-    int nit = 0;
+  int carry = pre, icarry = ipre, hcarry = hpre;
+  const int ie = d.item_edges;
+  const int nr = pull ? 0 : slab_rounds(d.nV);
+  for (int r = 0; r < nr; ++r) {
+    const int v = slab_vertex(r);
+    const uint64_t mv = v < d.nV ? m[v] : 0ull;
+    const bool on = mv != 0;
+    int nit = 0, o = 0, o1 = 0;
     if (on) {
-      const int deg = d.csr_off[v + 1] - d.csr_off[v];
-      nit = (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      o = d.csr_off[v];
+      o1 = d.csr_off[v + 1];
+      nit = (o1 - o + ie - 1) / ie;
     }
     int tot, itot;
     const int pos = carry + block_excl_scan(on ? 1 : 0, tot, lds);
-    const int ipos = icarry + block_excl_scan(nit, itot, lds);
-    if (on) {
-      d.list[pos] = v;
-      d.ioff[pos] = ipos;
-      for (int k = 0; k < nit; ++k) d.iown[ipos + k] = pos;
+    const int iex = block_excl_scan(nit, itot, lds);
+    if (on) d.list[pos] = v;
+    // the workgroup writes the round's non-hub items together: the j-th of
+    // them belongs to the last thread whose exclusive count of non-hub
+    // items (s_jpos) is <= j; hubs' items are left to kb_hub_items
+    int jtot;
+    const int jex = block_excl_scan(nit > HUB_ITEMS ? 0 : nit, jtot, lds);
+    s_ipos[threadIdx.x] = iex;
+    s_jpos[threadIdx.x] = jex;
+    s_v[threadIdx.x] = v;
+    s_o[threadIdx.x] = o;
+    s_o1[threadIdx.x] = o1;
+    s_m[threadIdx.x] = mv;
+    if (threadIdx.x == 0) s_jpos[NT] = jtot;
+    int htot;
+    const int h = hcarry + block_excl_scan(nit > HUB_ITEMS ? 1 : 0, htot, lds);
+    if (nit > HUB_ITEMS) {
+      d.hub[h] = make_int4(v, icarry + iex, o, o1);
+      d.hub_mask[h] = mv;
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < jtot; j += NT) {
+      int lo = 0, hi = NT;  // largest t with s_jpos[t] <= j
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_jpos[mid] <= j) lo = mid; else hi = mid;
+      }
+      const int k = j - s_jpos[lo];
+      const int e0 = s_o[lo] + k * ie;
+      const int e1 = min(e0 + ie, s_o1[lo]);
+      const uint64_t mk = s_m[lo];
+      // BFS items carry the frontier mask, SSSP items the vertex
+      d.item[icarry + s_ipos[lo] + k] =
+          d.dist ? make_int4(s_v[lo], e0, e1, 0)
+                 : make_int4(e0, e1, (int)(uint32_t)mk,
+                             (int)(uint32_t)(mk >> 32));
+    }
+    __syncthreads();
     carry += tot;
     icarry += itot;
+    hcarry += htot;
   }
   if (blockIdx.x == 0) {
     uint64_t any = 0;
@@ -184,7 +259,25 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
       c->list_edges = ed;
       c->lanes_alive = any;
       c->pull = pull ? 1 : 0;
-      d.ioff[total] = itotal;
+      c->n_hubs = pull ? 0 : htotal;
+    }
+  }
+}
+
+// Items of the list's hubs: one workgroup per hub.
+static __global__ __launch_bounds__(NT) void kb_hub_items(BatchDev d) {
+  const int n = d.ctl->n_hubs;
+  const int ie = d.item_edges;
+  for (int h = blockIdx.x; h < n; h += gridDim.x) {
+    const int4 hb = d.hub[h];
+    const uint64_t mk = d.hub_mask[h];
+    const int nit = (hb.w - hb.z + ie - 1) / ie;
+    for (int k = threadIdx.x; k < nit; k += NT) {
+      const int e0 = hb.z + k * ie;
+      const int e1 = min(e0 + ie, hb.w);
+      d.item[hb.y + k] = d.dist ? make_int4(hb.x, e0, e1, 0)
+                                : make_int4(e0, e1, (int)(uint32_t)mk,
+                                            (int)(uint32_t)(mk >> 32));
     }
   }
 }
@@ -195,7 +288,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
                                                    uint64_t reset) {
   const int stride = gridDim.x * NT;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += stride) {
-    d.visited[v] &= ~reset;
+    d.vn[2 * (size_t)v] &= ~reset;
     d.front[p][v] &= ~reset;
   }
   // depth rows of the reset lanes, 4 bytes per thread
@@ -215,40 +308,57 @@ static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
   if (i >= n) return;
   const int s = lanes[i], v = sources[i];
   const uint64_t bit = 1ull << s;
-  atomicOr((unsigned long long *)&d.visited[v], (unsigned long long)bit);
+  atomicOr((unsigned long long *)&d.vn[2 * (size_t)v], (unsigned long long)bit);
   atomicOr((unsigned long long *)&d.front[p][v], (unsigned long long)bit);
   d.depth8[(size_t)s * d.rowV + v] = 0;
 }
 
-// Push: one wave per item, one lane per out-edge.
+// Push: a wave takes BFS_ITEMS_PER_TRIP items at a time, 16 lanes per item
+// and one lane per out-edge; all their loads are issued before any use.
+// New bits (frontier mask minus the destination's visited bits) are OR-ed
+// into the destination's next mask.
 static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
                                                   uint64_t used) {
   const BatchCtl *c = d.ctl;
   if (c->pull) return;
   const int n_items = c->n_items;
   const int lane = lane_id();
+  const int sub = lane / BFS_ITEM_EDGES, k = lane % BFS_ITEM_EDGES;
+  constexpr int G = BFS_ITEMS_PER_TRIP / (WAVE / BFS_ITEM_EDGES);
   const int waves = gridDim.x * (NT / WAVE);
-  for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < n_items;
-       it += waves) {
-    const int li = d.iown[it];
-    const int u = d.list[li];
-    const uint64_t f = d.front[p][u];
-    const int k = it - d.ioff[li];
-    const int e0 = d.csr_off[u] + k * ITEM_EDGES;
-    const int e1 = min(e0 + ITEM_EDGES, d.csr_off[u + 1]);
-    const int ei = e0 + lane;
-    if (ei < e1) {
-      const int w = d.csr_dst[ei];
-      const uint64_t nb = f & used & ~d.visited[w];
+  const int wid = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE;
+  for (int base = wid * BFS_ITEMS_PER_TRIP; base < n_items;
+       base += waves * BFS_ITEMS_PER_TRIP) {
+    int4 it[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int i = base + g * (WAVE / BFS_ITEM_EDGES) + sub;
+      it[g] = i < n_items ? d.item[i] : make_int4(0, 0, 0, 0);
+    }
+    int w[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int ei = it[g].x + k;
+      w[g] = ei < it[g].y ? d.csr_dst[ei] : -1;
+    }
+    ulonglong2 vn[G];  // visited, next of the destination: one 16-byte load
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      vn[g] = w[g] >= 0 ? reinterpret_cast<const ulonglong2 *>(d.vn)[w[g]]
+                        : make_ulonglong2(~0ull, ~0ull);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const uint64_t f =
+          ((uint64_t)(uint32_t)it[g].w << 32) | (uint32_t)it[g].z;
+      // bits already in next (possibly a stale read: then the atomic is
+      // merely redundant) need no atomic
+      const uint64_t nb = f & used & ~vn[g].x & ~vn[g].y;
       if (nb)
-        atomicOr((unsigned long long *)&d.next[w], (unsigned long long)nb);
+        atomicOr((unsigned long long *)&d.vn[2 * (size_t)w[g] + 1],
+                 (unsigned long long)nb);
     }
   }
 }
-
-// Pull: every vertex missing some used lane ORs its in-neighbours'
-// frontier masks until all missing lanes are found (thread per vertex up to
-// 32 in-edges, wave per vertex above).
 static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
                                                   uint64_t used) {
   const BatchCtl *c = d.ctl;
@@ -263,7 +373,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
     uint64_t need = 0;
     int o = 0, deg = 0;
     if (v < d.nV) {
-      need = used & ~d.visited[v];
+      need = used & ~d.vn[2 * (size_t)v];
       if (need) {
         o = d.csc_off[v];
         deg = d.csc_off[v + 1] - o;
@@ -271,12 +381,21 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
     }
     const bool wide = need && deg > 32;
     if (need && !wide) {
+      // in-edges in chunks of 8: all loads of a chunk issued before use
       uint64_t acc = 0;
-      for (int k = 0; k < deg; ++k) {
-        acc |= fr[d.csc_src[o + k]];
+      for (int k0 = 0; k0 < deg; k0 += 8) {
+        int src[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x)
+          src[x] = k0 + x < deg ? d.csc_src[o + k0 + x] : -1;
+        uint64_t f[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) f[x] = src[x] >= 0 ? fr[src[x]] : 0ull;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) acc |= f[x];
         if ((acc & need) == need) break;
       }
-      if (acc & need) d.next[v] = acc & need;
+      if (acc & need) d.vn[2 * (size_t)v + 1] = acc & need;
     }
     uint64_t m = __ballot(wide);
     while (m) {
@@ -294,7 +413,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
         acc |= wave_or(x);
         if ((acc & nj) == nj) break;
       }
-      if (lane == j && (acc & nj)) d.next[v] = acc & nj;
+      if (lane == j && (acc & nj)) d.vn[2 * (size_t)v + 1] = acc & nj;
     }
   }
 }
@@ -312,21 +431,22 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
   if (threadIdx.x < BW) s_start[threadIdx.x] = d.lane_start[threadIdx.x];
   __syncthreads();
   uint64_t *nf = d.front[p ^ 1];
-  int b, e;
-  scan_chunk(d.nV, blockIdx.x, b, e);
-  int cnt = 0, items = 0;
+  int cnt = 0, items = 0, hubs = 0;
   int64_t edges = 0;
   uint64_t any = 0;
-  for (int v = b + threadIdx.x; v < e; v += NT) {
-    const uint64_t n = d.next[v];
+  const int nr = slab_rounds(d.nV);
+  for (int r = 0; r < nr; ++r) {
+    const int v = slab_vertex(r);
+    if (v >= d.nV) continue;
+    const ulonglong2 vnv = reinterpret_cast<const ulonglong2 *>(d.vn)[v];
+    const uint64_t n = vnv.y;
     uint64_t nv = 0;
     if (n) {
-      d.next[v] = 0;
-      nv = n & used & ~d.visited[v];
+      nv = n & used & ~vnv.x;
+      reinterpret_cast<ulonglong2 *>(d.vn)[v] = make_ulonglong2(vnv.x | nv, 0ull);
     }
     nf[v] = nv;
     if (nv) {
-      d.visited[v] |= nv;
       uint64_t x = nv;
       while (x) {
         const int s = __ffsll((unsigned long long)x) - 1;
@@ -335,16 +455,20 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
       }
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      items += nit;
+      hubs += nit > HUB_ITEMS;
       edges += deg;
       any |= nv;
     }
   }
   cnt = block_sum(cnt, lds);
   items = block_sum(items, lds);
+  hubs = block_sum(hubs, lds);
   edges = block_sum64(edges, lds64);
   any = block_or(any, ldsu);
   if (threadIdx.x == 0) {
+    d.bsum[2 * SCAN_BLOCKS + blockIdx.x] = hubs;
     d.bsum[blockIdx.x] = cnt;
     d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
     d.esum[blockIdx.x] = edges;
@@ -397,14 +521,11 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   const int waves = gridDim.x * (NT / WAVE);
   for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < n_items;
        it += waves) {
-    const int li = d.iown[it];
-    const int u = d.list[li];
+    const int4 item = d.item[it];
+    const int u = item.x, e0 = item.y, ne = item.z - item.y;
     const uint64_t m = d.chg[u];
     const bool act = (m >> lane) & 1ull;
     const int du = d.dist[(size_t)u * BW + lane];
-    const int k = it - d.ioff[li];
-    const int e0 = d.csr_off[u] + k * ITEM_EDGES;
-    const int ne = min(ITEM_EDGES, d.csr_off[u + 1] - e0);
     int my_w = 0, my_wt = 0;
     if (lane < ne) {
       my_w = d.csr_dst[e0 + lane];
@@ -439,28 +560,33 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
-  int b, e;
-  scan_chunk(d.nV, blockIdx.x, b, e);
-  int cnt = 0, items = 0;
+  int cnt = 0, items = 0, hubs = 0;
   int64_t edges = 0;
   uint64_t any = 0;
-  for (int v = b + threadIdx.x; v < e; v += NT) {
+  const int nr = slab_rounds(d.nV);
+  for (int r = 0; r < nr; ++r) {
+    const int v = slab_vertex(r);
+    if (v >= d.nV) continue;
     const uint64_t x = d.chg_next[v];
     if (x) d.chg_next[v] = 0;
     d.chg[v] = x;
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      items += (deg + ITEM_EDGES - 1) / ITEM_EDGES;
+      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      items += nit;
+      hubs += nit > HUB_ITEMS;
       edges += deg;
       any |= x;
     }
   }
   cnt = block_sum(cnt, lds);
   items = block_sum(items, lds);
+  hubs = block_sum(hubs, lds);
   edges = block_sum64(edges, lds64);
   any = block_or(any, ldsu);
   if (threadIdx.x == 0) {
+    d.bsum[2 * SCAN_BLOCKS + blockIdx.x] = hubs;
     d.bsum[blockIdx.x] = cnt;
     d.bsum[SCAN_BLOCKS + blockIdx.x] = items;
     d.esum[blockIdx.x] = edges;
@@ -472,6 +598,48 @@ static __global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
                                                       int32_t *out) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
     out[v] = d.dist[(size_t)v * BW + s];
+}
+
+// Results of every lane in `lanes` at once, lane s into outs.p[s] (V int32).
+struct LaneOutputs {
+  int32_t *p[BW];
+};
+
+// BFS: depth rows are lane-major, each output is a row conversion.
+static __global__ __launch_bounds__(NT) void kb_bfs_extract_many(
+    BatchDev d, uint64_t lanes, LaneOutputs outs) {
+  for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int s = __ffsll((unsigned long long)m) - 1;
+      const uint8_t x = d.depth8[(size_t)s * d.rowV + v];
+      outs.p[s][v] = x == 255 ? -1 : (int32_t)x;
+    }
+  }
+}
+
+// SSSP: distances are vertex-major (one 256-byte row of 64 lanes per
+// vertex); a workgroup reads 64 rows coalesced into LDS and writes each
+// requested lane's 64 consecutive vertices coalesced, so the array is read
+// once for any number of lanes.
+static __global__ __launch_bounds__(NT) void kb_sssp_extract_many(
+    BatchDev d, uint64_t lanes, LaneOutputs outs) {
+  __shared__ int32_t tile[64][BW + 1];
+  const int w = threadIdx.x / WAVE, lane = lane_id();
+  for (int v0 = blockIdx.x * 64; v0 < d.nV; v0 += gridDim.x * 64) {
+    const int nv = min(64, d.nV - v0);
+    for (int i = threadIdx.x; i < 64 * BW; i += NT) {
+      const int j = i / BW, s = i % BW;
+      tile[j][s] = j < nv ? d.dist[(size_t)(v0 + j) * BW + s] : 0;
+    }
+    __syncthreads();
+    int r = 0;
+    for (uint64_t m = lanes; m; m &= m - 1, ++r) {
+      if (r % (NT / WAVE) != w) continue;
+      const int s = __ffsll((unsigned long long)m) - 1;
+      if (lane < nv) outs.p[s][v0 + lane] = tile[lane][s];
+    }
+    __syncthreads();
+  }
 }
 
 // ---- host side -----------------------------------------------------------------
@@ -500,35 +668,32 @@ class BatchEngine {
     d.csr_off = gv.csr_offsets;
     d.csr_dst = gv.csr_dsts;
     d.csr_w = gv.csr_weights;
-    d.pull_div = kind == BFS_KIND ? 1 : 0;
+    d.pull_div = 0;  // push only: see setPullDivisor
+    d.item_edges = kind == BFS_KIND ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
-    m_ioff.alloc(V + 1);
-    m_iown.alloc((size_t)gv.n_edges / ITEM_EDGES + V + 1);
-    m_bsum.alloc(2 * SCAN_BLOCKS);
+    m_item.alloc((size_t)gv.n_edges / d.item_edges + V + 1);
+    m_bsum.alloc(3 * SCAN_BLOCKS);
     m_esum.alloc(SCAN_BLOCKS);
     m_lor.alloc(SCAN_BLOCKS);
     m_lanes.alloc(BW);
     m_sources.alloc(BW);
     if (kind == BFS_KIND) {
       d.rowV = (int32_t)((V + 3) / 4 * 4);
-      m_visited.alloc(V);
+      m_vn.alloc(2 * V);
       m_front0.alloc(V);
       m_front1.alloc(V);
-      m_next.alloc(V);
       m_depth.alloc((size_t)d.rowV * BW);
       m_lane_start.alloc(BW);
-      check(gasrt_memset(m_visited.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_vn.p, 0, 2 * V * 8, m_stream), "memset");
       check(gasrt_memset(m_front0.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_front1.p, 0, V * 8, m_stream), "memset");
-      check(gasrt_memset(m_next.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW, m_stream),
             "memset");
       check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
-      d.visited = m_visited.p;
+      d.vn = m_vn.p;
       d.front[0] = m_front0.p;
       d.front[1] = m_front1.p;
-      d.next = m_next.p;
       d.depth8 = m_depth.p;
       d.lane_start = m_lane_start.p;
     } else {
@@ -543,8 +708,12 @@ class BatchEngine {
     }
     d.ctl = m_ctl.p;
     d.list = m_list.p;
-    d.ioff = m_ioff.p;
-    d.iown = m_iown.p;
+    d.item = m_item.p;
+    // a hub has > HUB_ITEMS items, so at most E / (HUB_ITEMS * item_edges)
+    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS * d.item_edges) + 1);
+    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS * d.item_edges) + 1);
+    d.hub = m_hub.p;
+    d.hub_mask = m_hub_mask.p;
     d.bsum = m_bsum.p;
     d.esum = m_esum.p;
     d.lor = m_lor.p;
@@ -564,6 +733,11 @@ class BatchEngine {
   gasrt_stream_t stream() const { return m_stream; }
   uint64_t lanesUsed() const { return m_used; }
   int round() const { return m_round; }
+  // BFS rounds pull instead of push when the frontier's out-edges reach
+  // E / div (0: never, the default).  With 64 lanes a pull cannot stop
+  // early while any lane's frontier is still far from the vertex, so on
+  // the R-MAT batches measured here every pull round cost more than the
+  // push it replaced.
   void setPullDivisor(int div) {
     if (m_kind == BFS_KIND) m_dev.pull_div = div;
   }
@@ -597,6 +771,7 @@ class BatchEngine {
              (const uint64_t *)m_dev.front[m_par]);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.front[m_par]);
+      launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       launch(kb_sssp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, reset);
       launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n);
@@ -605,6 +780,7 @@ class BatchEngine {
              (const uint64_t *)m_dev.chg);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg);
+      launch(kb_hub_items, 256, m_stream, m_dev);
     }
   }
 
@@ -624,11 +800,13 @@ class BatchEngine {
       m_par ^= 1;
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.front[m_par]);
+      launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       launch(kb_sssp_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg);
+      launch(kb_hub_items, 256, m_stream, m_dev);
     }
     ++m_round;
   }
@@ -687,6 +865,24 @@ class BatchEngine {
       launch(kb_sssp_extract, gridFor(m_nV), m_stream, m_dev, lane, dev_out);
   }
 
+  // Results of every lane in `lanes` into outs[lane] (device, V int32
+  // each), in one pass over the lane state.
+  void laneResultsDevice(uint64_t lanes, int32_t *const *outs) {
+    if (!lanes) return;
+    LaneOutputs o{};
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int s = __builtin_ctzll(m);
+      if (!outs[s]) throw Error(GAS_ERR_ARG, "null lane output");
+      o.p[s] = outs[s];
+    }
+    if (m_kind == BFS_KIND)
+      launch(kb_bfs_extract_many, gridFor(m_nV), m_stream, m_dev, lanes, o);
+    else
+      launch(kb_sssp_extract_many,
+             (unsigned)std::min<int64_t>(2048, (m_nV + 63) / 64), m_stream,
+             m_dev, lanes, o);
+  }
+
   void laneResult(int lane, int32_t *host_out) {
     if (m_tmp.n < (size_t)m_nV) m_tmp.alloc((size_t)m_nV);
     laneResultDevice(lane, m_tmp.p);
@@ -712,11 +908,13 @@ class BatchEngine {
   int32_t m_hstart[BW] = {};
   BatchDev m_dev{};
   DevBuf<BatchCtl> m_ctl;
-  DevBuf<int32_t> m_list, m_ioff, m_iown, m_bsum, m_lanes, m_sources,
+  DevBuf<int4> m_item;
+  DevBuf<int32_t> m_list, m_bsum, m_lanes, m_sources,
       m_lane_start, m_dist, m_tmp;
   DevBuf<int64_t> m_esum;
-  DevBuf<uint64_t> m_lor, m_visited, m_front0, m_front1, m_next, m_chg,
-      m_chg_next;
+  DevBuf<uint64_t> m_lor, m_vn, m_front0, m_front1, m_chg, m_chg_next,
+      m_hub_mask;
+  DevBuf<int4> m_hub;
   DevBuf<uint8_t> m_depth;
 };
 

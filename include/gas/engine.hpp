@@ -196,6 +196,7 @@ class GASEngineGPU {
                            sizeof(VertexData) * (size_t)m_nV, m_stream),
           "resetVertexData");
     m_other_stale = true;
+    m_srcval_stale = true;
   }
 
   // Sets every vertex to `value` and vertex `at` to `one` on the device
@@ -208,18 +209,26 @@ class GASEngineGPU {
     launch(k_fill_vdata<Program>, grid, m_stream, m_dev, value, (int32_t)at,
            one);
     m_other_stale = true;
+    m_srcval_stale = true;
+  }
+
+  // The caller wrote vertex state through deviceVertexData().
+  void stateChanged() {
+    m_other_stale = true;
+    m_srcval_stale = true;
   }
 
   void setActive(Int vertexStart, Int vertexEnd) {
     if (vertexStart < 0 || vertexEnd > m_nV || vertexEnd < vertexStart)
       throw Error(GAS_ERR_ARG, "setActive: range outside [0, nVertices)");
-    const int64_t work =
-        std::max<int64_t>(m_dev.nfwords, (int64_t)vertexEnd - vertexStart);
+    const int64_t work = std::max<int64_t>(m_flags_dirty ? m_dev.nfwords : 0,
+                                           (int64_t)vertexEnd - vertexStart);
     const unsigned grid =
         (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (work + NT - 1) / NT));
     m_dev.seq = (int32_t)m_seq;
     launch(k_set_active<Program>, grid, m_stream, m_dev, m_par, vertexStart,
-           vertexEnd);
+           vertexEnd, m_flags_dirty ? 1 : 0);
+    m_flags_dirty = false;
     m_in = (vertexStart == 0 && vertexEnd == m_nV) ? IN_ALL : IN_LIST;
     m_prev_in = IN_LIST;  // nothing of an earlier superstep left to report
     m_first = true;
@@ -262,7 +271,8 @@ class GASEngineGPU {
   // otherwise as activation bytes only (bitmap).
   void superstep(bool haveGather, bool outList) {
     m_out_list = outList ? 1 : 0;
-    if (haveGather && !inHubs() && m_in != IN_LIST) {
+    if (haveGather && !inHubs() && m_in != IN_LIST &&
+        !is_source_only<Program>::value) {
       // gather, apply and scatter in one launch, state double-buffered
       if (m_other_stale)
         check(gasrt_memcpy_d2d(m_dev.vnext, m_dev.vdata,
@@ -385,6 +395,7 @@ class GASEngineGPU {
           gatherPhase();
         } else {
           m_out_list = 0;
+          m_flags_dirty = true;
           klaunch(GAS_K_SCATTER, 1, k_scatter_vp<Program>, SCAN_BLOCKS);
           if (outHubs()) {
             klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
@@ -491,13 +502,26 @@ class GASEngineGPU {
   bool outHubs() const { return m_max_out > m_st2; }
 
   void gatherPhase() {
+    refreshSourceValues();
     klaunch(GAS_K_GATHER, 0, k_gather_vp<Program>, SCAN_BLOCKS);
     if (inHubs()) {
-      klaunch(GAS_K_HSCAN1, 0, k_hscan1<Program, 0>, SCAN_BLOCKS);
-      klaunch(GAS_K_HSCAN2, 0, k_hscan2<Program, 0>, SCAN_BLOCKS);
+      if (m_in != IN_ALL) {  // all active: the hub list is fixed per graph
+        klaunch(GAS_K_HSCAN1, 0, k_hscan1<Program, 0>, SCAN_BLOCKS);
+        klaunch(GAS_K_HSCAN2, 0, k_hscan2<Program, 0>, SCAN_BLOCKS);
+      }
       klaunch(GAS_K_HGATHER, 0, k_hgather<Program>, MP_BLOCKS);
       klaunch(GAS_K_HFIX, 0, k_hfix<Program>, SCAN_BLOCKS);
     }
+  }
+
+  // gatherMap(v) of every vertex for source-only programs, after the state
+  // changed outside apply (upload, device fill, setGraph).
+  void refreshSourceValues() {
+    if (!is_source_only<Program>::value || !m_srcval_stale) return;
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)m_nV + NT - 1) / NT));
+    launch(k_srcval_all<Program>, grid, m_stream, m_dev);
+    m_srcval_stale = false;
   }
 
   // Hub scatter, optional ordered compaction, and the move to the next
@@ -512,6 +536,7 @@ class GASEngineGPU {
       klaunch(GAS_K_COMPACT1, 2, k_compact1<Program>, SCAN_BLOCKS);
       klaunch(GAS_K_COMPACT2, 2, k_compact2<Program>, SCAN_BLOCKS);
     }
+    m_flags_dirty = true;
     m_prev_in = m_in;
     m_in = (compact || m_out_list) ? IN_LIST : IN_BITMAP;
     m_par ^= 1;
@@ -644,6 +669,10 @@ class GASEngineGPU {
     d.edata = ed_dev;
     d.vdata = m_vdata.p;
     d.vnext = m_vdata2.p;
+    m_srcval.alloc(is_source_only<Program>::value ? (size_t)m_nV : 1);
+    d.srcval = m_srcval.p;
+    m_srcval_stale = true;
+    m_flags_dirty = true;
     d.chg[0] = m_chg0.p;
     d.chg[1] = m_chg1.p;
     m_other_stale = true;
@@ -737,6 +766,8 @@ class GASEngineGPU {
   bool m_first = true;              // next launch is the superstep's first
   int m_switch = 0;                 // density switch divisor (0: bitmap only)
   bool m_other_stale = true;        // vnext does not mirror vdata
+  bool m_srcval_stale = true;       // srcval does not match the state
+  bool m_flags_dirty = true;        // a scatter ran since the last clear
   gas_graph_t m_graph = nullptr;
   gas_graph_t m_own_graph = nullptr;
   Int m_nV = 0, m_nE = 0;
@@ -754,7 +785,7 @@ class GASEngineGPU {
   DevBuf<int64_t> m_stat;
   DevBuf<uint8_t> m_fl0, m_fl1, m_chg0, m_chg1, m_rflag;
   DevBuf<double> m_rpart;
-  DevBuf<GatherResult> m_gtmp;
+  DevBuf<GatherResult> m_gtmp, m_srcval;
   DevBuf<Acc<Program>> m_pfirst, m_plast;
   Dev<Program> m_dev{};
   std::vector<std::pair<gasrt_event_t, gasrt_event_t>> m_timing;

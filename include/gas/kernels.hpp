@@ -132,6 +132,17 @@ template <class P>
 struct is_compensated<P, std::void_t<decltype(P::compensatedSum)>>
     : std::integral_constant<bool, P::compensatedSum> {};
 
+// Optional program trait: `static constexpr bool gatherSourceOnly = true`
+// declares that gatherMap reads only its source vertex (not the
+// destination, not the edge).  The engine then keeps gatherMap(src) per
+// vertex in an array refreshed by every apply, and gathers read that array
+// (sizeof(GatherResult) per edge) instead of the source's whole state.
+template <class P, class = void>
+struct is_source_only : std::false_type {};
+template <class P>
+struct is_source_only<P, std::void_t<decltype(P::gatherSourceOnly)>>
+    : std::integral_constant<bool, P::gatherSourceOnly> {};
+
 template <class T>
 __device__ inline T shfl_xor_4b(T x, int m) {
   static_assert(sizeof(T) == 4, "cross-lane reduce needs a 4-byte result");
@@ -311,6 +322,7 @@ struct Dev {
   const ED *edata;  // CSC order
   VD *vdata;        // current vertex state (read by gathers)
   VD *vnext;        // the other state buffer, written by k_fused's apply
+  GR *srcval;       // gatherMap(v) per vertex (source-only programs)
   uint8_t *chg[2];  // k_fused: apply returned true, by parity (V rounded to 4)
   Ctl *ctl;
   int32_t *q[2];     // active lists by parity
@@ -341,6 +353,17 @@ struct Dev {
   int32_t pub;       // PUB_* bits
   Acc<P> *part_first, *part_last;  // per-tile partials of crossing segments
 };
+
+// gatherMap of in-edge e = (src -> dst): from the per-vertex array for
+// source-only programs, from the states otherwise.
+template <class P>
+__device__ __forceinline__ typename P::GatherResult gather_value(
+    const Dev<P> &d, int dst, int src, int e) {
+  if constexpr (is_source_only<P>::value)
+    return d.srcval[src];
+  else
+    return P::gatherMap(d.vdata + dst, d.vdata + src, d.edata + e);
+}
 
 // ---------------------------------------------------------------------------
 // Wave / block helpers.
@@ -545,10 +568,14 @@ __device__ inline bool claim_vertex(uint8_t *fl, int u) {
 }
 
 // Per-workgroup run totals (0 gathered, 1 scattered, 2 active): each
-// workgroup adds into its own slot, so no atomics.
+// workgroup adds into its own slot (no contention); a non-returning atomic
+// add, so the workgroup does not wait for a read before it ends.
 template <class P>
 __device__ inline void add_stat(const Dev<P> &d, int which, int64_t x) {
-  if (threadIdx.x == 0 && x) d.stat_acc[which * SCAN_BLOCKS + blockIdx.x] += x;
+  if (threadIdx.x == 0 && x)
+    atomicAdd(reinterpret_cast<unsigned long long *>(
+                  d.stat_acc + which * SCAN_BLOCKS + blockIdx.x),
+              (unsigned long long)x);
 }
 
 // Calls body(valid, slot, vertex) for every active slot of the superstep,
@@ -560,9 +587,13 @@ __device__ inline void add_stat(const Dev<P> &d, int which, int64_t x) {
 // bitmap costs one read of its words plus one round; with clear = true the
 // words read are zeroed (the last reader of this superstep's bitmap).
 constexpr int BM_WORDS = 2 * NT;
-template <class P, class Body>
+struct NoWordHook {
+  __device__ void operator()(int, uint32_t) const {}
+};
+template <class P, class Body, class WordHook = NoWordHook>
 __device__ __forceinline__ void for_each_active(const Dev<P> &d, int p, bool clear,
-                                       int32_t *s_ids, int *lds, Body body) {
+                                                int32_t *s_ids, int *lds, Body body,
+                                                WordHook hook = WordHook()) {
   if (d.in_mode != IN_BITMAP) {
     const int A = frontier_size(d);
     const int stride = gridDim.x * NT;
@@ -586,6 +617,11 @@ __device__ __forceinline__ void for_each_active(const Dev<P> &d, int p, bool cle
       const int wi = w0 + k * NT + threadIdx.x;
       wd[k] = wi < we ? fw[wi] : 0u;
       cnt += __popc(wd[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < BM_WORDS / NT; ++k) {
+      const int wi = w0 + k * NT + threadIdx.x;
+      if (wi < we) hook(wi, wd[k]);
     }
     int tot;
     int pos = block_excl_scan(cnt, tot, lds);
@@ -612,11 +648,12 @@ __device__ __forceinline__ void for_each_active(const Dev<P> &d, int p, bool cle
 
 // ---------------------------------------------------------------------------
 // Frontier initialisation: setActive(start, end) (gpugas.h:342-349); clears
-// the activation bytes of both parities and the counter ring, and lists
-// [start, end) unless it is every vertex.
+// the activation and changed bytes of both parities (unless the host knows
+// no scatter ran since they were last cleared) and the counter ring, and
+// lists [start, end) unless it is every vertex.
 template <class P>
 __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
-                                                   int end) {
+                                                   int end, int clear) {
   const bool all = (start == 0 && end == d.nV);
   const int n = end - start;
   int32_t *q = d.q[p];
@@ -625,7 +662,8 @@ __global__ __launch_bounds__(NT) void k_set_active(Dev<P> d, int p, int start,
   uint32_t *f1 = reinterpret_cast<uint32_t *>(d.fl[1]);
   uint32_t *c0 = reinterpret_cast<uint32_t *>(d.chg[0]);
   uint32_t *c1 = reinterpret_cast<uint32_t *>(d.chg[1]);
-  for (int i = blockIdx.x * NT + threadIdx.x; i < d.nfwords; i += stride) {
+  for (int i = blockIdx.x * NT + threadIdx.x; clear && i < d.nfwords;
+       i += stride) {
     f0[i] = 0u;
     f1[i] = 0u;
     c0[i] = 0u;
@@ -677,7 +715,7 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
 #pragma unroll
       for (int k = 0; k < T1_MAX; ++k)
         if (k < deg)
-          val[k] = P::gatherMap(d.vdata + v, d.vdata + src[k], d.edata + o + k);
+          val[k] = gather_value(d, v, src[k], o + k);
       Acc<P> acc = Acc<P>::empty();
 #pragma unroll
       for (int k = 0; k < T1_MAX; ++k)
@@ -705,8 +743,7 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
         for (int u = 0; u < MID_UNROLL; ++u) {
           const int k = base + u * WAVE + lane;
           if (k < dj)
-            val[u] = P::gatherMap(d.vdata + vj, d.vdata + src[u],
-                                  d.edata + oj + k);
+            val[u] = gather_value(d, vj, src[u], oj + k);
         }
 #pragma unroll
         for (int u = 0; u < MID_UNROLL; ++u)
@@ -883,8 +920,7 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
       if (kind[it] == 1)
-        val[it] = P::gatherMap(d.vdata + s_vtx[kk[it]], d.vdata + src[it],
-                               d.edata + eidx[it]);
+        val[it] = gather_value(d, s_vtx[kk[it]], src[it], eidx[it]);
       else
         val[it] = GR();
     }
@@ -955,23 +991,22 @@ struct has_residual<P, std::void_t<decltype(P::residual(
     : std::true_type {};
 
 // Thread and wave tiers of the scatter for the slots whose apply returned
-// true (`ret`); hubs go to the scatter hub list.  With d.out_list the
+// true (`ret`); so / sdeg are the vertex's CSR offset and out-degree, loaded
+// by the caller next to its other first loads.  Hubs go to the scatter hub
+// list.  With d.out_list the
 // destinations are claimed (claim_vertex) and the claimed ones appended to
 // the next list q[p^1] -- one atomicAdd per workgroup for the thread tier
 // and per wave for the wave tier; otherwise the activation byte is a plain
 // store (every writer stores the same value).  Reached by every thread.
 template <class P>
 __device__ __forceinline__ int scatter_tiers(const Dev<P> &d, int p, bool ret, int slot,
-                                    int v, int32_t *s_buf, int *s_cnt,
-                                    int *s_base, int *lds) {
+                                             int so, int sdeg, int32_t *s_buf,
+                                             int *s_cnt, int *s_base, int *lds) {
   uint8_t *fl = d.fl[p ^ 1];
   int32_t *qn = d.q[p ^ 1];
   int32_t *nn = &d.ctl->ring[(d.seq + 1) & 3].n_active;
-  int o = 0, deg = 0;
-  if (ret) {
-    o = d.csr_off[v];
-    deg = d.csr_off[v + 1] - o;
-  }
+  const int o = ret ? so : 0;
+  const int deg = ret ? sdeg : 0;
   const bool small = ret && deg <= d.st1;
   if (d.out_list) {
     int dst[T1_MAX];
@@ -1053,6 +1088,11 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   double res = 0.0;
   for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     bool ret = false;
+    int so = 0, sdeg = 0;
+    if (valid && Fuse) {
+      so = d.csr_off[v];
+      sdeg = d.csr_off[v + 1] - so;
+    }
     if (valid) {
       ++active;
       if (d.in_mode == IN_LIST) d.fl[p][v] = 0;
@@ -1068,9 +1108,11 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
       } else {
         ret = P::apply(d.vdata + v, g);
       }
+      if constexpr (is_source_only<P>::value)
+        d.srcval[v] = P::gatherMap(d.vdata + v, d.vdata + v, nullptr);
     }
     if (Fuse)
-      scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
+      scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
     else if (valid)
       d.rflag[i] = ret ? 1 : 0;
   });
@@ -1094,7 +1136,7 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
 // refgas.h:143-168) without a kernel boundary between gather and apply.
 // The other buffer is complete except for vertices changed in the previous
 // superstep that are not active now: the workgroup owning their activation
-// words copies them across (chg[p^1]) before processing its frontier.
+// words copies them across (chg[p^1]) while staging its frontier.
 template <class P>
 __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
   typedef typename P::GatherResult GR;
@@ -1106,33 +1148,33 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
   __shared__ double ldsd[NT / WAVE];
   superstep_prologue(d, p, lds);
   if (threadIdx.x == 0) s_superstep = d.iter;
-  // state of vertices changed last superstep and inactive now
-  {
-    const uint32_t *aw = reinterpret_cast<const uint32_t *>(d.fl[p]);
-    uint32_t *cw = reinterpret_cast<uint32_t *>(d.chg[p ^ 1]);
-    int wb, we;
-    scan_chunk(d.nfwords, blockIdx.x, wb, we);
-    for (int wi = wb + threadIdx.x; wi < we; wi += NT) {
-      const uint32_t c = cw[wi];
-      if (!c) continue;
-      cw[wi] = 0u;
-      const uint32_t a = d.in_mode == IN_ALL ? 0xffffffffu : aw[wi];
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        if (((c >> (8 * b)) & 0xffu) && !((a >> (8 * b)) & 0xffu))
-          d.vnext[wi * 4 + b] = d.vdata[wi * 4 + b];
-    }
-  }
   __syncthreads();
   uint8_t *chg = d.chg[p];
   const int lane = lane_id();
   int gathered = 0, scattered = 0, active = 0;
   double res = 0.0;
+  // vertices changed last superstep and inactive now: copy their state into
+  // the buffer this superstep writes (bitmap input; after an all input or a
+  // frontier initialisation nothing changed)
+  uint32_t *cw = reinterpret_cast<uint32_t *>(d.chg[p ^ 1]);
+  auto fixup = [&](int wi, uint32_t a) __attribute__((always_inline)) {
+    const uint32_t c = cw[wi];
+    if (!c) return;
+    cw[wi] = 0u;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (((c >> (8 * b)) & 0xffu) && !((a >> (8 * b)) & 0xffu))
+        d.vnext[wi * 4 + b] = d.vdata[wi * 4 + b];
+  };
   for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
-    int o = 0, deg = 0;
+    int o = 0, deg = 0, so = 0, sdeg = 0;
+    VD own{};
     if (valid) {
       o = d.csc_off[v];
       deg = d.csc_off[v + 1] - o;
+      so = d.csr_off[v];
+      sdeg = d.csr_off[v + 1] - so;
+      own = d.vdata[v];
       gathered += deg;
       ++active;
     }
@@ -1146,7 +1188,7 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
 #pragma unroll
       for (int k = 0; k < T1_MAX; ++k)
         if (k < deg)
-          val[k] = P::gatherMap(d.vdata + v, d.vdata + src[k], d.edata + o + k);
+          val[k] = gather_value(d, v, src[k], o + k);
 #pragma unroll
       for (int k = 0; k < T1_MAX; ++k)
         if (k < deg) acc.add(val[k]);
@@ -1171,8 +1213,7 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
         for (int u = 0; u < MID_UNROLL; ++u) {
           const int k = base + u * WAVE + lane;
           if (k < dj)
-            val[u] = P::gatherMap(d.vdata + vj, d.vdata + src[u],
-                                  d.edata + oj + k);
+            val[u] = gather_value(d, vj, src[u], oj + k);
         }
 #pragma unroll
         for (int u = 0; u < MID_UNROLL; ++u)
@@ -1184,7 +1225,7 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
     }
     bool ret = false;
     if (valid) {
-      VD x = d.vdata[v];
+      VD x = own;
       if constexpr (has_residual<P>::value) {
         const VD before = x;
         ret = P::apply(&x, acc.finish());
@@ -1195,8 +1236,8 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
       d.vnext[v] = x;
       if (ret) chg[v] = 1;
     }
-    scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
-  });
+    scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
+  }, fixup);
   active = block_sum(active, lds);
   if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
     d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
@@ -1222,7 +1263,12 @@ __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
   int scattered = 0;
   for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     const bool ret = valid && d.rflag[i] != 0;
-    scattered += scatter_tiers(d, p, ret, i, v, s_buf, &s_cnt, &s_base, lds);
+    int so = 0, sdeg = 0;
+    if (ret) {
+      so = d.csr_off[v];
+      sdeg = d.csr_off[v + 1] - so;
+    }
+    scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
   });
   scattered = block_sum(scattered, lds);
   add_stat(d, 1, scattered);
@@ -1327,6 +1373,16 @@ __global__ __launch_bounds__(NT) void k_compact2(Dev<P> d, int p) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0)
     d.ctl->ring[(d.seq + 1) & 3].n_active = total;
+}
+
+// gatherMap(v) of every vertex, after vertex state changed outside apply.
+template <class P>
+__global__ void k_srcval_all(Dev<P> d) {
+  if constexpr (is_source_only<P>::value) {
+    for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+         v += (int64_t)gridDim.x * NT)
+      d.srcval[v] = P::gatherMap(d.vdata + v, d.vdata + v, nullptr);
+  }
 }
 
 // Vertex-data initialisation on the device: `value` everywhere, `one` at

@@ -73,12 +73,15 @@ struct SSSP {
                                           const VertexData *, EdgeData *) {}
 };
 
-// pagerank.cu:27-73, plus two opt-in traits of this engine: compensated
-// summation of the gather, and an FP64 residual |new - old| per vertex.
+// pagerank.cu:27-73, plus three opt-in traits of this engine: compensated
+// summation of the gather, gatherMap reading only the source (so the engine
+// can keep rank / numOutEdges per vertex), and an FP64 residual
+// |new - old| per vertex.
 struct PageRank {
   static constexpr float pageConst = 0.15f;
   static constexpr float tol = 0.01f;
   static constexpr bool compensatedSum = true;
+  static constexpr bool gatherSourceOnly = true;
 
   struct VertexData {
     float rank;
@@ -119,6 +122,7 @@ struct PageRank {
 // reference; it is the one-query-at-a-time baseline of the batched PPR.
 struct PersonalizedPageRank {
   static constexpr bool compensatedSum = true;
+  static constexpr bool gatherSourceOnly = true;
   struct VertexData {
     float rank;
     int numOutEdges;

@@ -49,13 +49,33 @@ def test_batch_rmat_hubs(gpu, kind):
     src[0] = 0  # the largest hub
     b.add(np.arange(64), src)
     b.run()
-    for lane in list(range(0, 64, 5)) + [0]:
-        got = b.result(lane)
+    many = b.results(range(64))  # one extraction pass for all lanes
+    for lane in range(64):
         if kind == gas.BATCH_BFS:
             want = oracle.bfs(n, s, d, int(src[lane]))[0]
         else:
             want = oracle.sssp(n, s, d, w, int(src[lane]))[0]
-        assert np.array_equal(got, want), lane
+        assert np.array_equal(many[lane], want), lane
+    for lane in (0, 17, 63):
+        assert np.array_equal(b.result(lane), many[lane])
+
+
+@pytest.mark.parametrize("divisor", [0, 1, 4, 64])
+def test_batch_bfs_push_pull_mix(gpu, divisor):
+    # 0: push only; larger divisors pull once the frontier's out-edges reach
+    # E / divisor -- the direction must not change any depth
+    el = workloads.rmat(13, symmetrize=True)
+    s, d, _ = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    b = gas.Batch(g, gas.BATCH_BFS)
+    b.set_pull_divisor(divisor)
+    src = _sources(np.random.default_rng(9), el, 64)
+    b.add(np.arange(64), src)
+    b.run()
+    many = b.results(range(0, 64, 3))
+    for lane, got in many.items():
+        assert np.array_equal(got, oracle.bfs(n, s, d, int(src[lane]))[0]), lane
 
 
 @pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP])

@@ -473,6 +473,18 @@ class Batch:
         _check(lib().gas_batch_run(self._h, C.byref(st)))
         return st
 
+    def results(self, lanes) -> dict:
+        """{lane: result} for several lanes, extracted in one device pass."""
+        lanes = list(lanes)
+        bufs = {s: DeviceBuffer(4 * self.graph.n_vertices) for s in lanes}
+        ptrs = (C.c_void_p * 64)()
+        mask = 0
+        for s, b in bufs.items():
+            ptrs[s] = b.ptr
+            mask |= 1 << s
+        _check(lib().gas_batch_results_device(self._h, C.c_uint64(mask), ptrs))
+        return {s: b.to_numpy() for s, b in bufs.items()}
+
     def result(self, lane: int) -> np.ndarray:
         if self.kind == BATCH_PPR:
             out = np.zeros(self.graph.n_vertices, np.float32)

@@ -443,6 +443,7 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_ppr_init, grid, e.stream(), e.dev(), source);
+    e.stateChanged();
     for (int it = 0; it < iterations; ++it) {
       e.setActive(0, g->nV);
       e.gatherApply(true);

@@ -148,7 +148,21 @@ int gas_batch_result_device(gas_batch_t b, int32_t lane, int32_t *dev_out) {
   BATCH_OR_FAIL(b);
   if (!dev_out || !b->e)
     return set_error(GAS_ERR_ARG, "gas_batch_result_device: null or PPR batch");
-  return guarded_b([&] { b->e->laneResultDevice(lane, dev_out); });
+  return guarded_b([&] {
+    b->e->laneResultDevice(lane, dev_out);
+    b->e->sync();
+  });
+}
+
+int gas_batch_results_device(gas_batch_t b, uint64_t lanes,
+                             int32_t *const *dev_outs) {
+  BATCH_OR_FAIL(b);
+  if (!dev_outs || !b->e)
+    return set_error(GAS_ERR_ARG, "gas_batch_results_device: null or PPR batch");
+  return guarded_b([&] {
+    b->e->laneResultsDevice(lanes, dev_outs);
+    b->e->sync();
+  });
 }
 
 int gas_batch_result_f32(gas_batch_t b, int32_t lane, float *host_out) {
@@ -162,7 +176,10 @@ int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out) {
   BATCH_OR_FAIL(b);
   if (!dev_out || !b->pp)
     return set_error(GAS_ERR_ARG, "gas_batch_result_f32_device: bad batch");
-  return guarded_b([&] { b->pp->laneResultDevice(lane, dev_out); });
+  return guarded_b([&] {
+    b->pp->laneResultDevice(lane, dev_out);
+    b->pp->sync();
+  });
 }
 
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor) {

@@ -184,16 +184,18 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
             e.be ? e.be->aliveSnapshot() : e.pe->lanesAlive();
         const uint64_t fin_all = e.running & ~alive;
         if (!fin_all) continue;
+        int32_t *outs[BW] = {};
         for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
           const int lane = __builtin_ctzll(fin);
           if (e.be) {
             if (e.res_i[lane].n < (size_t)e.V) e.res_i[lane].alloc((size_t)e.V);
-            e.be->laneResultDevice(lane, e.res_i[lane].p);
+            outs[lane] = e.res_i[lane].p;
           } else {
             if (e.res_f[lane].n < (size_t)e.V) e.res_f[lane].alloc((size_t)e.V);
             e.pe->laneResultDevice(lane, e.res_f[lane].p);
           }
         }
+        if (e.be) e.be->laneResultsDevice(fin_all, outs);
         check(gasrt_stream_synchronize(e.stream()), "sync");
         const double tf = now_s() - t0;
         for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
