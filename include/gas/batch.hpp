@@ -813,8 +813,8 @@ class BatchEngine {
 
   // step() followed by an asynchronous snapshot of the control block and an
   // event; poll() tells when both are done, aliveSnapshot() reads it.
-  void stepAsync() {
-    step();
+  void stepAsync(int rounds = 1) {
+    for (int i = 0; i < rounds; ++i) step();
     snapshotAsync();
   }
   void snapshotAsync() {
@@ -939,6 +939,7 @@ struct PprDev {
   const int32_t *mv;              // vertices with several items
   int32_t n_items, n_multi;
   float *rank[2];                 // [V][BW]
+  float *contrib[2];              // [V][BW] rank / outdeg, what gathers read
   uint64_t *mask[2];              // lanes with a nonzero rank
   float *part_s, *part_c;         // [slots][BW]
   const int32_t *src_lane;        // [BW]
@@ -950,7 +951,9 @@ __device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
   const bool on = (run >> lane) & 1ull;
   const float tele = (on && d.src_lane[lane] == v) ? 0.15f : 0.0f;
   const float val = on ? tele + 0.85f * sum : 0.0f;
+  const int od = d.outdeg[v];
   d.rank[p ^ 1][(size_t)v * BW + lane] = val;
+  d.contrib[p ^ 1][(size_t)v * BW + lane] = od > 0 ? val / (float)od : 0.0f;
   const uint64_t m = __ballot(val != 0.0f);
   if (lane == 0) d.mask[p ^ 1][v] = m;
 }
@@ -958,7 +961,7 @@ __device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
 static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
-  const float *rk = d.rank[p];
+  const float *ct = d.contrib[p];
   const uint64_t *mk = d.mask[p];
   for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < d.n_items;
        it += waves) {
@@ -968,16 +971,15 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
     float s = 0.f, c = 0.f;
     for (int base = e0; base < e1; base += WAVE) {
       const int k = base + lane;
-      int u_l = 0, od_l = 1;
+      int u_l = 0;
       uint64_t m_l = 0;
       if (k < e1) {
         u_l = d.csc_src[k];
         m_l = mk[u_l] & run;
-        od_l = d.outdeg[u_l];
       }
       uint64_t nz = __ballot(m_l != 0);
       while (nz) {
-        int u[PPR_UNROLL], od[PPR_UNROLL];
+        int u[PPR_UNROLL];
         bool on[PPR_UNROLL];
         float r[PPR_UNROLL];
 #pragma unroll
@@ -986,7 +988,6 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
           const bool have = nz != 0;
           nz &= nz - 1;
           u[x] = __shfl(u_l, j, WAVE);
-          od[x] = __shfl(od_l, j, WAVE);
           const uint64_t mj =
               ((uint64_t)__shfl((uint32_t)(m_l >> 32), j, WAVE) << 32) |
               __shfl((uint32_t)m_l, j, WAVE);
@@ -994,11 +995,11 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
         }
 #pragma unroll
         for (int x = 0; x < PPR_UNROLL; ++x)
-          r[x] = on[x] ? rk[(size_t)u[x] * BW + lane] : 0.f;
+          r[x] = on[x] ? ct[(size_t)u[x] * BW + lane] : 0.f;
 #pragma unroll
         for (int x = 0; x < PPR_UNROLL; ++x) {
           if (!on[x]) continue;
-          const float y = r[x] / (float)od[x];
+          const float y = r[x];  // rank(u) / outdeg(u), divided once by the writer
           const float t = s + y;
           const float bp = t - s;
           c += (s - (t - bp)) + (y - bp);
@@ -1043,7 +1044,10 @@ static __global__ __launch_bounds__(NT) void kp_reset(PprDev d, int p,
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < total;
        i += stride) {
     const int s = (int)(i & (BW - 1));
-    if ((reset >> s) & 1ull) d.rank[p][i] = 0.f;
+    if ((reset >> s) & 1ull) {
+      d.rank[p][i] = 0.f;
+      d.contrib[p][i] = 0.f;
+    }
     if (s == 0) d.mask[p][i / BW] &= ~reset;
   }
 }
@@ -1054,6 +1058,8 @@ static __global__ void kp_seed(PprDev d, int p, const int32_t *lanes,
   if (i >= n) return;
   const int s = lanes[i], v = sources[i];
   d.rank[p][(size_t)v * BW + s] = 0.15f;
+  const int od = d.outdeg[v];
+  d.contrib[p][(size_t)v * BW + s] = od > 0 ? 0.15f / (float)od : 0.0f;
   atomicOr((unsigned long long *)&d.mask[p][v],
            (unsigned long long)(1ull << s));
 }
@@ -1099,6 +1105,8 @@ class PprBatch {
     m_mv.upload(mv, m_stream);
     m_rank0.alloc(V * BW);
     m_rank1.alloc(V * BW);
+    m_contrib0.alloc(V * BW);
+    m_contrib1.alloc(V * BW);
     m_mask0.alloc(V);
     m_mask1.alloc(V);
     m_ps.alloc((size_t)std::max(slots, 1) * BW);
@@ -1108,6 +1116,8 @@ class PprBatch {
     m_sources.alloc(BW);
     check(gasrt_memset(m_rank0.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_rank1.p, 0, V * BW * 4, m_stream), "memset");
+    check(gasrt_memset(m_contrib0.p, 0, V * BW * 4, m_stream), "memset");
+    check(gasrt_memset(m_contrib1.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_mask0.p, 0, V * 8, m_stream), "memset");
     check(gasrt_memset(m_mask1.p, 0, V * 8, m_stream), "memset");
     for (int i = 0; i < BW; ++i) m_hsrc[i] = -1;
@@ -1127,6 +1137,8 @@ class PprBatch {
     d.n_multi = (int32_t)mv.size();
     d.rank[0] = m_rank0.p;
     d.rank[1] = m_rank1.p;
+    d.contrib[0] = m_contrib0.p;
+    d.contrib[1] = m_contrib1.p;
     d.mask[0] = m_mask0.p;
     d.mask[1] = m_mask1.p;
     d.part_s = m_ps.p;
@@ -1187,8 +1199,18 @@ class PprBatch {
       if ((run >> s) & 1ull) ++m_done[s];
   }
 
-  void stepAsync() {
-    step();
+  // Rounds until the first running lane finishes (a finished lane must be
+  // extracted before the next round overwrites its buffer).
+  int roundsToFirstFinish() const {
+    int r = 1 << 30;
+    for (int s = 0; s < BW; ++s)
+      if (((m_used >> s) & 1ull) && m_done[s] < m_iters)
+        r = std::min(r, m_iters - m_done[s]);
+    return r == (1 << 30) ? 1 : r;
+  }
+  void stepAsync(int rounds = 1) {
+    rounds = std::max(1, std::min(rounds, roundsToFirstFinish()));
+    for (int i = 0; i < rounds; ++i) step();
     if (!m_ev) check(gasrt_event_create(&m_ev), "event");
     check(gasrt_event_record(m_ev, m_stream), "event");
   }
@@ -1240,7 +1262,7 @@ class PprBatch {
   PprDev m_dev{};
   DevBuf<int32_t> m_outdeg, m_itv, m_ite0, m_pslot, m_vitem0, m_mv, m_src,
       m_lanes, m_sources;
-  DevBuf<float> m_rank0, m_rank1, m_ps, m_pc, m_tmp;
+  DevBuf<float> m_rank0, m_rank1, m_contrib0, m_contrib1, m_ps, m_pc, m_tmp;
   DevBuf<uint64_t> m_mask0, m_mask1;
 };
 

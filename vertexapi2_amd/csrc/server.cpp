@@ -6,9 +6,10 @@
 // loop (gas_server_replay) admits arrived queries into free lanes of their
 // engine while the per-query state memory fits the budget, adds them between
 // rounds of the engines already running, steps every engine that has work,
-// and retires lanes whose query finished: the lane's result is extracted
-// into a per-lane buffer in HBM (and optionally copied to the caller), the
-// lane and its memory are released, and the next waiting query can take it.
+// and retires lanes whose query finished: the lanes' results are extracted
+// into per-lane buffers in HBM in one pass (and optionally copied to the
+// caller), the lanes and their memory are released, and waiting queries can
+// take them.
 // A query's latency is the time from its arrival to its result being in HBM.
 #include <hip/hip_runtime.h>
 
@@ -34,12 +35,16 @@ double now_s() {
   return std::chrono::duration<double>(C::now() - t0).count();
 }
 
+// Rounds an engine runs between two looks at its state when no query is
+// waiting for it: a finished lane waits at most this many rounds longer.
+constexpr int kRoundsPerPoll = 4;
+
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
   switch (kind) {
     case GAS_BATCH_BFS: return (size_t)(V * 1 + V * 3 / 8 + V * 4);
     case GAS_BATCH_SSSP: return (size_t)(V * 4 + V / 4 + V * 4);
-    default: return (size_t)(V * 8 + V / 4 + V * 4);  // PPR
+    default: return (size_t)(V * 16 + V / 4 + V * 4);  // PPR
   }
 }
 
@@ -248,8 +253,14 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
         s->reserved += need;
         it = waiting.erase(it);
       }
-      // 3. start the next round of every idle engine with work
-      for (auto &ep : s->engines) {
+      // 3. start the next rounds of every idle engine with work: one round
+      // when a query waits for the engine (it joins after that round),
+      // several otherwise (fewer host round trips per query)
+      std::vector<char> wants(s->engines.size(), 0);
+      for (int q : waiting)
+        wants[(size_t)kinds[q] * s->graphs.size() + graph_ids[q]] = 1;
+      for (size_t ei = 0; ei < s->engines.size(); ++ei) {
+        auto &ep = s->engines[ei];
         if (!ep || ep->inflight) continue;
         Engine &e = *ep;
         if (!e.add_lanes.empty()) {
@@ -264,10 +275,11 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
           e.add_src.clear();
         }
         if (e.running) {
+          const int k = wants[ei] ? 1 : kRoundsPerPoll;
           if (e.be)
-            e.be->stepAsync();
+            e.be->stepAsync(k);
           else
-            e.pe->stepAsync();
+            e.pe->stepAsync(k);
           e.inflight = true;
           ++rounds;
           progressed = true;
