@@ -364,7 +364,7 @@ def served_lines(graphs, rates=(100, 1000, 5000, 20000), mix=(0.5, 0.3, 0.2)):
         lat_p, end_p = serve.fifo_latencies(st["arrivals"], s_plain)
         out["rates"][str(rate)] = {
             "served": {k: res[k] for k in ("qps", "p50_ms", "p99_ms", "max_concurrent",
-                                           "rounds")},
+                                           "rounds", "solo")},
             "plain_fifo": {"qps": round(n / end_p, 2),
                            "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
                            "p99_ms": round(float(np.percentile(lat_p, 99)), 3)},

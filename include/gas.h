@@ -331,7 +331,8 @@ typedef struct {
   int64_t rounds;               /* host loop iterations that stepped work  */
   int64_t admissions_deferred;  /* times a waiting query did not fit      */
   int32_t max_concurrent;       /* most queries running at once           */
-  int32_t reserved;
+  int32_t solo;                 /* BFS/SSSP queries answered by the plain  */
+                                /* engine alone (own engine idle)         */
 } gas_serve_stats;
 
 /* Per-query state bytes charged against the budget (lane state + result). */

@@ -57,3 +57,26 @@ def test_admission_by_memory(gpu):
     with pytest.raises(gas.GasError) as ex:
         serve.serve(tiny, stream)
     assert ex.value.code == 6  # GAS_ERR_FULL
+
+
+def test_idle_server_answers_alone(gpu):
+    # at a low rate queries find the server idle: BFS / SSSP are answered by
+    # the plain engine at once (a batch of one), PPR still by its batch
+    el = workloads.road(64, 64, seed=8)
+    road = gas.Graph(el)
+    srv = gas.Server([road], memory_budget=1 << 30)
+    stream = serve.make_stream(rate_qps=200, n=24, graphs=[road], seed=13)
+    out, lat, results = serve.serve(srv, stream, keep=list(range(24)))
+    n_bfs_sssp = int(np.sum(stream["kinds"] != gas.BATCH_PPR))
+    assert 0 < out["solo"] <= n_bfs_sssp
+    s, d, w = el.arrays()
+    n = road.n_vertices
+    for q in range(24):
+        k, src = stream["kinds"][q], int(stream["sources"][q])
+        if k == gas.BATCH_BFS:
+            assert np.array_equal(results[q], oracle.check_bfs(n, s, d, src)), q
+        elif k == gas.BATCH_SSSP:
+            assert np.array_equal(results[q], oracle.check_dijkstra(n, s, d, w, src)), q
+        else:
+            ref = oracle.check_ppr_fp64(n, s, d, src, 10)
+            assert np.abs(results[q] - ref).sum() <= 1e-5 * np.abs(ref).sum(), q

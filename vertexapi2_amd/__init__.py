@@ -502,10 +502,10 @@ class Batch:
 class ServeStats(C.Structure):
     _fields_ = [("elapsed_s", C.c_double), ("rounds", C.c_int64),
                 ("admissions_deferred", C.c_int64), ("max_concurrent", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("solo", C.c_int32)]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def query_bytes(kind: int, n_vertices: int) -> int:

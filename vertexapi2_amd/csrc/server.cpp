@@ -1,5 +1,10 @@
 // Continuous-batching query server over graphs resident in HBM.
 //
+// A BFS or SSSP query whose engine is idle when it waits (and that no
+// earlier query for that engine precedes) is answered by the plain engine at
+// once when that is cheap for its kind and graph (a batch of one); queries
+// that find their engine running are batched.
+//
 // One batch engine per (query kind, graph) -- BFS and SSSP lanes
 // (BatchEngine), personalised PageRank lanes (PprBatch) -- each on its own
 // stream, so rounds of different engines overlap on the device.  The host
@@ -39,6 +44,11 @@ double now_s() {
 // waiting for it: a finished lane waits at most this many rounds longer.
 constexpr int kRoundsPerPoll = 4;
 
+// A query that arrives at an idle server is answered by the plain engine
+// alone (a batch of one) when that takes at most this long on average for
+// its kind and graph (measured as the server runs).
+constexpr double kSoloMaxMs = 2.0;
+
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
   switch (kind) {
@@ -70,6 +80,9 @@ struct gas_server {
   size_t budget = 0, reserved = 0;
   int ppr_iterations = 10;
   std::vector<std::unique_ptr<Engine>> engines;  // [kind * n_graphs + graph]
+  std::vector<double> solo_ms;                   // plain-engine cost per key
+  bool calibrated = false;
+  std::vector<DevBuf<int32_t>> solo_out;         // [graph] plain results
 
   Engine &engine(int kind, int graph) {
     auto &slot = engines[(size_t)kind * graphs.size() + graph];
@@ -134,6 +147,8 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
     s->budget = memory_budget_bytes;
     s->ppr_iterations = ppr_iterations;
     s->engines.resize(3 * (size_t)n_graphs);
+    s->solo_ms.assign(3 * (size_t)n_graphs, 1e30);
+    s->solo_out.resize((size_t)n_graphs);
     *out = s.release();
   });
 }
@@ -165,10 +180,31 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       return set_error(GAS_ERR_ARG, "gas_server_replay: arrivals not sorted");
   }
   return guarded_s([&] {
+    // plain-engine cost of BFS and SSSP per graph, measured once on an idle
+    // server (the first run of each builds that plain engine)
+    if (!s->calibrated) {
+      for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
+        auto &buf = s->solo_out[gi];
+        if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
+        for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP}) {
+          auto run = [&] {
+            return kind == GAS_BATCH_BFS
+                       ? gas_bfs_device(s->graphs[gi], 0, buf.p, nullptr)
+                       : gas_sssp_device(s->graphs[gi], 0, buf.p, nullptr);
+          };
+          if (run() != GAS_OK) continue;  // e.g. SSSP on an unweighted graph
+          const double ts = now_s();
+          if (run() != GAS_OK) continue;
+          s->solo_ms[(size_t)kind * s->graphs.size() + gi] = (now_s() - ts) * 1e3;
+        }
+      }
+      clear_error();
+      s->calibrated = true;
+    }
     std::deque<int> waiting;
     int next = 0, done = 0;
     int64_t rounds = 0, admissions_deferred = 0;
-    int max_running = 0;
+    int max_running = 0, solo = 0;
     const double t0 = now_s();
     auto running_total = [&] {
       int r = 0;
@@ -228,6 +264,45 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       // engines between rounds
       const double t = now_s() - t0;
       while (next < n && arrival_s[next] <= t) waiting.push_back(next++);
+      // a waiting BFS / SSSP query whose engine is idle and which is the
+      // first waiting query for that engine: the plain engine answers it now
+      // on its own stream (other engines' rounds keep running; the cost cap
+      // bounds how long their polling pauses).  One per loop iteration.
+      {
+        std::vector<char> seen(s->engines.size(), 0);
+        for (auto it = waiting.begin(); it != waiting.end(); ++it) {
+          const int q = *it;
+          const size_t key = (size_t)kinds[q] * s->graphs.size() + graph_ids[q];
+          if (seen[key]) continue;
+          seen[key] = 1;
+          const auto &ep = s->engines[key];
+          const bool idle =
+              !ep || (!ep->inflight && !ep->running && ep->add_lanes.empty());
+          // BFS / SSSP only: their results are exact on either engine, while
+          // a PPR lane stays bit-identical alone and in any batch
+          if (!idle || kinds[q] == GAS_BATCH_PPR || s->solo_ms[key] > kSoloMaxMs)
+            continue;
+          const int gi = graph_ids[q];
+          auto &buf = s->solo_out[gi];
+          if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
+          const int rc =
+              kinds[q] == GAS_BATCH_BFS
+                  ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
+                  : gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr);
+          if (rc != GAS_OK) throw Error(rc, gas_last_error());
+          const double tf = now_s();
+          latency_ms[q] = (tf - t0 - arrival_s[q]) * 1e3;
+          if (host_results && host_results[q])
+            check(gasrt_memcpy_d2h(host_results[q], buf.p, 4 * (size_t)s->nV[gi],
+                                   nullptr),
+                  "result copy");
+          waiting.erase(it);
+          ++done;
+          ++solo;
+          progressed = true;
+          break;
+        }
+      }
       bool blocked[3 * 64] = {};
       for (auto it = waiting.begin(); it != waiting.end();) {
         const int q = *it;
@@ -305,6 +380,7 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       stats->rounds = rounds;
       stats->admissions_deferred = admissions_deferred;
       stats->max_concurrent = max_running;
+      stats->solo = solo;
     }
   });
 }
