@@ -83,6 +83,10 @@ def reached_edges(values, csr_off, unreached):
 #   k_compact1    V (activation bytes); k_compact2: V + 4 per listed id
 #   k_fused       per active vertex: 8 (offsets) + state read + state write;
 #                 per in-edge as k_gather_vp; per scattered edge 5
+#   k_small       (runs that never left the one-workgroup mode) per active
+#                 vertex: 4 (list id) + 8 (CSR offsets) + state read + write,
+#                 with a gather + 8 (CSC offsets); per in-edge as
+#                 k_gather_vp; per scattered edge 5
 def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
     """{kernel: total algorithmic bytes over the run}.  hub_split =
     (hub_vertices, hub_edges) per sweep for all-active runs, splitting the
@@ -93,6 +97,10 @@ def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
     A, Eg, Es = st.active_total, st.gathered_edges, st.scattered_edges
     out = {}
     ks = st.kernels()
+    if set(ks) <= {"k_small"}:
+        out["k_small"] = (A * (12 + 2 * vd + (8 if have_gather else 0)) +
+                          Eg * (4 + ed + sv) + Es * 5)
+        return out
     if have_gather and "k_fused" in ks and "k_gather_vp" not in ks:
         # every superstep ran as one launch: gather + apply + scatter
         out["k_fused"] = A * (8 + 2 * vd) + Eg * (4 + ed + sv) + Es * 5

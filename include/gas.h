@@ -184,7 +184,8 @@ enum {
   GAS_K_COMPACT1 = 8,  /* k_compact1: frontier count per block             */
   GAS_K_COMPACT2 = 9,  /* k_compact2: ordered frontier write               */
   GAS_K_FUSED = 10,    /* k_fused: gather + apply + scatter in one launch  */
-  GAS_K_SLOTS = 12     /* slot 11 unused                                  */
+  GAS_K_SMALL = 11,    /* k_small: supersteps of a small frontier, one WG  */
+  GAS_K_SLOTS = 12
 };
 
 typedef struct {
@@ -224,6 +225,16 @@ int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers);
  * 2^24 vertices, 0 below.  Replaces the reference's dense / sparse scatter
  * choice (gpugas.h:669-751). */
 int gas_graph_set_frontier_switch(gas_graph_t g, int program, int32_t divisor);
+/* Small-frontier mode of whole runs on graphs whose in- and out-degrees are
+ * all at most 64: while a list frontier has at most max_vertices vertices,
+ * consecutive supersteps run inside one workgroup (barriers instead of
+ * kernel launches).  0 turns it off; default 4096 for graphs of at most
+ * 8192 vertices, 0 above (measured slower there: each vertex's loads and
+ * claims form one long dependent chain inside the workgroup).  No
+ * counterpart in the reference, which launches every superstep
+ * (gpugas.h:764-772). */
+int gas_graph_set_small_frontier(gas_graph_t g, int program,
+                                 int32_t max_vertices);
 /* Autotuner: times candidate tiers on this graph for the program (gather
  * and scatter phases separately, vertex data untouched) and keeps the
  * fastest; tiers_out (may be NULL) receives them. */

@@ -46,7 +46,8 @@ constexpr int SSSP_ITEM_EDGES = 64;  // SSSP push: out-edges per item (one wave)
 constexpr int BFS_ITEMS_PER_TRIP = 8;  // BFS push: items in flight per wave
 constexpr int HUB_ITEMS = 256;    // a vertex with more items is written by
                                   // a whole workgroup of kb_hub_items
-constexpr int SSSP_UNROLL = 8;    // edges per wave trip in the SSSP push
+constexpr int SSSP_UNROLL = 8;    // edges per item per wave trip in the SSSP push
+constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks together
 constexpr int32_t SSSP_INF = INT32_MAX - 100000;  // sssp.cu:32-33
 
 struct BatchCtl {
@@ -519,38 +520,61 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   const int n_items = c->n_items;
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
-  for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < n_items;
-       it += waves) {
-    const int4 item = d.item[it];
-    const int u = item.x, e0 = item.y, ne = item.z - item.y;
-    const uint64_t m = d.chg[u];
-    const bool act = (m >> lane) & 1ull;
-    const int du = d.dist[(size_t)u * BW + lane];
-    int my_w = 0, my_wt = 0;
-    if (lane < ne) {
-      my_w = d.csr_dst[e0 + lane];
-      my_wt = d.csr_w[e0 + lane];
+  constexpr int G = SSSP_ITEMS_PER_TRIP;
+  for (int base = (blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE) * G;
+       base < n_items; base += waves * G) {
+    int u[G], e0[G], ne[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int4 item =
+          base + g < n_items ? d.item[base + g] : make_int4(0, 0, 0, 0);
+      u[g] = item.x;
+      e0[g] = item.y;
+      ne[g] = item.z - item.y;
     }
-    for (int j0 = 0; j0 < ne; j0 += SSSP_UNROLL) {
-      int w[SSSP_UNROLL], cand[SSSP_UNROLL], cur[SSSP_UNROLL];
+    bool act[G];
+    int du[G], my_w[G], my_wt[G];
 #pragma unroll
-      for (int x = 0; x < SSSP_UNROLL; ++x) {
-        w[x] = __shfl(my_w, j0 + x, WAVE);
-        cand[x] = du + __shfl(my_wt, j0 + x, WAVE);
+    for (int g = 0; g < G; ++g) {
+      act[g] = ne[g] > 0 && ((d.chg[u[g]] >> lane) & 1ull);
+      du[g] = ne[g] > 0 ? d.dist[(size_t)u[g] * BW + lane] : SSSP_INF;
+      my_w[g] = 0;
+      my_wt[g] = 0;
+      if (lane < ne[g]) {
+        my_w[g] = d.csr_dst[e0[g] + lane];
+        my_wt[g] = d.csr_w[e0[g] + lane];
       }
+    }
+    int mx = 0;
 #pragma unroll
-      for (int x = 0; x < SSSP_UNROLL; ++x)
-        cur[x] = (act && j0 + x < ne) ? d.dist[(size_t)w[x] * BW + lane]
-                                      : SSSP_INF;
+    for (int g = 0; g < G; ++g) mx = max(mx, ne[g]);
+    for (int j0 = 0; j0 < mx; j0 += SSSP_UNROLL) {
+      int w[G][SSSP_UNROLL], cand[G][SSSP_UNROLL], cur[G][SSSP_UNROLL];
 #pragma unroll
-      for (int x = 0; x < SSSP_UNROLL; ++x) {
-        const bool imp = act && j0 + x < ne && cand[x] < cur[x];
-        if (imp) atomicMin(&d.dist[(size_t)w[x] * BW + lane], cand[x]);
-        const uint64_t bm = __ballot(imp);
-        if (bm && lane == 0)
-          atomicOr((unsigned long long *)&d.chg_next[w[x]],
-                   (unsigned long long)bm);
-      }
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int x = 0; x < SSSP_UNROLL; ++x) {
+          w[g][x] = __shfl(my_w[g], j0 + x, WAVE);
+          cand[g][x] = du[g] + __shfl(my_wt[g], j0 + x, WAVE);
+        }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int x = 0; x < SSSP_UNROLL; ++x)
+          cur[g][x] = (act[g] && j0 + x < ne[g])
+                          ? d.dist[(size_t)w[g][x] * BW + lane]
+                          : SSSP_INF;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int x = 0; x < SSSP_UNROLL; ++x) {
+          const bool imp = act[g] && j0 + x < ne[g] && cand[g][x] < cur[g][x];
+          if (imp) atomicMin(&d.dist[(size_t)w[g][x] * BW + lane], cand[g][x]);
+          const uint64_t bm = __ballot(imp);
+          if (bm && lane == 0)
+            atomicOr((unsigned long long *)&d.chg_next[w[g][x]],
+                     (unsigned long long)bm);
+        }
     }
   }
 }

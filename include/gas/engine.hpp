@@ -70,6 +70,16 @@ inline void launch(void (*k)(KP...), unsigned grid, gasrt_stream_t s,
   check(gasrt_launch((const void *)k, grid, NT, argv, 0, s), "launch");
 }
 
+// The same with an explicit workgroup size.
+template <class... KP, class... A>
+inline void launch_wg(void (*k)(KP...), unsigned grid, unsigned block,
+                      gasrt_stream_t s, A &&...a) {
+  std::tuple<std::decay_t<KP>...> t(std::forward<A>(a)...);
+  void *argv[sizeof...(KP) + 1];
+  arg_ptrs(t, argv, std::index_sequence_for<KP...>{});
+  check(gasrt_launch((const void *)k, grid, block, argv, 0, s), "launch");
+}
+
 // Process-wide switch for per-launch kernel timing (gas_set_kernel_timing).
 inline int &kernel_timing_flag() {
   static int v = 0;
@@ -301,14 +311,23 @@ class GASEngineGPU {
   // scatter of each superstep builds a list when the last reported frontier
   // is sparse (|A| * switch < V), a bitmap otherwise.  The run ends at the
   // first reported empty frontier; supersteps enqueued after it are no-ops.
+  // On graphs whose degrees are all small, a list frontier of at most
+  // m_small_max vertices is handed to k_small, which runs supersteps in one
+  // workgroup until the frontier outgrows it or empties.
   void runUntilEmpty(bool haveGather, bool /*haveScatter*/) {
-    const int64_t a0 = countActive();
-    if (a0 == 0) return;
-    for (int i = 0; i < STATUS_SLOTS; ++i)
-      __atomic_store_n(m_status + i, (int64_t)0, __ATOMIC_RELEASE);
+    int64_t known = countActive();  // exact size of a list / all input
+    if (known == 0) return;
+    const bool small_ok = m_small_max > 0 && m_max_in <= SMALL_DEG_MAX &&
+                          m_max_out <= SMALL_DEG_MAX;
     const int64_t s0 = m_seq;
     const int it0 = m_iter;
-    int64_t next = s0, last_a = a0, empty_at = -1;
+    int64_t next = s0, last_a = known, empty_at = -1;
+    auto reset_status = [&] {
+      for (int i = 0; i < STATUS_SLOTS; ++i)
+        __atomic_store_n(m_status + i, (int64_t)0, __ATOMIC_RELEASE);
+      next = m_seq;
+    };
+    reset_status();
     // consumes every report that has arrived, in superstep order
     auto poll = [&]() {
       bool got = false;
@@ -323,10 +342,46 @@ class GASEngineGPU {
       }
       return got;
     };
+    bool drain = false;  // the superstep just enqueued builds a list: sync
     while (empty_at < 0) {
-      superstep(haveGather, m_switch > 0 &&
-                                last_a * (int64_t)m_switch < (int64_t)m_nV);
+      if (small_ok && m_in == IN_LIST && known >= 0 && known <= m_small_max) {
+        smallSteps(haveGather);
+        const Ctl c = readCtl();
+        if (c.small_empty) {
+          empty_at = m_seq;
+          break;
+        }
+        known = c.small_n;
+        last_a = known;
+        reset_status();
+      }
+      const bool want_list =
+          (m_switch > 0 && last_a * (int64_t)m_switch < (int64_t)m_nV) ||
+          (small_ok && last_a <= m_small_max / 2);
+      superstep(haveGather, want_list);
+      known = -1;
       poll();
+      if (empty_at >= 0) break;
+      if (small_ok && want_list && last_a <= m_small_max / 2) drain = true;
+      if (drain) {
+        // the list just built has an exact size once the stream is idle
+        check(gasrt_stream_synchronize(m_stream), "sync");
+        poll();
+        if (empty_at >= 0) break;
+        if (next < m_seq && bitmapInputSize(m_par ^ 1) == 0) {
+          // the last superstep's input was an (unreported) empty bitmap
+          empty_at = m_seq - 1;
+          break;
+        }
+        known = readCtl().ring[m_seq & 3].n_active;
+        if (known == 0) {
+          empty_at = m_seq;
+          break;
+        }
+        drain = false;
+        reset_status();
+        continue;
+      }
       for (int64_t spin = 0; empty_at < 0 && m_seq - next > kDepth + 1; ++spin) {
         if (poll()) continue;
         if ((spin & 255) == 255) {
@@ -343,6 +398,12 @@ class GASEngineGPU {
     m_supersteps -= (m_seq - s0) - real;
     m_iter = it0 + (int)real;
     setActive(0, 0);
+  }
+
+  // Largest list frontier handed to k_small (0: never).
+  void setSmallFrontier(int max_vertices) {
+    if (max_vertices < 0) throw Error(GAS_ERR_ARG, "small frontier must be >= 0");
+    m_small_max = max_vertices;
   }
 
   void setIteration(int it) { m_iter = it; }
@@ -512,6 +573,50 @@ class GASEngineGPU {
       klaunch(GAS_K_HGATHER, 0, k_hgather<Program>, MP_BLOCKS);
       klaunch(GAS_K_HFIX, 0, k_hfix<Program>, SCAN_BLOCKS);
     }
+  }
+
+  // Active vertices of the bitmap input of the last superstep of parity p
+  // (its workgroups' counts; the stream must be idle).
+  int64_t bitmapInputSize(int p) {
+    std::vector<int32_t> c(SCAN_BLOCKS);
+    check(gasrt_memcpy_d2h(c.data(), m_bcnt.p + (size_t)p * SCAN_BLOCKS,
+                           sizeof(int32_t) * SCAN_BLOCKS, m_stream),
+          "bitmap count");
+    int64_t t = 0;
+    for (int32_t x : c) t += x;
+    return t;
+  }
+
+  // Supersteps of k_small from the current list frontier, then the host's
+  // bookkeeping of the supersteps it ran (synchronises the stream).
+  void smallSteps(bool haveGather) {
+    refreshSourceValues();
+    m_dev.seq = (int32_t)m_seq;
+    m_dev.iter = m_iter;
+    m_dev.in_mode = IN_LIST;
+    m_dev.out_list = 1;
+    m_dev.first = 0;
+    m_dev.pub = 0;
+    const bool timed = kernel_timing_flag() != 0;
+    if (timed) timedMark(true);
+    if (haveGather)
+      launch_wg(k_small<Program, true>, 1, SMALL_NT, m_stream, m_dev, m_par,
+                1 << 20, m_small_max);
+    else
+      launch_wg(k_small<Program, false>, 1, SMALL_NT, m_stream, m_dev, m_par,
+                1 << 20, m_small_max);
+    if (timed) timedMark(false);
+    const Ctl c = readCtl();  // synchronises
+    const int k = c.small_steps;
+    m_seq += k;
+    m_iter += k;
+    m_supersteps += k;
+    if (k & 1) m_par ^= 1;
+    m_prev_in = IN_LIST;
+    m_in = IN_LIST;
+    m_first = true;
+    m_flags_dirty = true;
+    m_other_stale = true;
   }
 
   // gatherMap(v) of every vertex for source-only programs, after the state
@@ -703,6 +808,10 @@ class GASEngineGPU {
     // atomics even for one-vertex frontiers; above, sparse frontiers use
     // the list.
     m_switch = m_nV >= (1 << 24) ? 64 : 0;
+    // k_small's chain of dependent loads and claims per vertex costs more
+    // than a grid superstep once frontiers reach a few hundred vertices, so
+    // it is on by default only for graphs whose whole run stays that small
+    m_small_max = m_nV <= kSmallGraph ? 4096 : 0;
     buildHubs();
     setActive(0, 0);
   }
@@ -738,6 +847,24 @@ class GASEngineGPU {
       if (m_tpos >= 4096) collectTiming();
     }
   }
+  // Event brackets around a launch that does not go through klaunch.
+  void timedMark(bool begin) {
+    if (begin) {
+      if (m_tpos == m_timing.size()) {
+        gasrt_event_t a, b;
+        check(gasrt_event_create(&a), "event");
+        check(gasrt_event_create(&b), "event");
+        m_timing.push_back({a, b});
+        m_tkind.push_back(0);
+      }
+      check(gasrt_event_record(m_timing[m_tpos].first, m_stream), "event");
+    } else {
+      check(gasrt_event_record(m_timing[m_tpos].second, m_stream), "event");
+      m_tkind[m_tpos] = GAS_K_SMALL * 4 + 2;
+      ++m_tpos;
+      if (m_tpos >= 4096) collectTiming();
+    }
+  }
   void collectTiming() {
     if (m_tpos == 0) return;
     check(gasrt_stream_synchronize(m_stream), "sync");
@@ -765,6 +892,8 @@ class GASEngineGPU {
   int m_out_list = 0;               // scatter of this superstep builds a list
   bool m_first = true;              // next launch is the superstep's first
   int m_switch = 0;                 // density switch divisor (0: bitmap only)
+  int m_small_max = 0;              // k_small frontier limit (0: off)
+  static constexpr int kSmallGraph = 8192;  // default k_small graphs: V <= this
   bool m_other_stale = true;        // vnext does not mirror vdata
   bool m_srcval_stale = true;       // srcval does not match the state
   bool m_flags_dirty = true;        // a scatter ran since the last clear

@@ -103,6 +103,9 @@ constexpr int STEP_COUNT_WORDS = sizeof(StepCounts) / 4;
 // Control block, one per engine, in device memory.
 struct Ctl {
   StepCounts ring[4];
+  // k_small's report: supersteps run, size of the list it left, and whether
+  // it stopped on an empty frontier
+  int32_t small_steps, small_n, small_empty, small_pad;
 };
 
 // How a superstep finds its active vertices.
@@ -1328,6 +1331,102 @@ __global__ __launch_bounds__(NT) void k_hscatter(Dev<P> d, int p) {
       }
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small-frontier mode for graphs whose degrees are all at most SMALL_DEG_MAX:
+// one workgroup of SMALL_NT threads runs consecutive supersteps from a list
+// frontier, with workgroup barriers between gather, apply and scatter (all
+// of one superstep's gathers finish before any of its applies, as
+// refgas.h:143-168 requires), while the frontier has at most small_max
+// vertices.  It stops when the frontier empties, outgrows small_max, or
+// after max_steps supersteps, leaving a list frontier (q, activation bytes,
+// counter ring) exactly as a superstep of the grid kernels would, and
+// reports what it did in the control block.  One workgroup means one CU and
+// one L2: the barriers are cheap and no cross-XCD fence is needed.
+constexpr int SMALL_NT = 1024;
+constexpr int SMALL_DEG_MAX = 64;
+template <class P, bool HaveGather>
+__global__ __launch_bounds__(SMALL_NT) void k_small(Dev<P> d, int p,
+                                                    int max_steps,
+                                                    int small_max) {
+  typedef typename P::GatherResult GR;
+  typedef typename P::VertexData VD;
+  __shared__ int s_n, s_next;
+  Ctl *c = d.ctl;
+  if (threadIdx.x == 0) {
+    s_n = c->ring[d.seq & 3].n_active;
+    s_next = 0;
+  }
+  __syncthreads();
+  int cur = p, steps = 0, n = s_n;
+  int64_t gathered = 0, scattered = 0, active = 0;
+  while (n > 0 && n <= small_max && steps < max_steps) {
+    const int32_t *q = d.q[cur];
+    int32_t *qn = d.q[cur ^ 1];
+    uint8_t *fn = d.fl[cur ^ 1];
+    if (HaveGather) {
+      for (int i = threadIdx.x; i < n; i += SMALL_NT) {
+        const int v = q[i];
+        const int o = d.csc_off[v], o1 = d.csc_off[v + 1];
+        gathered += o1 - o;
+        Acc<P> acc = Acc<P>::empty();
+        for (int e = o; e < o1; ++e) acc.add(gather_value(d, v, d.csc_src[e], e));
+        d.gtmp[i] = acc.finish();
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) s_superstep = d.iter + steps;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += SMALL_NT) {
+      const int v = q[i];
+      d.fl[cur][v] = 0;
+      ++active;
+      const GR g = HaveGather ? d.gtmp[i] : P::gatherZero;
+      VD x = d.vdata[v];
+      const bool ret = P::apply(&x, g);
+      d.vdata[v] = x;
+      if constexpr (is_source_only<P>::value)
+        d.srcval[v] = P::gatherMap(&x, &x, nullptr);
+      if (ret) {
+        const int o = d.csr_off[v], o1 = d.csr_off[v + 1];
+        scattered += o1 - o;
+        for (int e = o; e < o1; ++e) {
+          const int u = d.csr_dst[e];
+          if (claim_vertex(fn, u)) qn[atomicAdd(&s_next, 1)] = u;
+        }
+      }
+    }
+    __syncthreads();
+    n = s_next;
+    __syncthreads();
+    if (threadIdx.x == 0) s_next = 0;
+    cur ^= 1;
+    ++steps;
+  }
+  // statistics of this workgroup's threads into slot 0
+  __shared__ unsigned long long s_stat[3];
+  if (threadIdx.x < 3) s_stat[threadIdx.x] = 0;
+  __syncthreads();
+  if (gathered) atomicAdd(&s_stat[0], (unsigned long long)gathered);
+  if (scattered) atomicAdd(&s_stat[1], (unsigned long long)scattered);
+  if (active) atomicAdd(&s_stat[2], (unsigned long long)active);
+  __syncthreads();
+  if (threadIdx.x < 3 && s_stat[threadIdx.x])
+    atomicAdd(reinterpret_cast<unsigned long long *>(d.stat_acc +
+                                                     threadIdx.x * SCAN_BLOCKS),
+              s_stat[threadIdx.x]);
+  // the frontier left for the grid kernels: superstep seq + steps reads
+  // ring entry (seq + steps) & 3; the other entries start cleared
+  if (threadIdx.x < 4 * STEP_COUNT_WORDS)
+    reinterpret_cast<int32_t *>(c->ring)[threadIdx.x] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    c->ring[(d.seq + steps) & 3].n_active = n;
+    c->small_steps = steps;
+    c->small_n = n;
+    c->small_empty = n == 0;
   }
 }
 

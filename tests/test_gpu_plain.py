@@ -260,3 +260,35 @@ def test_frontier_modes(gpu, divisor):
     ref, ost = oracle.cc(n, s, d)
     assert np.array_equal(lab, ref)
     assert st.supersteps == ost.supersteps
+
+
+@pytest.mark.parametrize("small_max", [64, 4096])
+def test_small_frontier_mode(gpu, small_max):
+    # supersteps inside one workgroup while the list frontier is small: the
+    # road graph's frontiers outgrow the limit and shrink back (hand-back to
+    # the grid kernels and return to the workgroup), the grid's stay small.
+    # Results and superstep counts against the oracle, run statistics against
+    # the grid kernels alone.
+    for el in (workloads.road(128, 128, seed=9), workloads.grid64()):
+        s, d, w = el.arrays()
+        n = el.n_vertices
+        g = gas.Graph(el)
+        for src in (0, n // 2 + 17, n - 1):
+            for prog, run, ref in ((gas.PROGRAM_BFS, lambda: g.bfs(src),
+                                    lambda: oracle.bfs(n, s, d, src)),
+                                   (gas.PROGRAM_SSSP, lambda: g.sssp(src),
+                                    lambda: oracle.sssp(n, s, d, w, src))):
+                g.set_small_frontier(prog, 0)
+                _, st0 = run()
+                g.set_small_frontier(prog, small_max)
+                out, st = run()
+                want, ost = ref()
+                assert np.array_equal(out, want)
+                assert st.supersteps == ost.supersteps
+                for f in ("active_total", "gathered_edges", "scattered_edges"):
+                    assert getattr(st, f) == getattr(st0, f), f
+        g.set_small_frontier(gas.PROGRAM_CC, small_max)
+        lab, st = g.cc()
+        ref, ost = oracle.cc(n, s, d)
+        assert np.array_equal(lab, ref)
+        assert st.supersteps == ost.supersteps

@@ -65,7 +65,7 @@ class RunStats(C.Structure):
 
 KERNEL_NAMES = ["k_gather_vp", "k_apply", "k_scatter_vp", "k_hscan1", "k_hscan2",
                 "k_hgather", "k_hfix", "k_hscatter", "k_compact1", "k_compact2",
-                "k_fused"]
+                "k_fused", "k_small"]
 
 
 _lib = None
@@ -292,6 +292,12 @@ class Graph:
         """List frontier while |A| * divisor < V, bitmap otherwise."""
         _check(lib().gas_graph_set_frontier_switch(self._h, C.c_int(program),
                                                    C.c_int32(divisor)))
+
+    def set_small_frontier(self, program: int, max_vertices: int) -> None:
+        """Runs supersteps of list frontiers of at most `max_vertices` inside
+        one workgroup on graphs of degree <= 64 (0: off)."""
+        _check(lib().gas_graph_set_small_frontier(self._h, C.c_int(program),
+                                                  C.c_int32(max_vertices)))
 
     def autotune(self, program: int):
         """Times candidate degree tiers for `program` on this graph and keeps

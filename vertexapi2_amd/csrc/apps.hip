@@ -218,6 +218,14 @@ int gas_graph_set_frontier_switch(gas_graph_t g, int program,
                      [&](auto &e) { e.setSwitchDivisor(divisor); });
 }
 
+int gas_graph_set_small_frontier(gas_graph_t g, int program,
+                                 int32_t max_vertices) {
+  if (max_vertices < 0)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_small_frontier: need >= 0");
+  return with_engine(g, program,
+                     [&](auto &e) { e.setSmallFrontier(max_vertices); });
+}
+
 int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers) {
   if (!tiers) return set_error(GAS_ERR_ARG, "gas_graph_get_tiers: null");
   return with_engine(g, program, [&](auto &e) {
