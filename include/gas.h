@@ -326,6 +326,12 @@ int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out);
  * out-edges times `divisor` reach |E| (default 0 = never pull: with 64
  * lanes a pull rarely stops early, measured slower on every R-MAT round). */
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
+/* Batched SSSP: in round r a lane relaxes the out-edges of a changed vertex
+ * only while its distance there is below (r - r0 + 1) * delta (r0: the
+ * lane's first round); changed lanes above the bound wait.  Fewer
+ * re-relaxations than Bellman-Ford rounds, same (unique) distances.
+ * 0 = no bound. */
+int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta);
 int gas_batch_free(gas_batch_t b);
 
 /* ---- continuous-batching query server -------------------------------------

@@ -70,7 +70,7 @@ struct BatchDev {
   uint64_t *front[2];                   // BFS frontier masks by parity
   uint8_t *depth8;                      // BFS [BW][rowV]
   int32_t rowV;                         // depth row stride (V rounded to 4)
-  int32_t *lane_start;                  // BFS [BW]: round the lane began
+  int32_t *lane_start;                  // [BW]: round the lane began
   int32_t *dist;                        // SSSP [V][BW]
   uint64_t *chg, *chg_next;             // SSSP
   int32_t *list;                        // current list (slab order)
@@ -85,7 +85,20 @@ struct BatchDev {
   uint64_t *lor;                        // [SCAN_BLOCKS] OR of masks
   int32_t pull_div;                     // pull when list_edges*div >= E
                                         // (0: never, SSSP)
+  int32_t sssp_delta;                   // SSSP distance bound step (0: none)
+  int32_t round;                        // round of this launch
 };
+
+// SSSP: a lane relaxes from a vertex in a round only while its distance
+// there is below (rounds since the lane began) * delta; changed lanes above
+// the bound stay marked for a later round.  delta = 0: no bound
+// (Bellman-Ford rounds).
+__device__ inline int32_t sssp_bound(const BatchDev &d, int lane) {
+  if (d.sssp_delta <= 0) return INT32_MAX;
+  const int64_t b =
+      (int64_t)(d.round + 1 - d.lane_start[lane]) * (int64_t)d.sssp_delta;
+  return b > INT32_MAX ? INT32_MAX : (int32_t)b;
+}
 
 __device__ inline uint64_t wave_or(uint64_t x) {
 #pragma unroll
@@ -520,6 +533,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   const int n_items = c->n_items;
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
+  const int32_t bound = sssp_bound(d, lane);
   constexpr int G = SSSP_ITEMS_PER_TRIP;
   for (int base = (blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE) * G;
        base < n_items; base += waves * G) {
@@ -536,8 +550,14 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
     int du[G], my_w[G], my_wt[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      act[g] = ne[g] > 0 && ((d.chg[u[g]] >> lane) & 1ull);
+      const bool changed = ne[g] > 0 && ((d.chg[u[g]] >> lane) & 1ull);
       du[g] = ne[g] > 0 ? d.dist[(size_t)u[g] * BW + lane] : SSSP_INF;
+      act[g] = changed && du[g] < bound;
+      // changed lanes above the bound wait for a later round
+      const uint64_t wait = __ballot(changed && !act[g]);
+      if (wait && lane == 0)
+        atomicOr((unsigned long long *)&d.chg_next[u[g]],
+                 (unsigned long long)wait);
       my_w[g] = 0;
       my_wt[g] = 0;
       if (lane < ne[g]) {
@@ -708,18 +728,15 @@ class BatchEngine {
       m_front0.alloc(V);
       m_front1.alloc(V);
       m_depth.alloc((size_t)d.rowV * BW);
-      m_lane_start.alloc(BW);
       check(gasrt_memset(m_vn.p, 0, 2 * V * 8, m_stream), "memset");
       check(gasrt_memset(m_front0.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_front1.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW, m_stream),
             "memset");
-      check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
       d.vn = m_vn.p;
       d.front[0] = m_front0.p;
       d.front[1] = m_front1.p;
       d.depth8 = m_depth.p;
-      d.lane_start = m_lane_start.p;
     } else {
       m_dist.alloc(V * BW);
       m_chg.alloc(V);
@@ -730,6 +747,10 @@ class BatchEngine {
       d.chg = m_chg.p;
       d.chg_next = m_chg_next.p;
     }
+    m_lane_start.alloc(BW);
+    check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
+    d.lane_start = m_lane_start.p;
+    d.sssp_delta = 0;
     d.ctl = m_ctl.p;
     d.list = m_list.p;
     d.item = m_item.p;
@@ -765,6 +786,10 @@ class BatchEngine {
   void setPullDivisor(int div) {
     if (m_kind == BFS_KIND) m_dev.pull_div = div;
   }
+  // SSSP distance bound step (0: Bellman-Ford rounds).
+  void setSsspDelta(int delta) {
+    if (m_kind == SSSP_KIND) m_dev.sssp_delta = delta;
+  }
 
   // Starts queries: lane[i] runs from source[i].  The lanes must be free.
   void addLanes(const int32_t *lanes, const int32_t *sources, int n) {
@@ -783,11 +808,11 @@ class BatchEngine {
     check(gasrt_memcpy_h2d(m_sources.p, sources, 4 * (size_t)n, m_stream),
           "sources");
     const unsigned grid = gridFor(m_nV);
+    for (int i = 0; i < n; ++i) m_hstart[lanes[i]] = m_round;
+    check(gasrt_memcpy_h2d(m_lane_start.p, m_hstart, sizeof(m_hstart),
+                           m_stream),
+          "lane start");
     if (m_kind == BFS_KIND) {
-      for (int i = 0; i < n; ++i) m_hstart[lanes[i]] = m_round;
-      check(gasrt_memcpy_h2d(m_lane_start.p, m_hstart, sizeof(m_hstart),
-                             m_stream),
-            "lane start");
       launch(kb_bfs_reset, grid, m_stream, m_dev, m_par, reset);
       launch(kb_bfs_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
       m_used |= reset;
@@ -826,6 +851,7 @@ class BatchEngine {
              (const uint64_t *)m_dev.front[m_par]);
       launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
+      m_dev.round = m_round;
       launch(kb_sssp_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,

@@ -178,3 +178,25 @@ def test_ppr_alone_equals_mixed_bitwise(gpu):
         if not (b.alive() >> 11) & 1:
             break
     assert np.array_equal(b.result(11), alone)
+
+
+@pytest.mark.parametrize("delta", [1, 16, 255])
+def test_batch_sssp_distance_bound(gpu, delta):
+    # a lane relaxes only below (rounds since it began) * delta; lanes that
+    # join a running batch start their own bound; distances stay exact
+    for el in (workloads.road(128, 128, seed=5),
+               workloads.rmat(12, symmetrize=True, weights=True)):
+        s, d, w = el.arrays()
+        n = el.n_vertices
+        g = gas.Graph(el)
+        src = _sources(np.random.default_rng(delta), el, 20)
+        b = gas.Batch(g, gas.BATCH_SSSP)
+        b.set_sssp_delta(delta)
+        b.add(np.arange(16), src[:16])
+        b.step(4)
+        b.add([16, 17, 18, 19], src[16:20])
+        b.run()
+        res = b.results(range(20))
+        for lane in range(20):
+            want = oracle.check_dijkstra(n, s, d, w, int(src[lane]))
+            assert np.array_equal(res[lane], want), lane

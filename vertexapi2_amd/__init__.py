@@ -504,6 +504,11 @@ class Batch:
     def set_pull_divisor(self, divisor: int) -> None:
         _check(lib().gas_batch_set_pull_divisor(self._h, C.c_int32(divisor)))
 
+    def set_sssp_delta(self, delta: int) -> None:
+        """SSSP lanes relax a vertex only while its distance is below
+        (rounds since the lane began) * delta (0: no bound)."""
+        _check(lib().gas_batch_set_sssp_delta(self._h, C.c_int32(delta)))
+
 
 class ServeStats(C.Structure):
     _fields_ = [("elapsed_s", C.c_double), ("rounds", C.c_int64),

@@ -190,6 +190,14 @@ int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor) {
   return GAS_OK;
 }
 
+int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta) {
+  BATCH_OR_FAIL(b);
+  if (delta < 0)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_sssp_delta: need >= 0");
+  if (b->e) b->e->setSsspDelta(delta);
+  return GAS_OK;
+}
+
 int gas_batch_free(gas_batch_t b) {
   delete b;
   return GAS_OK;
