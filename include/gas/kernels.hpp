@@ -867,6 +867,10 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
   __shared__ int32_t s_vtx[HUB_SEG_MAX];
   __shared__ int32_t s_slot[HUB_SEG_MAX];
   __shared__ SegItem<P> s_wtot[NT / WAVE];
+  constexpr int PADDED = MP_NV + MP_NV / 32;
+  __shared__ int32_t s_e[PADDED];
+  __shared__ int32_t s_k[is_source_only<P>::value ? 1 : PADDED];
+  __shared__ GR s_val[PADDED];
 
   if (frontier_size(d) == 0) return;
   const bool all = d.in_mode == IN_ALL;
@@ -894,16 +898,17 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
     __syncthreads();
     const bool first_fresh = (h.off[s0] + s0 == d0);
 
-    // decode this thread's items
+    // decode this thread's consecutive items; edge index (or -1 for a
+    // segment end) into LDS, padded one word per 32 against bank conflicts
     const int p0 = d0 + tid * MP_VT;
     int k = (p0 < d1) ? lds_lower_bound(s_end, nseg, p0) : 0;
-    int kind[MP_VT], kk[MP_VT], eidx[MP_VT];
+    int kind[MP_VT], kk[MP_VT];
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
       const int pos = p0 + it;
       kind[it] = 0;
       kk[it] = 0;
-      eidx[it] = 0;
+      int e = -1;
       if (pos < d1) {
         kk[it] = k;
         if (s_end[k] == pos) {
@@ -911,21 +916,38 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
           ++k;
         } else {
           kind[it] = 1;
-          eidx[it] = s_base[k] + (pos - s0 - k);
+          e = s_base[k] + (pos - s0 - k);
         }
       }
+      const int j = tid * MP_VT + it;
+      s_e[j + (j >> 5)] = e;
+      if constexpr (!is_source_only<P>::value) s_k[j + (j >> 5)] = kk[it];
     }
-    int src[MP_VT];
+    __syncthreads();
+    // striped: lanes of a wave take consecutive items, so the source ids
+    // load coalesced and neighbouring edges' gathers share cache lines
+    int ee[MP_VT], src[MP_VT];
 #pragma unroll
-    for (int it = 0; it < MP_VT; ++it)
-      src[it] = (kind[it] == 1) ? d.csc_src[eidx[it]] : 0;
+    for (int it = 0; it < MP_VT; ++it) {
+      const int j = it * NT + tid;
+      ee[it] = s_e[j + (j >> 5)];
+      src[it] = ee[it] >= 0 ? d.csc_src[ee[it]] : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < MP_VT; ++it) {
+      const int j = it * NT + tid;
+      if (ee[it] >= 0) {
+        int dst = 0;
+        if constexpr (!is_source_only<P>::value) dst = s_vtx[s_k[j + (j >> 5)]];
+        s_val[j + (j >> 5)] = gather_value(d, dst, src[it], ee[it]);
+      }
+    }
+    __syncthreads();
     GR val[MP_VT];
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
-      if (kind[it] == 1)
-        val[it] = gather_value(d, s_vtx[kk[it]], src[it], eidx[it]);
-      else
-        val[it] = GR();
+      const int j = tid * MP_VT + it;
+      val[it] = kind[it] == 1 ? s_val[j + (j >> 5)] : GR();
     }
     // thread-serial segmented reduce
     Acc<P> acc = Acc<P>::empty(), head = Acc<P>::empty();
