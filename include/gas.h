@@ -366,6 +366,12 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
                       const int32_t *graph_ids, const int32_t *sources,
                       const double *arrival_s, void *const *host_results,
                       double *latency_ms, gas_serve_stats *stats);
+/* Batch admission under load: while at least `backlog` queries wait for a
+ * BFS or SSSP engine, they join its running batch only within `rounds`
+ * rounds of the batch's start and otherwise wait for it to drain, so a
+ * batch's lanes move level by level together and share edge reads.
+ * backlog 0 = always admit into free lanes.  Default 16 and 1. */
+int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds);
 int gas_server_free(gas_server_t s);
 
 /* ---- the engine API, step by step --------------------------------------- */

@@ -12,13 +12,17 @@ from vertexapi2_amd import serve, workloads
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_alone(gpu):
+@pytest.mark.parametrize("gang", [(16, 1), (0, 0), (1, 0)])
+def test_served_results_equal_alone(gpu, gang):
+    # gang (16, 1): the default; (0, 0): always join a running batch;
+    # (1, 0): any waiting query holds until the running batch drains
     road = gas.Graph(workloads.road(96, 96, seed=2))
     rmat = gas.Graph(workloads.rmat(12, symmetrize=True, weights=True))
     graphs = [road, rmat]
     edges = [workloads.road(96, 96, seed=2), workloads.rmat(12, symmetrize=True, weights=True)]
-    stream = serve.make_stream(rate_qps=3000, n=150, graphs=graphs, seed=11)
+    stream = serve.make_stream(rate_qps=30000, n=150, graphs=graphs, seed=11)
     srv = gas.Server(graphs, memory_budget=1 << 34)
+    srv.set_gang(*gang)
     keep = list(range(0, 150, 3))
     out, lat, results = serve.serve(srv, stream, keep=keep)
     assert out["queries"] == 150 and np.all(lat > 0)

@@ -542,6 +542,11 @@ class Server:
             lib().gas_server_free(self._h)
             self._h = None
 
+    def set_gang(self, backlog: int, rounds: int) -> None:
+        """While `backlog` queries wait for a BFS/SSSP engine, they join its
+        batch only within `rounds` rounds of its start (0: always join)."""
+        _check(lib().gas_server_set_gang(self._h, C.c_int32(backlog), C.c_int32(rounds)))
+
     def replay(self, kinds, graph_ids, sources, arrivals_s, keep=None):
         """Serves the stream in real time; returns (latency_ms, results, stats)
         where results maps query index -> array for the indices in `keep`."""

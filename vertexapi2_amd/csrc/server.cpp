@@ -15,6 +15,10 @@
 // into per-lane buffers in HBM in one pass (and optionally copied to the
 // caller), the lanes and their memory are released, and waiting queries can
 // take them.
+// Under a backlog, BFS / SSSP queries join a running batch only during its
+// first rounds and otherwise wait for it to drain, so the lanes of a batch
+// traverse the graph level by level together and share its edge reads
+// (lanes started rounds apart each pay for their own dense levels).
 // A query's latency is the time from its arrival to its result being in HBM.
 #include <hip/hip_runtime.h>
 
@@ -60,6 +64,7 @@ size_t query_bytes(int kind, int64_t V) {
 
 struct Engine {
   int kind = 0, graph = 0;
+  int gang_start = 0;               // round at which the current batch began
   int64_t V = 0;
   std::unique_ptr<BatchEngine> be;
   std::unique_ptr<PprBatch> pe;
@@ -79,6 +84,7 @@ struct gas_server {
   std::vector<int64_t> nV;
   size_t budget = 0, reserved = 0;
   int ppr_iterations = 10;
+  int gang_backlog = 16, gang_rounds = 1;        // gas_server_set_gang
   std::vector<std::unique_ptr<Engine>> engines;  // [kind * n_graphs + graph]
   std::vector<double> solo_ms;                   // plain-engine cost per key
   bool calibrated = false;
@@ -151,6 +157,14 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
     s->solo_out.resize((size_t)n_graphs);
     *out = s.release();
   });
+}
+
+int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds) {
+  if (!s || backlog < 0 || rounds < 0)
+    return set_error(GAS_ERR_ARG, "gas_server_set_gang: bad arguments");
+  s->gang_backlog = backlog;
+  s->gang_rounds = rounds;
+  return GAS_OK;
 }
 
 int gas_server_free(gas_server_t s) {
@@ -303,11 +317,14 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
           break;
         }
       }
-      bool blocked[3 * 64] = {};
+      std::vector<char> blocked(s->engines.size(), 0);
+      std::vector<int> backlog(s->engines.size(), 0);
+      for (int q : waiting)
+        ++backlog[(size_t)kinds[q] * s->graphs.size() + graph_ids[q]];
       for (auto it = waiting.begin(); it != waiting.end();) {
         const int q = *it;
-        const int key = kinds[q] * (int)s->graphs.size() + graph_ids[q];
-        if (key < 3 * 64 && blocked[key]) {
+        const size_t key = (size_t)kinds[q] * s->graphs.size() + graph_ids[q];
+        if (blocked[key]) {
           ++it;
           continue;
         }
@@ -315,12 +332,20 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
         const size_t need = query_bytes(kinds[q], s->nV[graph_ids[q]]);
         uint64_t busy = e.be ? e.be->lanesUsed() : e.pe->lanesUsed();
         for (int32_t l : e.add_lanes) busy |= 1ull << l;
-        if (e.inflight || busy == ~0ull || s->reserved + need > s->budget) {
-          if (key < 3 * 64) blocked[key] = true;
+        // a backlog waits for a BFS / SSSP batch past its first rounds to
+        // drain (PPR lanes cost the same dense sweeps at any offset)
+        const bool hold = e.be && e.running && s->gang_backlog > 0 &&
+                          backlog[key] >= s->gang_backlog &&
+                          e.be->round() - e.gang_start > s->gang_rounds;
+        if (e.inflight || busy == ~0ull || hold ||
+            s->reserved + need > s->budget) {
+          blocked[key] = 1;
           if (!e.inflight) ++admissions_deferred;
           ++it;
           continue;
         }
+        if (e.be && !e.running && e.add_lanes.empty())
+          e.gang_start = e.be->round();
         const int lane = __builtin_ctzll(~busy);
         e.add_lanes.push_back(lane);
         e.add_src.push_back(sources[q]);
