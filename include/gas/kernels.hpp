@@ -43,6 +43,10 @@
 //   k_compact1/2         ordered compaction of activation bytes into a list
 //                        (the step-by-step API, whose next frontier is in
 //                        ascending vertex order as refgas.h:201-210)
+//   k_fused              gather + apply + scatter of a superstep in one
+//                        launch (no gather hubs, all / bitmap frontier)
+//   k_small              consecutive supersteps of a small list frontier
+//                        in one workgroup (graphs of degree <= 64)
 //
 // The first kernel of a superstep reports a frontier size to a host-mapped
 // status word; the host reads it a few supersteps behind and stops once a
