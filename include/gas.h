@@ -285,7 +285,7 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
  * per engine, bfs.cu:91-112, sssp.cu:67-82).  Lanes can be added between
  * rounds while others are running (continuous batching); a lane is finished
  * when its bit leaves gas_batch_alive.  Results per lane are identical to
- * the single-source runs: BFS depth (-1 unreached, at most 254 levels),
+ * the single-source runs: BFS depth (-1 unreached, at most 65534 levels),
  * SSSP distance (2147383647 unreached). */
 #define GAS_BATCH_BFS   0
 #define GAS_BATCH_SSSP  1

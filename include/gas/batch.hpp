@@ -3,8 +3,8 @@
 // wave64 is exactly one vertex's row of lanes.
 //
 //   BFS   per-vertex uint64 masks: visited, frontier (by parity), next;
-//         depth stored as one byte per (lane, vertex) (255 = unreached),
-//         levels up to 254.  A round is either a push from the frontier list
+//         depth stored as 16 bits per (lane, vertex) (0xffff = unreached),
+//         levels up to 65534.  A round is either a push from the frontier list
 //         (items of up to 16 out-edges carrying the frontier mask, 4 items
 //         per wave and 8 in flight, one lane per edge, 64-bit atomicOr of the
 //         new bits) or a pull over every
@@ -68,7 +68,7 @@ struct BatchDev {
   uint64_t *vn;                         // BFS [V][2]: visited, next bits
                                         // (one 16-byte pair per vertex)
   uint64_t *front[2];                   // BFS frontier masks by parity
-  uint8_t *depth8;                      // BFS [BW][rowV]
+  uint16_t *depth16;                    // BFS [BW][rowV]
   int32_t rowV;                         // depth row stride (V rounded to 4)
   int32_t *lane_start;                  // [BW]: round the lane began
   int32_t *dist;                        // SSSP [V][BW]
@@ -305,9 +305,9 @@ static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
     d.vn[2 * (size_t)v] &= ~reset;
     d.front[p][v] &= ~reset;
   }
-  // depth rows of the reset lanes, 4 bytes per thread
-  uint32_t *rows = reinterpret_cast<uint32_t *>(d.depth8);
-  const int wpr = d.rowV / 4;
+  // depth rows of the reset lanes, two depths per thread
+  uint32_t *rows = reinterpret_cast<uint32_t *>(d.depth16);
+  const int wpr = d.rowV / 2;
   for (int s = 0; s < BW; ++s) {
     if (!((reset >> s) & 1ull)) continue;
     for (int i = blockIdx.x * NT + threadIdx.x; i < wpr; i += stride)
@@ -324,7 +324,7 @@ static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
   const uint64_t bit = 1ull << s;
   atomicOr((unsigned long long *)&d.vn[2 * (size_t)v], (unsigned long long)bit);
   atomicOr((unsigned long long *)&d.front[p][v], (unsigned long long)bit);
-  d.depth8[(size_t)s * d.rowV + v] = 0;
+  d.depth16[(size_t)s * d.rowV + v] = 0;
 }
 
 // Push: a wave takes BFS_ITEMS_PER_TRIP items at a time, 16 lanes per item
@@ -465,7 +465,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
       while (x) {
         const int s = __ffsll((unsigned long long)x) - 1;
         x &= x - 1;
-        d.depth8[(size_t)s * d.rowV + v] = (uint8_t)(round + 1 - s_start[s]);
+        d.depth16[(size_t)s * d.rowV + v] = (uint16_t)(round + 1 - s_start[s]);
       }
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
@@ -493,10 +493,10 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
 // Depth of lane s as int32 (-1 unreached), for the caller.
 static __global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
                                                      int32_t *out) {
-  const uint8_t *row = d.depth8 + (size_t)s * d.rowV;
+  const uint16_t *row = d.depth16 + (size_t)s * d.rowV;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
-    const uint8_t x = row[v];
-    out[v] = x == 255 ? -1 : (int32_t)x;
+    const uint16_t x = row[v];
+    out[v] = x == 0xffff ? -1 : (int32_t)x;
   }
 }
 
@@ -655,8 +655,8 @@ static __global__ __launch_bounds__(NT) void kb_bfs_extract_many(
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
     for (uint64_t m = lanes; m; m &= m - 1) {
       const int s = __ffsll((unsigned long long)m) - 1;
-      const uint8_t x = d.depth8[(size_t)s * d.rowV + v];
-      outs.p[s][v] = x == 255 ? -1 : (int32_t)x;
+      const uint16_t x = d.depth16[(size_t)s * d.rowV + v];
+      outs.p[s][v] = x == 0xffff ? -1 : (int32_t)x;
     }
   }
 }
@@ -731,12 +731,12 @@ class BatchEngine {
       check(gasrt_memset(m_vn.p, 0, 2 * V * 8, m_stream), "memset");
       check(gasrt_memset(m_front0.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_front1.p, 0, V * 8, m_stream), "memset");
-      check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW, m_stream),
+      check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW * 2, m_stream),
             "memset");
       d.vn = m_vn.p;
       d.front[0] = m_front0.p;
       d.front[1] = m_front1.p;
-      d.depth8 = m_depth.p;
+      d.depth16 = m_depth.p;
     } else {
       m_dist.alloc(V * BW);
       m_chg.alloc(V);
@@ -840,8 +840,8 @@ class BatchEngine {
   void step() {
     if (m_kind == BFS_KIND) {
       for (int s = 0; s < BW; ++s)
-        if (((m_used >> s) & 1ull) && m_round + 1 - m_hstart[s] > 254)
-          throw Error(GAS_ERR_STATE, "batched BFS deeper than 254 levels");
+        if (((m_used >> s) & 1ull) && m_round + 1 - m_hstart[s] > 65534)
+          throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
       launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used,
@@ -965,7 +965,7 @@ class BatchEngine {
   DevBuf<uint64_t> m_lor, m_vn, m_front0, m_front1, m_chg, m_chg_next,
       m_hub_mask;
   DevBuf<int4> m_hub;
-  DevBuf<uint8_t> m_depth;
+  DevBuf<uint16_t> m_depth;
 };
 
 // ---- personalised PageRank, 64 FP32 lanes ----------------------------------

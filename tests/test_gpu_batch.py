@@ -200,3 +200,22 @@ def test_batch_sssp_distance_bound(gpu, delta):
         for lane in range(20):
             want = oracle.check_dijkstra(n, s, d, w, int(src[lane]))
             assert np.array_equal(res[lane], want), lane
+
+
+def test_batch_bfs_deep_levels(gpu):
+    # a 2 x 700 grid: depths up to ~700, past what one byte per lane holds
+    el = gas.Edges.grid(2, 700)
+    s, d, _ = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    b = gas.Batch(g, gas.BATCH_BFS)
+    src = np.array([0, 699, 700, 350], np.int32)
+    b.add(np.arange(4), src)
+    b.step(3)
+    b.add([4], [1399])  # joins three rounds late
+    b.run()
+    res = b.results(range(5))
+    for lane, v in enumerate([0, 699, 700, 350, 1399]):
+        want = oracle.check_bfs(n, s, d, v)
+        assert want.max() > 254 or v == 350
+        assert np.array_equal(res[lane], want), lane

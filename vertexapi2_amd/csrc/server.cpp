@@ -56,7 +56,7 @@ constexpr double kSoloMaxMs = 2.0;
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
   switch (kind) {
-    case GAS_BATCH_BFS: return (size_t)(V * 1 + V * 3 / 8 + V * 4);
+    case GAS_BATCH_BFS: return (size_t)(V * 2 + V * 3 / 8 + V * 4);
     case GAS_BATCH_SSSP: return (size_t)(V * 4 + V / 4 + V * 4);
     default: return (size_t)(V * 16 + V / 4 + V * 4);  // PPR
   }
