@@ -714,6 +714,15 @@ class BatchEngine {
     d.csr_w = gv.csr_weights;
     d.pull_div = 0;  // push only: see setPullDivisor
     d.item_edges = kind == BFS_KIND ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
+    {
+      // kb_hub_items runs only where some vertex can have > HUB_ITEMS items
+      std::vector<int32_t> off((size_t)gv.n_vertices + 1);
+      check(gas_graph_host_csr(g, off.data(), nullptr, nullptr), "csr");
+      int64_t max_out = 0;
+      for (int32_t v = 0; v < gv.n_vertices; ++v)
+        max_out = std::max<int64_t>(max_out, off[v + 1] - off[v]);
+      m_may_hub = max_out > (int64_t)HUB_ITEMS * d.item_edges;
+    }
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
     m_item.alloc((size_t)gv.n_edges / d.item_edges + V + 1);
@@ -820,7 +829,7 @@ class BatchEngine {
              (const uint64_t *)m_dev.front[m_par]);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.front[m_par]);
-      launch(kb_hub_items, 256, m_stream, m_dev);
+      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       launch(kb_sssp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, reset);
       launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n);
@@ -829,7 +838,7 @@ class BatchEngine {
              (const uint64_t *)m_dev.chg);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg);
-      launch(kb_hub_items, 256, m_stream, m_dev);
+      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     }
   }
 
@@ -843,20 +852,21 @@ class BatchEngine {
         if (((m_used >> s) & 1ull) && m_round + 1 - m_hstart[s] > 65534)
           throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
       launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
-      launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
+      if (m_dev.pull_div > 0)
+        launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used,
              m_round);
       m_par ^= 1;
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.front[m_par]);
-      launch(kb_hub_items, 256, m_stream, m_dev);
+      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       m_dev.round = m_round;
       launch(kb_sssp_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg);
-      launch(kb_hub_items, 256, m_stream, m_dev);
+      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     }
     ++m_round;
   }
@@ -966,6 +976,7 @@ class BatchEngine {
       m_hub_mask;
   DevBuf<int4> m_hub;
   DevBuf<uint16_t> m_depth;
+  bool m_may_hub = true;  // some vertex has more than HUB_ITEMS items
 };
 
 // ---- personalised PageRank, 64 FP32 lanes ----------------------------------
