@@ -133,6 +133,46 @@ __device__ inline int64_t block_sum64(int64_t x, int64_t *lds) {
   return r;
 }
 
+// Exclusive workgroup scans of two counters at once (one LDS exchange):
+// a (int) and b (int64); totals in ta / tb.
+__device__ inline void block_excl_scan2(int a, int64_t b, int &pa, int64_t &pb,
+                                        int &ta, int64_t &tb, int *lds,
+                                        int64_t *lds64) {
+  const int lane = lane_id(), w = threadIdx.x / WAVE;
+  int ia = a;
+  int64_t ib = b;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int ya = __shfl_up(ia, o, WAVE);
+    const int64_t yb = __shfl_up(ib, o, WAVE);
+    if (lane >= o) {
+      ia += ya;
+      ib += yb;
+    }
+  }
+  if (lane == WAVE - 1) {
+    lds[w] = ia;
+    lds64[w] = ib;
+  }
+  __syncthreads();
+  int qa = 0, sa = 0;
+  int64_t qb = 0, sb = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) {
+    const int xa = lds[i];
+    const int64_t xb = lds64[i];
+    qa += (i < w) ? xa : 0;
+    qb += (i < w) ? xb : 0;
+    sa += xa;
+    sb += xb;
+  }
+  __syncthreads();
+  pa = qa + ia - a;
+  pb = qb + ib - b;
+  ta = sa;
+  tb = sb;
+}
+
 // ---- list construction -------------------------------------------------------
 // Vertices of a workgroup in the list passes: 64-vertex slabs dealt
 // round-robin to the SCAN_BLOCKS workgroups, NT / 64 slabs per round, so
@@ -218,15 +258,22 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
       o1 = d.csr_off[v + 1];
       nit = (o1 - o + ie - 1) / ie;
     }
-    int tot, itot;
-    const int pos = carry + block_excl_scan(on ? 1 : 0, tot, lds);
-    const int iex = block_excl_scan(nit, itot, lds);
+    // one scan for four counters: list slots | hub slots (16 bits each,
+    // at most NT per round) and items | non-hub items (32 bits each)
+    const bool hub = nit > HUB_ITEMS;
+    int pk, tk;
+    int64_t pi, ti;
+    block_excl_scan2((on ? 1 : 0) | (hub ? 1 << 16 : 0),
+                     ((int64_t)nit << 32) | (int64_t)(hub ? 0 : nit), pk, pi, tk,
+                     ti, lds, lds64);
+    const int tot = tk & 0xffff, htot = tk >> 16;
+    const int itot = (int)(ti >> 32), jtot = (int)(ti & 0xffffffff);
+    const int pos = carry + (pk & 0xffff);
+    const int iex = (int)(pi >> 32), jex = (int)(pi & 0xffffffff);
     if (on) d.list[pos] = v;
     // the workgroup writes the round's non-hub items together: the j-th of
     // them belongs to the last thread whose exclusive count of non-hub
     // items (s_jpos) is <= j; hubs' items are left to kb_hub_items
-    int jtot;
-    const int jex = block_excl_scan(nit > HUB_ITEMS ? 0 : nit, jtot, lds);
     s_ipos[threadIdx.x] = iex;
     s_jpos[threadIdx.x] = jex;
     s_v[threadIdx.x] = v;
@@ -234,8 +281,7 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
     s_o1[threadIdx.x] = o1;
     s_m[threadIdx.x] = mv;
     if (threadIdx.x == 0) s_jpos[NT] = jtot;
-    int htot;
-    const int h = hcarry + block_excl_scan(nit > HUB_ITEMS ? 1 : 0, htot, lds);
+    const int h = hcarry + (pk >> 16);
     if (nit > HUB_ITEMS) {
       d.hub[h] = make_int4(v, icarry + iex, o, o1);
       d.hub_mask[h] = mv;
@@ -451,22 +497,26 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
   const int nr = slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
-    if (v >= d.nV) continue;
-    const ulonglong2 vnv = reinterpret_cast<const ulonglong2 *>(d.vn)[v];
-    const uint64_t n = vnv.y;
+    const bool valid = v < d.nV;
     uint64_t nv = 0;
-    if (n) {
-      nv = n & used & ~vnv.x;
-      reinterpret_cast<ulonglong2 *>(d.vn)[v] = make_ulonglong2(vnv.x | nv, 0ull);
-    }
-    nf[v] = nv;
-    if (nv) {
-      uint64_t x = nv;
-      while (x) {
-        const int s = __ffsll((unsigned long long)x) - 1;
-        x &= x - 1;
-        d.depth16[(size_t)s * d.rowV + v] = (uint16_t)(round + 1 - s_start[s]);
+    if (valid) {
+      const ulonglong2 vnv = reinterpret_cast<const ulonglong2 *>(d.vn)[v];
+      const uint64_t n = vnv.y;
+      if (n) {
+        nv = n & used & ~vnv.x;
+        reinterpret_cast<ulonglong2 *>(d.vn)[v] = make_ulonglong2(vnv.x | nv, 0ull);
       }
+      nf[v] = nv;
+    }
+    // depths of the newly visited lanes: a wave holds 64 consecutive
+    // vertices of a slab and writes lane by lane, one contiguous run of
+    // the lane's depth row per store
+    for (uint64_t wx = wave_or(nv); wx; wx &= wx - 1) {
+      const int s = __ffsll((unsigned long long)wx) - 1;
+      if ((nv >> s) & 1ull)
+        d.depth16[(size_t)s * d.rowV + v] = (uint16_t)(round + 1 - s_start[s]);
+    }
+    if (nv) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
       const int nit = (deg + d.item_edges - 1) / d.item_edges;
