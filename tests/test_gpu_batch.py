@@ -219,3 +219,45 @@ def test_batch_bfs_deep_levels(gpu):
         want = oracle.check_bfs(n, s, d, v)
         assert want.max() > 254 or v == 350
         assert np.array_equal(res[lane], want), lane
+
+
+def test_batch_and_server_edge_cases(gpu):
+    # sources without out-edges, unreachable vertices, a one-vertex graph
+    # with no edges, and a server stream of one query per kind
+    src = np.array([1, 2, 0, 3], np.int32)
+    dst = np.array([2, 3, 4, 4], np.int32)
+    w = np.array([7, 9, 2, 5], np.int32)
+    el = gas.Edges.from_arrays(6, src, dst, w)
+    g = gas.Graph(el)
+    s, d, ww = el.arrays()
+    n = 6
+    for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
+        b = gas.Batch(g, kind)
+        sources = [4, 5, 1, 0]  # 4 and 5 have no out-edges
+        b.add(np.arange(4), sources)
+        b.run()
+        res = b.results(range(4))
+        for lane, v in enumerate(sources):
+            want = (oracle.check_bfs(n, s, d, v) if kind == gas.BATCH_BFS
+                    else oracle.check_dijkstra(n, s, d, ww, v))
+            assert np.array_equal(res[lane], want), (kind, lane)
+    b = gas.Batch(g, gas.BATCH_PPR)
+    b.add([0, 1], [5, 1])
+    b.run()
+    for lane, v in ((0, 5), (1, 1)):
+        ref = oracle.check_ppr_fp64(n, s, d, v, 10)
+        got = b.result(lane).astype(np.float64)
+        assert np.abs(got - ref).sum() <= 1e-5 * np.abs(ref).sum(), lane
+    one = gas.Graph(gas.Edges.from_arrays(1, [], [], []))
+    b = gas.Batch(one, gas.BATCH_BFS)
+    b.add([7], [0])
+    b.run()
+    assert b.result(7).tolist() == [0]
+    srv = gas.Server([g], memory_budget=1 << 30)
+    lat, results, st = srv.replay([0, 1, 2], [0, 0, 0], [4, 1, 5],
+                                  np.array([0.0, 0.0, 0.0]), keep=[0, 1, 2])
+    assert np.all(lat > 0)
+    assert np.array_equal(results[0], oracle.check_bfs(n, s, d, 4))
+    assert np.array_equal(results[1], oracle.check_dijkstra(n, s, d, ww, 1))
+    ref = oracle.check_ppr_fp64(n, s, d, 5, 10)
+    assert np.abs(results[2] - ref).sum() <= 1e-5 * np.abs(ref).sum()
