@@ -438,6 +438,13 @@ def main():
             road = gas.Graph(workloads.road())
             g22 = gas.Graph(workloads.rmat(22, symmetrize=True, weights=True))
             log("rmat22 built")
+            # batch autotuner: BFS pull divisor and SSSP distance step per graph
+            tuned = {}
+            for name, g in (("road1024", road), ("rmat22s", g22)):
+                tuned[name] = {"bfs_pull_divisor": g.autotune_batch(gas.BATCH_BFS),
+                               "sssp_delta": g.autotune_batch(gas.BATCH_SSSP)}
+            line["batch_tuning"] = tuned
+            log(f"batch tuning {tuned}")
             if "batched" in want:
                 line["batched"] = batched_lines(g22)
                 log("batched done")

@@ -332,6 +332,15 @@ int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
  * re-relaxations than Bellman-Ford rounds, same (unique) distances.
  * 0 = no bound. */
 int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta);
+/* Batch autotuner: times a 64-lane batch from a fixed sample of sources on
+ * this graph for each candidate setting of `kind` and keeps the fastest in
+ * the graph -- GAS_BATCH_BFS: pull divisor in {0, 2, 8, 32};
+ * GAS_BATCH_SSSP: distance step in {0, w/4, w/2, w} for the largest edge
+ * weight w.  Batch engines created on the graph afterwards (gas_batch_create,
+ * the server's engines) start with it; param_out (may be NULL) receives it. */
+int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out);
+/* The setting batch engines on this graph start with (0 until tuned). */
+int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param);
 int gas_batch_free(gas_batch_t b);
 
 /* ---- continuous-batching query server -------------------------------------

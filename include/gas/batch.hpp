@@ -824,6 +824,15 @@ class BatchEngine {
     BatchCtl c{};
     check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(c), m_stream), "ctl");
     check(gasrt_stream_synchronize(m_stream), "sync");
+    // the graph's tuned setting (gas_graph_autotune_batch)
+    int32_t param = 0;
+    check(gas_graph_batch_param(g, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP,
+                                &param),
+          "batch param");
+    if (kind == BFS_KIND)
+      setPullDivisor(param);
+    else
+      setSsspDelta(param);
   }
 
   ~BatchEngine() {

@@ -261,3 +261,30 @@ def test_batch_and_server_edge_cases(gpu):
     assert np.array_equal(results[1], oracle.check_dijkstra(n, s, d, ww, 1))
     ref = oracle.check_ppr_fp64(n, s, d, 5, 10)
     assert np.abs(results[2] - ref).sum() <= 1e-5 * np.abs(ref).sum()
+
+
+def test_batch_autotune(gpu):
+    el = workloads.road(128, 128, seed=8)
+    s, d, w = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    assert g.batch_param(gas.BATCH_SSSP) == 0
+    delta = g.autotune_batch(gas.BATCH_SSSP)
+    assert delta in (0, 16, 32, 64)
+    assert g.batch_param(gas.BATCH_SSSP) == delta
+    pull = g.autotune_batch(gas.BATCH_BFS)
+    assert pull in (0, 2, 8, 32)
+    # batches created now start with the tuned settings; results stay exact
+    src = _sources(np.random.default_rng(4), el, 10)
+    for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
+        b = gas.Batch(g, kind)
+        b.add(np.arange(10), src)
+        b.run()
+        res = b.results(range(10))
+        for lane in range(10):
+            want = (oracle.check_bfs(n, s, d, int(src[lane])) if kind == gas.BATCH_BFS
+                    else oracle.check_dijkstra(n, s, d, w, int(src[lane])))
+            assert np.array_equal(res[lane], want), (kind, lane)
+    nw = gas.Graph(gas.Edges.from_arrays(3, [0, 1], [1, 2]))
+    with pytest.raises(gas.GasError):
+        nw.autotune_batch(gas.BATCH_SSSP)  # no weights

@@ -293,6 +293,19 @@ class Graph:
         _check(lib().gas_graph_set_frontier_switch(self._h, C.c_int(program),
                                                    C.c_int32(divisor)))
 
+    def autotune_batch(self, kind: int) -> int:
+        """Times a 64-lane batch of `kind` (BATCH_BFS: pull divisor,
+        BATCH_SSSP: distance step) per candidate and keeps the fastest for
+        the batch engines created on this graph afterwards; returns it."""
+        p = C.c_int32(0)
+        _check(lib().gas_graph_autotune_batch(self._h, C.c_int(kind), C.byref(p)))
+        return p.value
+
+    def batch_param(self, kind: int) -> int:
+        p = C.c_int32(0)
+        _check(lib().gas_graph_batch_param(self._h, C.c_int(kind), C.byref(p)))
+        return p.value
+
     def set_small_frontier(self, program: int, max_vertices: int) -> None:
         """Runs supersteps of list frontiers of at most `max_vertices` inside
         one workgroup on graphs of degree <= 64 (0: off)."""

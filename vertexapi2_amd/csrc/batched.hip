@@ -2,8 +2,11 @@
 // (BatchEngine) and personalised PageRank lanes (PprBatch).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 #include "gas.h"
 #include "gas/batch.hpp"
@@ -196,6 +199,72 @@ int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta) {
     return set_error(GAS_ERR_ARG, "gas_batch_set_sssp_delta: need >= 0");
   if (b->e) b->e->setSsspDelta(delta);
   return GAS_OK;
+}
+
+int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param) {
+  if (!g || !param || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_graph_batch_param: bad arguments");
+  *param = g->batch_param[kind];
+  return GAS_OK;
+}
+
+int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
+  if (!g || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_graph_autotune_batch: bad arguments");
+  if (kind == GAS_BATCH_SSSP && !g->csr_w && g->nE > 0)
+    return set_error(GAS_ERR_ARG, "gas_graph_autotune_batch: no weights");
+  return guarded_b([&] {
+    // up to 64 sources with out-edges, spread over the ids by a fixed
+    // multiplicative stride
+    std::vector<int32_t> src, lanes;
+    const int64_t V = g->nV;
+    for (int64_t i = 0; i < V && (int)src.size() < BW; ++i) {
+      const int32_t v = (int32_t)((i * 2654435761ll + 12345) % V);
+      if (g->h_csr_off[v + 1] > g->h_csr_off[v] &&
+          std::find(src.begin(), src.end(), v) == src.end())
+        src.push_back(v);
+    }
+    if (src.empty()) src.push_back(0);
+    for (int i = 0; i < (int)src.size(); ++i) lanes.push_back(i);
+    std::vector<int32_t> cand;
+    if (kind == GAS_BATCH_BFS) {
+      cand = {0, 2, 8, 32};
+    } else {
+      const int32_t w = g->max_w;
+      for (int32_t c : {0, w / 4, w / 2, w})
+        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+    BatchEngine e(g, kind == GAS_BATCH_BFS ? BatchEngine::BFS_KIND
+                                           : BatchEngine::SSSP_KIND);
+    const uint64_t all = src.size() == 64 ? ~0ull : ((1ull << src.size()) - 1);
+    auto time_one = [&](int32_t c) {
+      if (kind == GAS_BATCH_BFS)
+        e.setPullDivisor(c);
+      else
+        e.setSsspDelta(c);
+      e.addLanes(lanes.data(), src.data(), (int)src.size());
+      e.sync();
+      const auto t0 = std::chrono::steady_clock::now();
+      e.runToCompletion();
+      e.sync();
+      const double s =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      e.releaseLanes(all);
+      return s;
+    };
+    time_one(cand[0]);  // warm-up
+    int32_t best = cand[0];
+    double best_s = 1e30;
+    for (int32_t c : cand) {
+      const double s = time_one(c);
+      if (s < best_s) {
+        best_s = s;
+        best = c;
+      }
+    }
+    g->batch_param[kind] = best;
+    if (param_out) *param_out = best;
+  });
 }
 
 int gas_batch_free(gas_batch_t b) {

@@ -38,7 +38,11 @@ struct gas_graph {
   int32_t *csc_off = nullptr, *csc_src = nullptr;
   int32_t *csr_off = nullptr, *csr_dst = nullptr;
   int32_t *csc_w = nullptr, *csr_w = nullptr;
+  int32_t max_w = 0;  // largest edge weight (0 without weights)
   size_t device_bytes = 0;
+  // batch settings new batch engines on this graph start with, by kind:
+  // BFS pull divisor, SSSP distance step (gas_graph_autotune_batch)
+  int32_t batch_param[2] = {0, 0};
   // Engines kept bound to this graph by the whole-run entry points
   // (gas_bfs, ...), one per program, released with the graph.
   void *engine_cache[8] = {};

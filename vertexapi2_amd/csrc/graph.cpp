@@ -501,6 +501,7 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
   up(g->csr_off, g->h_csr_off.data(), vb);
   up(g->csr_dst, g->h_csr_dst.data(), eb);
   if (!edges->w.empty()) {
+    for (int32_t x : edges->w) g->max_w = std::max(g->max_w, x);
     std::vector<int32_t> wc(g->nE);
     for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csc_eid[k]];
     up(g->csc_w, wc.data(), eb);
