@@ -1,6 +1,7 @@
 """GPU tests of the continuous-batching server: every served result equals
-the query run alone (bit-exact for BFS/SSSP, identical for PPR) and the CPU
-oracle; admission waits when memory is short and refuses queries that can
+the query run alone (bit-exact for BFS/SSSP; for PPR identical to a batch
+lane alone, or to the plain engine when the idle server answered it alone)
+and the CPU oracle; admission waits when memory is short and refuses queries that can
 never fit."""
 import numpy as np
 import pytest
@@ -36,10 +37,14 @@ def test_served_results_equal_alone(gpu, gang):
         elif k == gas.BATCH_SSSP:
             assert np.array_equal(results[q], oracle.check_dijkstra(n, src, dst, w, s)), q
         else:
+            # a batch lane is bit-identical alone and mixed; a query the
+            # idle server answered alone comes from the plain engine
             b = gas.Batch(graphs[gi], gas.BATCH_PPR)
             b.add([0], [s])
             b.run()
-            assert np.array_equal(results[q], b.result(0)), q  # alone == mixed
+            plain, _ = graphs[gi].ppr(s, 10)
+            assert (np.array_equal(results[q], b.result(0))
+                    or np.array_equal(results[q], plain)), q
             ref = oracle.check_ppr_fp64(n, src, dst, s, 10)
             assert np.abs(results[q] - ref).sum() <= 1e-5 * np.abs(ref).sum()
 
@@ -64,15 +69,14 @@ def test_admission_by_memory(gpu):
 
 
 def test_idle_server_answers_alone(gpu):
-    # at a low rate queries find the server idle: BFS / SSSP are answered by
-    # the plain engine at once (a batch of one), PPR still by its batch
+    # at a low rate queries find the server idle and are answered by the
+    # plain engine at once (a batch of one), PPR included
     el = workloads.road(64, 64, seed=8)
     road = gas.Graph(el)
     srv = gas.Server([road], memory_budget=1 << 30)
     stream = serve.make_stream(rate_qps=200, n=24, graphs=[road], seed=13)
     out, lat, results = serve.serve(srv, stream, keep=list(range(24)))
-    n_bfs_sssp = int(np.sum(stream["kinds"] != gas.BATCH_PPR))
-    assert 0 < out["solo"] <= n_bfs_sssp
+    assert 0 < out["solo"] <= 24
     s, d, w = el.arrays()
     n = road.n_vertices
     for q in range(24):

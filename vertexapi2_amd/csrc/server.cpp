@@ -1,9 +1,10 @@
 // Continuous-batching query server over graphs resident in HBM.
 //
-// A BFS or SSSP query whose engine is idle when it waits (and that no
-// earlier query for that engine precedes) is answered by the plain engine at
-// once when that is cheap for its kind and graph (a batch of one); queries
-// that find their engine running are batched.
+// A query whose engine is idle when it waits (and that no earlier query for
+// that engine precedes) is answered by the plain engine at once when that
+// is cheap for its kind and graph, or when it is the only query in the
+// server (a batch of one); queries that find their engine running are
+// batched.
 //
 // One batch engine per (query kind, graph) -- BFS and SSSP lanes
 // (BatchEngine), personalised PageRank lanes (PprBatch) -- each on its own
@@ -48,10 +49,13 @@ double now_s() {
 // waiting for it: a finished lane waits at most this many rounds longer.
 constexpr int kRoundsPerPoll = 4;
 
-// A query that arrives at an idle server is answered by the plain engine
-// alone (a batch of one) when that takes at most this long on average for
-// its kind and graph (measured as the server runs).
+// A query whose engine is idle is answered by the plain engine alone (a
+// batch of one) when that takes at most kSoloMaxMs for its kind and graph
+// (measured at the first replay), or at most kSoloIdleMaxMs when nothing
+// else is running or waiting: a batched round costs about as much for one
+// lane as for 64 (R-MAT-22 PPR: 73 ms as a batch of one, 11 ms plain).
 constexpr double kSoloMaxMs = 2.0;
+constexpr double kSoloIdleMaxMs = 50.0;
 
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
@@ -200,11 +204,14 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
         auto &buf = s->solo_out[gi];
         if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
-        for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP}) {
+        for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP, GAS_BATCH_PPR}) {
           auto run = [&] {
             return kind == GAS_BATCH_BFS
                        ? gas_bfs_device(s->graphs[gi], 0, buf.p, nullptr)
-                       : gas_sssp_device(s->graphs[gi], 0, buf.p, nullptr);
+                   : kind == GAS_BATCH_SSSP
+                       ? gas_sssp_device(s->graphs[gi], 0, buf.p, nullptr)
+                       : gas_ppr_device(s->graphs[gi], 0, s->ppr_iterations,
+                                        reinterpret_cast<float *>(buf.p), nullptr);
           };
           if (run() != GAS_OK) continue;  // e.g. SSSP on an unweighted graph
           const double ts = now_s();
@@ -278,11 +285,18 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       // engines between rounds
       const double t = now_s() - t0;
       while (next < n && arrival_s[next] <= t) waiting.push_back(next++);
-      // a waiting BFS / SSSP query whose engine is idle and which is the
-      // first waiting query for that engine: the plain engine answers it now
-      // on its own stream (other engines' rounds keep running; the cost cap
-      // bounds how long their polling pauses).  One per loop iteration.
+      // a waiting query whose engine is idle and which is the first waiting
+      // query for that engine: the plain engine answers it now on its own
+      // stream (other engines' rounds keep running; the cost caps bound how
+      // long their polling pauses).  BFS / SSSP results are exact on either
+      // engine; a PPR result from the plain engine is within the same 1e-5
+      // of FP64 as a batch lane but not bit-identical to one.  One per loop
+      // iteration.
       {
+        bool server_idle = waiting.size() == 1;
+        for (auto &ep : s->engines)
+          if (ep && (ep->inflight || ep->running || !ep->add_lanes.empty()))
+            server_idle = false;
         std::vector<char> seen(s->engines.size(), 0);
         for (auto it = waiting.begin(); it != waiting.end(); ++it) {
           const int q = *it;
@@ -292,17 +306,18 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
           const auto &ep = s->engines[key];
           const bool idle =
               !ep || (!ep->inflight && !ep->running && ep->add_lanes.empty());
-          // BFS / SSSP only: their results are exact on either engine, while
-          // a PPR lane stays bit-identical alone and in any batch
-          if (!idle || kinds[q] == GAS_BATCH_PPR || s->solo_ms[key] > kSoloMaxMs)
-            continue;
+          const double cap = server_idle ? kSoloIdleMaxMs : kSoloMaxMs;
+          if (!idle || s->solo_ms[key] > cap) continue;
           const int gi = graph_ids[q];
           auto &buf = s->solo_out[gi];
           if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
           const int rc =
               kinds[q] == GAS_BATCH_BFS
                   ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
-                  : gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr);
+              : kinds[q] == GAS_BATCH_SSSP
+                  ? gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr)
+                  : gas_ppr_device(s->graphs[gi], sources[q], s->ppr_iterations,
+                                   reinterpret_cast<float *>(buf.p), nullptr);
           if (rc != GAS_OK) throw Error(rc, gas_last_error());
           const double tf = now_s();
           latency_ms[q] = (tf - t0 - arrival_s[q]) * 1e3;
