@@ -265,10 +265,12 @@ def app_lines(steps):
     g = gas.Graph(workloads.rmat(21))
     V, E = g.n_vertices, g.n_edges
     tiers = g.autotune(gas.PROGRAM_PAGERANK)
-    g.pagerank(20)
-    t, (r, res, st0) = timed_runs(lambda: g.pagerank(20), reps)
+    _, res, _ = g.pagerank(20)  # host-result run: the per-sweep residuals
+    rbuf = gas.DeviceBuffer(4 * V)
+    g.pagerank_device(rbuf.ptr, 20)
+    t, st0 = timed_runs(lambda: g.pagerank_device(rbuf.ptr, 20), reps)
     gas.set_kernel_timing(True)
-    _, _, st = g.pagerank(20)
+    st = g.pagerank_device(rbuf.ptr, 20)
     gas.set_kernel_timing(False)
     coff = g.host_csc()[0]
     indeg = np.diff(coff)
@@ -286,8 +288,11 @@ def app_lines(steps):
     g = gas.Graph(workloads.rmat(21, symmetrize=True))
     V = g.n_vertices
     tiers = g.autotune(gas.PROGRAM_CC)
-    g.cc()
-    t, (lab, st0) = timed_runs(lambda: g.cc(), reps)
+    lab, _ = g.cc()
+    lbuf = gas.DeviceBuffer(4 * V)
+    g.cc_device(lbuf.ptr)
+    t, st0 = timed_runs(lambda: g.cc_device(lbuf.ptr), reps)
+    assert np.array_equal(lbuf.to_numpy(), lab)
     out["rmat21s_cc"] = {"gteps": round(st0.gathered_edges / t / 1e9, 4),
                          "ms": round(t * 1e3, 3), "device_ms": round(st0.ms, 3),
                          "supersteps": st0.supersteps, "E": g.n_edges, "tiers": tiers,

@@ -267,6 +267,13 @@ int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats);
  * FP64 sum of |new - old| reduced on the GPU. */
 int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
                  double *residual_out, gas_run_stats *stats);
+/* The same two runs with the result left in HBM (label_dev: n_vertices
+ * int32; rank_dev: n_vertices floats, residual_dev: `iterations` doubles or
+ * NULL); the initial state is written on the device.  PageRank here runs
+ * fixed sweeps only (iterations >= 1). */
+int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats);
+int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
+                        double *residual_dev, gas_run_stats *stats);
 
 /* Personalised PageRank from one source, `iterations` all-active sweeps in
  * FP32 with compensated summation: r_0 = 0.15 e_s, r_{k+1}(v) = 0.15 [v==s]

@@ -292,3 +292,27 @@ def test_small_frontier_mode(gpu, small_max):
         ref, ost = oracle.cc(n, s, d)
         assert np.array_equal(lab, ref)
         assert st.supersteps == ost.supersteps
+
+
+def test_device_result_entry_points(gpu):
+    # the *_device runs (state initialised on the GPU, result left in HBM)
+    # give the same values as the host-result runs
+    el = workloads.rmat(14, symmetrize=True, weights=True)
+    g = gas.Graph(el)
+    n = el.n_vertices
+    buf = gas.DeviceBuffer(4 * n)
+    src = int(el.max_outdegree_vertex()[0])
+    g.bfs_device(src, buf.ptr)
+    assert np.array_equal(buf.to_numpy(), g.bfs(src)[0])
+    g.sssp_device(src, buf.ptr)
+    assert np.array_equal(buf.to_numpy(), g.sssp(src)[0])
+    g.cc_device(buf.ptr)
+    assert np.array_equal(buf.to_numpy(), g.cc()[0])
+    rank, res, _ = g.pagerank(20)
+    rbuf = gas.DeviceBuffer(8 * 20)
+    st = g.pagerank_device(buf.ptr, 20, rbuf.ptr)
+    assert st.supersteps == 20
+    assert np.array_equal(buf.to_numpy().view(np.float32), rank)
+    assert np.array_equal(rbuf.to_numpy().view(np.float64), res)
+    g.ppr_device(src, buf.ptr, 10)
+    assert np.array_equal(buf.to_numpy().view(np.float32), g.ppr(src, 10)[0])

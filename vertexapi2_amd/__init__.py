@@ -346,6 +346,22 @@ class Graph:
                                      C.byref(st)))
         return st
 
+    def cc_device(self, out_ptr: int):
+        """Connected components with the labels written to device memory."""
+        st = RunStats()
+        _check(lib().gas_cc_device(self._h, C.c_void_p(out_ptr), C.byref(st)))
+        return st
+
+    def pagerank_device(self, out_ptr: int, iterations: int = 20, residual_ptr=None):
+        """Fixed-sweep PageRank with the ranks (and, if given, the per-sweep
+        FP64 residuals) written to device memory."""
+        st = RunStats()
+        _check(lib().gas_pagerank_device(self._h, C.c_int32(iterations),
+                                         C.c_void_p(out_ptr),
+                                         C.c_void_p(residual_ptr) if residual_ptr else None,
+                                         C.byref(st)))
+        return st
+
     def ppr_device(self, source: int, out_ptr: int, iterations: int = 10):
         """Personalised PageRank with the ranks written to device memory."""
         st = RunStats()

@@ -138,6 +138,30 @@ __global__ void k_ppr_init(Dev<PersonalizedPageRank> d, int32_t source) {
   }
 }
 
+// PageRank's initial state (pagerank.cu:89-90, 126-132) written on the device.
+__global__ void k_pr_init(Dev<PageRank> d) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT) {
+    PageRank::VertexData x;
+    x.rank = PageRank::pageConst;
+    x.numOutEdges = d.csr_off[v + 1] - d.csr_off[v];
+    d.vdata[v] = x;
+  }
+}
+
+__global__ void k_pr_rank_out(Dev<PageRank> d, float *out) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT)
+    out[v] = d.vdata[v].rank;
+}
+
+// CC's initial labels, label[v] = v (connected_component.cu:112-114).
+__global__ void k_cc_init(Dev<CC> d) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT)
+    d.vdata[v] = (int)v;
+}
+
 __global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, float *out) {
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
@@ -404,6 +428,62 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
         stats->gathered_edges = (int64_t)iterations * g->nE;
         stats->active_total = (int64_t)iterations * g->nV;
       }
+    }
+  });
+}
+
+int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats) {
+  if (!g || !label_dev) return set_error(GAS_ERR_ARG, "gas_cc_device: bad arguments");
+  return guarded([&] {
+    auto &e = cached_engine<CC>(g, GAS_PROGRAM_CC, nullptr);
+    e.resetStats();
+    RunTimer tm(e.stream());
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
+    launch(k_cc_init, grid, e.stream(), e.dev());
+    e.stateChanged();
+    e.setActive(0, g->nV);
+    e.setIteration(0);
+    e.runUntilEmpty(true, true);
+    check(gasrt_memcpy_d2d(label_dev, e.deviceVertexData(),
+                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
+          "result");
+    double ms = tm.stop();
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+    }
+  });
+}
+
+int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
+                        double *residual_dev, gas_run_stats *stats) {
+  if (!g || !rank_dev || iterations < 1)
+    return set_error(GAS_ERR_ARG, "gas_pagerank_device: bad arguments");
+  return guarded([&] {
+    auto &e = cached_engine<PageRank>(g, GAS_PROGRAM_PAGERANK, nullptr);
+    e.resetStats();
+    DevBuf<double> hist;
+    if (!residual_dev) hist.alloc((size_t)iterations);
+    double *res = residual_dev ? residual_dev : hist.p;
+    RunTimer tm(e.stream());
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
+    launch(k_pr_init, grid, e.stream(), e.dev());
+    e.stateChanged();
+    for (int it = 0; it < iterations; ++it) {
+      e.setActive(0, g->nV);
+      e.gatherApply(true);
+      launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
+    }
+    launch(k_pr_rank_out, grid, e.stream(), e.dev(), rank_dev);
+    double ms = tm.stop();
+    if (stats) {
+      e.stats(stats);
+      stats->ms = ms;
+      stats->supersteps = iterations;
+      stats->gathered_edges = (int64_t)iterations * g->nE;
+      stats->active_total = (int64_t)iterations * g->nV;
     }
   });
 }
