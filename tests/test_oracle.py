@@ -11,7 +11,6 @@ with:
     id, FP64 power iteration) and brute force on tiny graphs;
   * planted mistakes in a copy of the oracle's inputs must be caught.
 """
-import itertools
 import os
 
 import numpy as np

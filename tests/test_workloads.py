@@ -4,7 +4,6 @@ dedupe / symmetrisation and reproducibility."""
 import numpy as np
 import pytest
 
-import vertexapi2_amd as gas
 from vertexapi2_amd import workloads
 
 

@@ -12,7 +12,7 @@ import time
 
 import numpy as np
 
-from . import BATCH_BFS, BATCH_PPR, BATCH_SSSP, Graph
+from . import BATCH_BFS, BATCH_PPR, BATCH_SSSP
 
 KIND_NAMES = {BATCH_BFS: "bfs", BATCH_SSSP: "sssp", BATCH_PPR: "ppr"}
 
