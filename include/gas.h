@@ -388,6 +388,12 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
  * batch's lanes move level by level together and share edge reads.
  * backlog 0 = always admit into free lanes.  Default 16 and 1. */
 int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds);
+/* Batches of one: a query whose plain-engine time (measured per kind and
+ * graph at the first replay) is at most max_ms, or at most idle_max_ms
+ * while the server is lightly loaded, is answered on the plain engine by a
+ * second host thread beside the batch rounds.  0, 0 = always batch.
+ * Default 2 ms and 50 ms. */
+int gas_server_set_solo(gas_server_t s, double max_ms, double idle_max_ms);
 int gas_server_free(gas_server_t s);
 
 /* ---- the engine API, step by step --------------------------------------- */

@@ -55,6 +55,7 @@ def test_admission_by_memory(gpu):
     V = road.n_vertices
     budget = 2 * gas.query_bytes(gas.BATCH_SSSP, V)  # two SSSP queries at once
     srv = gas.Server([road], memory_budget=budget)
+    srv.set_solo(0, 0)  # every query through the batch engine
     stream = serve.make_stream(rate_qps=100000, n=12, graphs=[road], mix=(0, 1, 0), seed=4)
     out, lat, results = serve.serve(srv, stream, keep=list(range(12)))
     assert out["max_concurrent"] <= 2

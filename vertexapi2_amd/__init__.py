@@ -571,6 +571,10 @@ class Server:
             lib().gas_server_free(self._h)
             self._h = None
 
+    def set_solo(self, max_ms: float, idle_max_ms: float) -> None:
+        """Plain-engine cost caps for answering a query alone (0, 0: never)."""
+        _check(lib().gas_server_set_solo(self._h, C.c_double(max_ms), C.c_double(idle_max_ms)))
+
     def set_gang(self, backlog: int, rounds: int) -> None:
         """While `backlog` queries wait for a BFS/SSSP engine, they join its
         batch only within `rounds` rounds of its start (0: always join)."""

@@ -1,10 +1,10 @@
 // Continuous-batching query server over graphs resident in HBM.
 //
-// A query whose engine is idle when it waits (and that no earlier query for
-// that engine precedes) is answered by the plain engine at once when that
-// is cheap for its kind and graph, or when it is the only query in the
-// server (a batch of one); queries that find their engine running are
-// batched.
+// Waiting queries that are cheap on the plain engine for their kind and
+// graph (or any query on a lightly loaded server) are queued to a second
+// host thread that answers them there one at a time (a batch of one); the
+// plain engine's latency-bound supersteps run beside the batch rounds while
+// the loop keeps driving the batch engines.
 //
 // One batch engine per (query kind, graph) -- BFS and SSSP lanes
 // (BatchEngine), personalised PageRank lanes (PprBatch) -- each on its own
@@ -24,10 +24,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -49,13 +53,18 @@ double now_s() {
 // waiting for it: a finished lane waits at most this many rounds longer.
 constexpr int kRoundsPerPoll = 4;
 
-// A query whose engine is idle is answered by the plain engine alone (a
-// batch of one) when that takes at most kSoloMaxMs for its kind and graph
-// (measured at the first replay), or at most kSoloIdleMaxMs when nothing
-// else is running or waiting: a batched round costs about as much for one
-// lane as for 64 (R-MAT-22 PPR: 73 ms as a batch of one, 11 ms plain).
+// A query is answered by the plain engine alone (a batch of one) on the solo
+// thread when that takes at most kSoloMaxMs for its kind and graph, or at
+// most kSoloIdleMaxMs while few queries of its kind and graph wait, and the
+// solo queue would finish it no later than a batch of one would (both costs
+// measured at the first replay): a batched round costs about as much for
+// one lane as for 64 (R-MAT-22 PPR: 73 ms as a batch of one, 11 ms plain),
+// while the plain engine's latency-bound supersteps leave the GPU room for
+// batch rounds beside them.
 constexpr double kSoloMaxMs = 2.0;
 constexpr double kSoloIdleMaxMs = 50.0;
+constexpr int kSoloBacklog = 4;
+constexpr double kSoloQueueMs = 8.0;
 
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
@@ -65,6 +74,69 @@ size_t query_bytes(int kind, int64_t V) {
     default: return (size_t)(V * 16 + V / 4 + V * 4);  // PPR
   }
 }
+
+// The second host thread: queued queries one at a time on the plain engine
+// (gas_*_device into a per-graph buffer), each result optionally copied to
+// the caller on the thread's own stream.
+struct SoloWorker {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::thread th;
+  bool stop = false;
+  std::deque<int> jobs;                              // queued queries
+  std::vector<std::pair<int, double>> finished;      // query, finish time
+  int rc = GAS_OK;                                   // first failure
+  std::string err;
+  double queued_ms = 0;  // loop side: expected plain-engine time handed over
+  int pending = 0;       // loop side: queries handed over, not yet taken
+
+  template <class F>
+  void start(F run) {
+    th = std::thread([this, run] {
+      for (;;) {
+        int q;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv.wait(lk, [&] { return stop || !jobs.empty(); });
+          if (stop) return;
+          q = jobs.front();
+          jobs.pop_front();
+        }
+        const int r = run(q);
+        const double t = now_s();
+        std::lock_guard<std::mutex> lk(mu);
+        if (r != GAS_OK && rc == GAS_OK) {
+          rc = r;
+          err = gas_last_error();
+        }
+        finished.emplace_back(q, t);
+      }
+    });
+  }
+  void submit(int q) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      jobs.push_back(q);
+    }
+    cv.notify_one();
+  }
+  // finished queries since the last call; throws the first failure
+  std::vector<std::pair<int, double>> take() {
+    std::vector<std::pair<int, double>> out;
+    std::lock_guard<std::mutex> lk(mu);
+    if (rc != GAS_OK) throw Error(rc, err);
+    out.swap(finished);
+    return out;
+  }
+  ~SoloWorker() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_one();
+    if (th.joinable()) th.join();
+  }
+};
 
 struct Engine {
   int kind = 0, graph = 0;
@@ -89,8 +161,10 @@ struct gas_server {
   size_t budget = 0, reserved = 0;
   int ppr_iterations = 10;
   int gang_backlog = 16, gang_rounds = 1;        // gas_server_set_gang
+  double solo_max_ms = kSoloMaxMs, solo_idle_max_ms = kSoloIdleMaxMs;
   std::vector<std::unique_ptr<Engine>> engines;  // [kind * n_graphs + graph]
   std::vector<double> solo_ms;                   // plain-engine cost per key
+  std::vector<double> batch1_ms;                 // batch-of-one cost per key
   bool calibrated = false;
   std::vector<DevBuf<int32_t>> solo_out;         // [graph] plain results
 
@@ -158,9 +232,18 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
     s->ppr_iterations = ppr_iterations;
     s->engines.resize(3 * (size_t)n_graphs);
     s->solo_ms.assign(3 * (size_t)n_graphs, 1e30);
+    s->batch1_ms.assign(3 * (size_t)n_graphs, 0.0);
     s->solo_out.resize((size_t)n_graphs);
     *out = s.release();
   });
+}
+
+int gas_server_set_solo(gas_server_t s, double max_ms, double idle_max_ms) {
+  if (!s || !(max_ms >= 0) || !(idle_max_ms >= 0))
+    return set_error(GAS_ERR_ARG, "gas_server_set_solo: bad arguments");
+  s->solo_max_ms = max_ms;
+  s->solo_idle_max_ms = idle_max_ms;
+  return GAS_OK;
 }
 
 int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds) {
@@ -219,6 +302,28 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
           s->solo_ms[(size_t)kind * s->graphs.size() + gi] = (now_s() - ts) * 1e3;
         }
       }
+      // and a batch of one lane on each batch engine (from vertex 0)
+      for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
+        for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP, GAS_BATCH_PPR}) {
+          const size_t key = (size_t)kind * s->graphs.size() + gi;
+          if (s->solo_ms[key] >= 1e30) continue;  // kind unavailable
+          Engine &e = s->engine(kind, (int)gi);
+          const int32_t lane = 0, src = 0;
+          const double ts = now_s();
+          if (e.be) {
+            e.be->addLanes(&lane, &src, 1);
+            e.be->runToCompletion();
+            e.be->sync();
+            e.be->releaseLanes(1);
+          } else {
+            e.pe->addLanes(&lane, &src, 1);
+            e.pe->runToCompletion();
+            e.pe->sync();
+            e.pe->releaseLanes(1);
+          }
+          s->batch1_ms[key] = (now_s() - ts) * 1e3;
+        }
+      }
       clear_error();
       s->calibrated = true;
     }
@@ -227,6 +332,28 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
     int64_t rounds = 0, admissions_deferred = 0;
     int max_running = 0, solo = 0;
     const double t0 = now_s();
+    gasrt_stream_t copy_stream = nullptr;
+    check(gasrt_stream_create(&copy_stream), "stream");
+    struct StreamGuard {
+      gasrt_stream_t s;
+      ~StreamGuard() { gasrt_stream_destroy(s); }
+    } copy_guard{copy_stream};
+    SoloWorker worker;  // joined before copy_stream is destroyed
+    worker.start([&](int q) {
+      const int gi = graph_ids[q];
+      auto &buf = s->solo_out[gi];
+      if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
+      int rc = kinds[q] == GAS_BATCH_BFS
+                   ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
+               : kinds[q] == GAS_BATCH_SSSP
+                   ? gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr)
+                   : gas_ppr_device(s->graphs[gi], sources[q], s->ppr_iterations,
+                                    reinterpret_cast<float *>(buf.p), nullptr);
+      if (rc == GAS_OK && host_results && host_results[q])
+        rc = gasrt_memcpy_d2h(host_results[q], buf.p, 4 * (size_t)s->nV[gi],
+                              copy_stream);
+      return rc;
+    });
     auto running_total = [&] {
       int r = 0;
       for (auto &ep : s->engines)
@@ -285,51 +412,45 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       // engines between rounds
       const double t = now_s() - t0;
       while (next < n && arrival_s[next] <= t) waiting.push_back(next++);
-      // a waiting query whose engine is idle and which is the first waiting
-      // query for that engine: the plain engine answers it now on its own
-      // stream (other engines' rounds keep running; the cost caps bound how
-      // long their polling pauses).  BFS / SSSP results are exact on either
-      // engine; a PPR result from the plain engine is within the same 1e-5
-      // of FP64 as a batch lane but not bit-identical to one.  One per loop
-      // iteration.
+      // queries the solo thread finished
+      for (const auto &f : worker.take()) {
+        const int q = f.first;
+        latency_ms[q] = (f.second - t0 - arrival_s[q]) * 1e3;
+        worker.queued_ms -= s->solo_ms[(size_t)kinds[q] * s->graphs.size() +
+                                       graph_ids[q]];
+        if (--worker.pending == 0) worker.queued_ms = 0;
+        ++done;
+        ++solo;
+        progressed = true;
+      }
+      // waiting queries go to the solo thread in arrival order when their
+      // plain-engine time is at most solo_max_ms, or at most solo_idle_max_ms
+      // while fewer than kSoloBacklog queries of their kind and graph wait
+      // (more would share a batch), and the solo queue would finish them no
+      // later than a batch of one (an idle solo thread takes one
+      // regardless).  BFS / SSSP results are exact on either engine; a PPR
+      // result from the plain engine is within the same 1e-5 of FP64 as a
+      // batch lane but not bit-identical to one.
       {
-        bool server_idle = waiting.size() == 1;
-        for (auto &ep : s->engines)
-          if (ep && (ep->inflight || ep->running || !ep->add_lanes.empty()))
-            server_idle = false;
-        std::vector<char> seen(s->engines.size(), 0);
-        for (auto it = waiting.begin(); it != waiting.end(); ++it) {
+        std::vector<int> backlog(s->engines.size(), 0);
+        for (int q : waiting)
+          ++backlog[(size_t)kinds[q] * s->graphs.size() + graph_ids[q]];
+        for (auto it = waiting.begin(); it != waiting.end();) {
           const int q = *it;
           const size_t key = (size_t)kinds[q] * s->graphs.size() + graph_ids[q];
-          if (seen[key]) continue;
-          seen[key] = 1;
-          const auto &ep = s->engines[key];
-          const bool idle =
-              !ep || (!ep->inflight && !ep->running && ep->add_lanes.empty());
-          const double cap = server_idle ? kSoloIdleMaxMs : kSoloMaxMs;
-          if (!idle || s->solo_ms[key] > cap) continue;
-          const int gi = graph_ids[q];
-          auto &buf = s->solo_out[gi];
-          if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
-          const int rc =
-              kinds[q] == GAS_BATCH_BFS
-                  ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
-              : kinds[q] == GAS_BATCH_SSSP
-                  ? gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr)
-                  : gas_ppr_device(s->graphs[gi], sources[q], s->ppr_iterations,
-                                   reinterpret_cast<float *>(buf.p), nullptr);
-          if (rc != GAS_OK) throw Error(rc, gas_last_error());
-          const double tf = now_s();
-          latency_ms[q] = (tf - t0 - arrival_s[q]) * 1e3;
-          if (host_results && host_results[q])
-            check(gasrt_memcpy_d2h(host_results[q], buf.p, 4 * (size_t)s->nV[gi],
-                                   nullptr),
-                  "result copy");
-          waiting.erase(it);
-          ++done;
-          ++solo;
+          const double ms = s->solo_ms[key];
+          const double cap =
+              backlog[key] < kSoloBacklog ? s->solo_idle_max_ms : s->solo_max_ms;
+          const double budget = std::max(kSoloQueueMs, s->batch1_ms[key]);
+          if (ms > cap || (worker.pending > 0 && worker.queued_ms + ms > budget)) {
+            ++it;
+            continue;
+          }
+          worker.queued_ms += ms;
+          ++worker.pending;
+          worker.submit(q);
+          it = waiting.erase(it);
           progressed = true;
-          break;
         }
       }
       std::vector<char> blocked(s->engines.size(), 0);
@@ -402,7 +523,7 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       }
       max_running = std::max(max_running, running_total());
       if (!progressed) {
-        bool any_inflight = false;
+        bool any_inflight = worker.pending > 0;
         for (auto &ep : s->engines) any_inflight |= ep && ep->inflight;
         if (!any_inflight && waiting.empty() && next < n) {
           const double wait = arrival_s[next] - (now_s() - t0);
