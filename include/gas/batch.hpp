@@ -44,7 +44,7 @@ constexpr int BW = 64;            // lanes per batch (= wave size)
 constexpr int BFS_ITEM_EDGES = 16;   // BFS push: out-edges per item, 4 items per wave
 constexpr int SSSP_ITEM_EDGES = 64;  // SSSP push: out-edges per item (one wave)
 constexpr int BFS_ITEMS_PER_TRIP = 8;  // BFS push: items in flight per wave
-constexpr int HUB_ITEMS = 256;    // a vertex with more items is written by
+constexpr int HUB_ITEMS = 32;     // a vertex with more items is written by
                                   // a whole workgroup of kb_hub_items
 constexpr int SSSP_UNROLL = 8;    // edges per item per wave trip in the SSSP push
 constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks together
