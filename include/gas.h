@@ -377,7 +377,10 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
  * the call) in real time.  latency_ms[i] receives arrival-to-result time;
  * host_results[i], if non-NULL, receives the V int32 (BFS, SSSP) or float
  * (PPR) result.  Fails with GAS_ERR_FULL, serving nothing, if some query can
- * never fit the budget. */
+ * never fit the budget.  During the call a second host thread runs the
+ * graphs' whole-run engines (gas_bfs_device, ...) for batches of one, so
+ * the caller must not run whole-run entry points on the same graphs
+ * concurrently. */
 int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
                       const int32_t *graph_ids, const int32_t *sources,
                       const double *arrival_s, void *const *host_results,
