@@ -428,6 +428,29 @@ __device__ inline int block_sum(int x, int *lds) {
   return t;
 }
 
+// Workgroup sums of three counters in one LDS exchange (lds: 3 * NT / WAVE).
+__device__ inline void block_sum3(int &a, int &b, int &c, int *lds) {
+  const int sa = wave_sum(a), sb = wave_sum(b), sc = wave_sum(c);
+  const int w = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    lds[w] = sa;
+    lds[NT / WAVE + w] = sb;
+    lds[2 * (NT / WAVE) + w] = sc;
+  }
+  __syncthreads();
+  int ta = 0, tb = 0, tc = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) {
+    ta += lds[i];
+    tb += lds[NT / WAVE + i];
+    tc += lds[2 * (NT / WAVE) + i];
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+  c = tc;
+}
+
 __device__ inline double block_sum(double x, double *lds) {
   double s = wave_sum(x);
   if (lane_id() == 0) lds[threadIdx.x / WAVE] = s;
@@ -1145,11 +1168,14 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
     else if (valid)
       d.rflag[i] = ret ? 1 : 0;
   });
-  active = block_sum(active, lds);
+  {
+    __shared__ int lds3[3 * (NT / WAVE)];
+    int none = 0;
+    block_sum3(active, scattered, none, lds3);
+  }
   if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
     d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
   add_stat(d, 2, active);
-  scattered = block_sum(scattered, lds);
   add_stat(d, 1, scattered);
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
@@ -1267,13 +1293,14 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
     }
     scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
   }, fixup);
-  active = block_sum(active, lds);
+  {
+    __shared__ int lds3[3 * (NT / WAVE)];
+    block_sum3(active, gathered, scattered, lds3);
+  }
   if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
     d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
   add_stat(d, 2, active);
-  gathered = block_sum(gathered, lds);
   add_stat(d, 0, gathered);
-  scattered = block_sum(scattered, lds);
   add_stat(d, 1, scattered);
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
