@@ -1041,9 +1041,10 @@ class BatchEngine {
 // ---- personalised PageRank, 64 FP32 lanes ----------------------------------
 // r_0 = 0.15 e_s, r_{k+1}(v) = 0.15 [v == s] + 0.85 sum_{u->v} r_k(u)/outdeg(u)
 // per lane s.  Every vertex owns >= 1 fixed item of at most PPR_ITEM in-edges;
-// one wave per item walks them in CSC order, reading a source's row only for
-// the lanes whose mask says the row is nonzero, with a compensated sum per
-// lane.  Vertices with several items fold their partials in item order.
+// one wave per item walks them in CSC order, reading a source's row (the
+// running lanes) only where the source's byte says some lane is nonzero,
+// with a compensated sum per lane: a lane adds +0.0 for a source whose own
+// value is zero, which leaves its sum and compensation unchanged.  Vertices with several items fold their partials in item order.
 // The additions a lane performs depend on that lane alone, so a query's
 // result is bit-identical alone and inside any mix.
 constexpr int PPR_ITEM = 256;
@@ -1060,7 +1061,9 @@ struct PprDev {
   int32_t n_items, n_multi;
   float *rank[2];                 // [V][BW]
   float *contrib[2];              // [V][BW] rank / outdeg, what gathers read
-  uint64_t *mask[2];              // lanes with a nonzero rank
+  uint8_t *nz[3];                 // [V] some running lane is nonzero, by
+                                  // sweep mod 3 (read / written / cleared)
+  int32_t rot;                    // sweep mod 3: nz[rot] is this sweep's input
   float *part_s, *part_c;         // [slots][BW]
   const int32_t *src_lane;        // [BW]
 };
@@ -1075,14 +1078,21 @@ __device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
   d.rank[p ^ 1][(size_t)v * BW + lane] = val;
   d.contrib[p ^ 1][(size_t)v * BW + lane] = od > 0 ? val / (float)od : 0.0f;
   const uint64_t m = __ballot(val != 0.0f);
-  if (lane == 0) d.mask[p ^ 1][v] = m;
+  if (lane == 0 && m) d.nz[(d.rot + 1) % 3][v] = 1;
 }
 
 static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
   const float *ct = d.contrib[p];
-  const uint64_t *mk = d.mask[p];
+  const uint8_t *nzin = d.nz[d.rot];
+  // the byte array the next sweep writes, cleared while this one reads nz[rot]
+  {
+    uint32_t *clr = reinterpret_cast<uint32_t *>(d.nz[(d.rot + 2) % 3]);
+    const int words = (d.nV + 3) / 4;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < words; i += gridDim.x * NT)
+      clr[i] = 0u;
+  }
   for (int it = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; it < d.n_items;
        it += waves) {
     const int v = d.itv[it];
@@ -1092,12 +1102,13 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
     for (int base = e0; base < e1; base += WAVE) {
       const int k = base + lane;
       int u_l = 0;
-      uint64_t m_l = 0;
+      bool f_l = false;
       if (k < e1) {
         u_l = d.csc_src[k];
-        m_l = mk[u_l] & run;
+        f_l = nzin[u_l] != 0;
       }
-      uint64_t nz = __ballot(m_l != 0);
+      uint64_t nz = __ballot(f_l);
+      const bool mine = (run >> lane) & 1ull;
       while (nz) {
         int u[PPR_UNROLL];
         bool on[PPR_UNROLL];
@@ -1108,10 +1119,7 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
           const bool have = nz != 0;
           nz &= nz - 1;
           u[x] = __shfl(u_l, j, WAVE);
-          const uint64_t mj =
-              ((uint64_t)__shfl((uint32_t)(m_l >> 32), j, WAVE) << 32) |
-              __shfl((uint32_t)m_l, j, WAVE);
-          on[x] = have && ((mj >> lane) & 1ull);
+          on[x] = have && mine;
         }
 #pragma unroll
         for (int x = 0; x < PPR_UNROLL; ++x)
@@ -1168,7 +1176,6 @@ static __global__ __launch_bounds__(NT) void kp_reset(PprDev d, int p,
       d.rank[p][i] = 0.f;
       d.contrib[p][i] = 0.f;
     }
-    if (s == 0) d.mask[p][i / BW] &= ~reset;
   }
 }
 
@@ -1180,8 +1187,7 @@ static __global__ void kp_seed(PprDev d, int p, const int32_t *lanes,
   d.rank[p][(size_t)v * BW + s] = 0.15f;
   const int od = d.outdeg[v];
   d.contrib[p][(size_t)v * BW + s] = od > 0 ? 0.15f / (float)od : 0.0f;
-  atomicOr((unsigned long long *)&d.mask[p][v],
-           (unsigned long long)(1ull << s));
+  d.nz[d.rot][v] = 1;
 }
 
 static __global__ __launch_bounds__(NT) void kp_extract(PprDev d, int p, int s,
@@ -1227,8 +1233,7 @@ class PprBatch {
     m_rank1.alloc(V * BW);
     m_contrib0.alloc(V * BW);
     m_contrib1.alloc(V * BW);
-    m_mask0.alloc(V);
-    m_mask1.alloc(V);
+    for (auto &b : m_nz) b.alloc((V + 3) / 4 * 4);
     m_ps.alloc((size_t)std::max(slots, 1) * BW);
     m_pc.alloc((size_t)std::max(slots, 1) * BW);
     m_src.alloc(BW);
@@ -1238,8 +1243,8 @@ class PprBatch {
     check(gasrt_memset(m_rank1.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_contrib0.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_contrib1.p, 0, V * BW * 4, m_stream), "memset");
-    check(gasrt_memset(m_mask0.p, 0, V * 8, m_stream), "memset");
-    check(gasrt_memset(m_mask1.p, 0, V * 8, m_stream), "memset");
+    for (auto &b : m_nz)
+      check(gasrt_memset(b.p, 0, (V + 3) / 4 * 4, m_stream), "memset");
     for (int i = 0; i < BW; ++i) m_hsrc[i] = -1;
     check(gasrt_memcpy_h2d(m_src.p, m_hsrc, sizeof(m_hsrc), m_stream), "src");
     PprDev &d = m_dev;
@@ -1259,8 +1264,8 @@ class PprBatch {
     d.rank[1] = m_rank1.p;
     d.contrib[0] = m_contrib0.p;
     d.contrib[1] = m_contrib1.p;
-    d.mask[0] = m_mask0.p;
-    d.mask[1] = m_mask1.p;
+    for (int i = 0; i < 3; ++i) d.nz[i] = m_nz[i].p;
+    d.rot = 0;
     d.part_s = m_ps.p;
     d.part_c = m_pc.p;
     d.src_lane = m_src.p;
@@ -1315,6 +1320,7 @@ class PprBatch {
     if (m_dev.n_multi)
       launch(kp_fix, SCAN_BLOCKS, m_stream, m_dev, m_par, run);
     m_par ^= 1;
+    m_dev.rot = (m_dev.rot + 1) % 3;
     for (int s = 0; s < BW; ++s)
       if ((run >> s) & 1ull) ++m_done[s];
   }
@@ -1383,7 +1389,7 @@ class PprBatch {
   DevBuf<int32_t> m_outdeg, m_itv, m_ite0, m_pslot, m_vitem0, m_mv, m_src,
       m_lanes, m_sources;
   DevBuf<float> m_rank0, m_rank1, m_contrib0, m_contrib1, m_ps, m_pc, m_tmp;
-  DevBuf<uint64_t> m_mask0, m_mask1;
+  DevBuf<uint8_t> m_nz[3];
 };
 
 }  // namespace gas
