@@ -64,7 +64,7 @@ namespace gas {
 constexpr int NT = 256;            // threads per workgroup: 4 waves of 64
 constexpr int WAVE = 64;
 constexpr int SCAN_BLOCKS = 1024;  // chunks of per-vertex kernels and scans
-constexpr int MP_VT = 8;           // merge-path items per thread
+constexpr int MP_VT = 6;           // merge-path items per thread
 constexpr int MP_NV = NT * MP_VT;  // merge-path items per tile
 constexpr int MP_BLOCKS = 2048;    // grid of the merge-path kernels
 constexpr int HUB_MIN_T2 = 4;      // hubs have degree > t2 >= 4 ...
