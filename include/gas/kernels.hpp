@@ -66,7 +66,7 @@ constexpr int WAVE = 64;
 constexpr int SCAN_BLOCKS = 1024;  // chunks of per-vertex kernels and scans
 constexpr int MP_VT = 6;           // merge-path items per thread
 constexpr int MP_NV = NT * MP_VT;  // merge-path items per tile
-constexpr int MP_BLOCKS = 2048;    // grid of the merge-path kernels
+constexpr int MP_BLOCKS = 4096;    // grid of the merge-path kernels
 constexpr int HUB_MIN_T2 = 4;      // hubs have degree > t2 >= 4 ...
 // ... so a segment spans >= 6 merge-path items and a tile touches at most
 constexpr int HUB_SEG_MAX = MP_NV / (HUB_MIN_T2 + 2) + 2;  // segments
