@@ -1,10 +1,10 @@
 // Continuous-batching query server over graphs resident in HBM.
 //
 // Waiting queries that are cheap on the plain engine for their kind and
-// graph (or any query on a lightly loaded server) are queued to a second
-// host thread that answers them there one at a time (a batch of one); the
-// plain engine's latency-bound supersteps run beside the batch rounds while
-// the loop keeps driving the batch engines.
+// graph, or that few others of their kind and graph wait beside, are queued
+// to a second host thread that answers them there one at a time (a batch of
+// one); the plain engine's latency-bound supersteps run beside the batch
+// rounds while the loop keeps driving the batch engines.
 //
 // One batch engine per (query kind, graph) -- BFS and SSSP lanes
 // (BatchEngine), personalised PageRank lanes (PprBatch) -- each on its own
@@ -58,7 +58,7 @@ constexpr int kRoundsPerPoll = 4;
 // most kSoloIdleMaxMs while few queries of its kind and graph wait, and the
 // solo queue would finish it no later than a batch of one would (both costs
 // measured at the first replay): a batched round costs about as much for
-// one lane as for 64 (R-MAT-22 PPR: 73 ms as a batch of one, 11 ms plain),
+// one lane as for 64 (R-MAT-22 PPR: 44 ms as a batch of one, 8 ms plain),
 // while the plain engine's latency-bound supersteps leave the GPU room for
 // batch rounds beside them.
 constexpr double kSoloMaxMs = 2.0;
@@ -71,7 +71,7 @@ size_t query_bytes(int kind, int64_t V) {
   switch (kind) {
     case GAS_BATCH_BFS: return (size_t)(V * 2 + V * 3 / 8 + V * 4);
     case GAS_BATCH_SSSP: return (size_t)(V * 4 + V / 4 + V * 4);
-    default: return (size_t)(V * 16 + V / 4 + V * 4);  // PPR
+    default: return (size_t)(V * 16 + V * 3 / 64 + V * 4);  // PPR
   }
 }
 
