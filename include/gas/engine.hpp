@@ -231,8 +231,10 @@ class GASEngineGPU {
   void setActive(Int vertexStart, Int vertexEnd) {
     if (vertexStart < 0 || vertexEnd > m_nV || vertexEnd < vertexStart)
       throw Error(GAS_ERR_ARG, "setActive: range outside [0, nVertices)");
+    // all active: only the counts are written (one workgroup), no list
+    const bool all = vertexStart == 0 && vertexEnd == m_nV;
     const int64_t work = std::max<int64_t>(m_flags_dirty ? m_dev.nfwords : 0,
-                                           (int64_t)vertexEnd - vertexStart);
+                                           all ? 0 : (int64_t)vertexEnd - vertexStart);
     const unsigned grid =
         (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (work + NT - 1) / NT));
     m_dev.seq = (int32_t)m_seq;

@@ -1017,17 +1017,36 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
 }
 
 // Folds the tile partials of hubs whose edges cross tiles.
+constexpr int HFIX_LOADS = 16;
 template <class P>
 __global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
   if (frontier_size(d) == 0) return;
   const HubList<P> h = gather_hubs(d);
-  for (int s = blockIdx.x * NT + threadIdx.x; s < h.n; s += gridDim.x * NT) {
+  if (h.n == 0) return;
+  const int ntiles = (h.items + h.n + MP_NV - 1) / MP_NV;
+  // one thread per tile boundary t: the segment holding position t*MP_NV
+  // crosses it when it starts before; it is folded at its first crossing
+  // (it starts in tile t-1), so each crossing segment exactly once
+  for (int t = 1 + blockIdx.x * NT + threadIdx.x; t < ntiles;
+       t += gridDim.x * NT) {
+    const int s = h.part[t];
     const int st = h.off[s] + s;
+    const int t0 = t - 1;
+    if (st >= t * MP_NV || st < t0 * MP_NV) continue;
     const int en = h.off[s + 1] + s;
-    const int t0 = st / MP_NV, t1 = en / MP_NV;
-    if (t0 == t1) continue;
+    const int t1 = en / MP_NV;
     Acc<P> a = (st > t0 * MP_NV) ? d.part_last[t0] : d.part_first[t0];
-    for (int t = t0 + 1; t <= t1; ++t) a.merge(d.part_first[t]);
+    // a hub of many tiles: loads issued HFIX_LOADS at a time, merged in
+    // tile order
+    for (int u = t0 + 1; u <= t1; u += HFIX_LOADS) {
+      Acc<P> b[HFIX_LOADS];
+#pragma unroll
+      for (int j = 0; j < HFIX_LOADS; ++j)
+        b[j] = u + j <= t1 ? d.part_first[u + j] : Acc<P>::empty();
+#pragma unroll
+      for (int j = 0; j < HFIX_LOADS; ++j)
+        if (u + j <= t1) a.merge(b[j]);
+    }
     d.gtmp[h.list[s]] = a.finish();
   }
 }
