@@ -129,12 +129,41 @@ int gas_gen_road(int32_t rows, int32_t cols, double drop_frac,
 int gas_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
                  double c, uint64_t seed, int symmetrize, int32_t wmin,
                  int32_t wmax, gas_edges_t *out);
-/* MatrixMarket coordinate loader with the reference's rules
+/* Graph files (replace loadGraph and its loaders / writers,
+ * graphio.cpp:178-473).  Every file may be gzip-compressed ("x.mtx.gz").
+ * want_values: 1 as the reference's edgeValues argument, 0 for none, -1
+ * for the values the file holds, if any (the converters' use).
+ *
+ * MatrixMarket coordinate loader with the reference's rules
  * (graphio.cpp:69-262): n_vertices = max id + 1, self loops dropped,
  * symmetric files expanded to both directions, values read as float and
  * stored as int.  want_values mirrors passing edgeValues (sssp.cu:112):
  * with it a self loop does not count toward max id, as in the reference. */
 int gas_load_mtx(const char *path, int want_values, gas_edges_t *out);
+/* GraphLab SNAP edge list (loadGraph_GraphLabSnap, graphio.cpp:178-188):
+ * '#' comments, 0-based "src dst" lines, no values, self loops dropped
+ * (still counted toward max id). */
+int gas_load_edge(const char *path, gas_edges_t *out);
+/* Lonestar binary CSR version 1 (loadGraph_binaryCSR, graphio.cpp:266-362):
+ * u64 version, u64 edge-value bytes (0 or 4), u64 vertices, u64 edges,
+ * u64 end offset per vertex, u32 destination per edge (+ 4 bytes of padding
+ * when the count is odd), u32 value per edge; expanded to an edge list in
+ * CSR order.  want_values on a file without values fails as in the
+ * reference; unlike it (graphio.cpp:305-309), a file without values loads
+ * when none are wanted. */
+int gas_load_gr(const char *path, int want_values, gas_edges_t *out);
+/* Writes the list as Lonestar binary CSR (writeGraph_binaryCSR,
+ * graphio.cpp:365-414; with its values if it has them), each source's
+ * edges in list order (mtx2gr.cpp:41-51). */
+int gas_write_gr(const char *path, gas_edges_t edges);
+/* Writes the list as MatrixMarket (writeGraph_mtx, graphio.cpp:416-439):
+ * the reference's banner "%MatrixMarket matrix coordinate Integer general"
+ * (one '%'; "pattern" for a list without values), "V V E", then 1-based
+ * "src dst [value]" lines in list order. */
+int gas_write_mtx(const char *path, gas_edges_t edges);
+/* loadGraph (graphio.cpp:441-473): by extension, ".edge", ".mtx" or ".gr",
+ * after stripping a final ".gz". */
+int gas_load_graph(const char *path, int want_values, gas_edges_t *out);
 /* The reference's -m option (bfs.cu:148-166): vertex of maximum out-degree,
  * lowest id on ties. */
 int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,

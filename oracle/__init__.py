@@ -8,7 +8,9 @@ this package, and only as the checker.  It binds:
                       independent checkers;
   _ref/librefgas.so   the reference's own refgas.h/util.h/util.cpp compiled
                       from its sources (absent when they were not available
-                      at build time).
+                      at build time);
+  _ref/librefgraphio.so  the reference's graphio.cpp (graph file loaders and
+                      writers), likewise.
 """
 from __future__ import annotations
 
@@ -21,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "_lib", "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "librefgas.so")
+REFIO_LIB = os.path.join(HERE, "_ref", "librefgraphio.so")
 SSSP_UNREACHED = 2147483647 - 100000
 
 
@@ -68,6 +71,56 @@ def ref() -> C.CDLL:
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+_io = None
+
+
+def refio_available() -> bool:
+    return os.path.exists(REFIO_LIB)
+
+
+def _refio() -> C.CDLL:
+    global _io
+    if _io is None:
+        _io = C.CDLL(REFIO_LIB)
+    return _io
+
+
+def refio_load(path: str, want_values: bool):
+    """The reference's loadGraph (graphio.cpp:441-473) on a well-formed
+    file: (n_vertices, srcs, dsts, values or None)."""
+    r = _refio()
+    nv, ne = C.c_int(), C.c_int()
+    ps, pd, pw = (C.POINTER(C.c_int)() for _ in range(3))
+    r.refio_load(path.encode(), C.c_int(1 if want_values else 0), C.byref(nv),
+                 C.byref(ne), C.byref(ps), C.byref(pd), C.byref(pw))
+    n = ne.value
+
+    def take(p):
+        if not p:
+            return None
+        a = np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].astype(np.int32)
+        r.refio_free(p)
+        return a
+
+    return nv.value, take(ps), take(pd), take(pw)
+
+
+def refio_write_gr(path: str, n_vertices: int, offsets, dsts, values) -> None:
+    """The reference's writeGraph_binaryCSR (graphio.cpp:365-414)."""
+    off, d = _i32(offsets), _i32(dsts)
+    w = None if values is None else _i32(values)
+    _refio().refio_write_gr(path.encode(), C.c_int(n_vertices), C.c_int(len(d)),
+                            _p(off), _p(d), _p(w) if w is not None else None)
+
+
+def refio_write_mtx(path: str, n_vertices: int, srcs, dsts, values) -> None:
+    """The reference's writeGraph_mtx (graphio.cpp:416-439)."""
+    s, d = _i32(srcs), _i32(dsts)
+    w = None if values is None else _i32(values)
+    _refio().refio_write_mtx(path.encode(), C.c_int(n_vertices), C.c_int(len(d)),
+                             _p(s), _p(d), _p(w) if w is not None else None)
 
 
 def _p(a, t=C.c_int32):

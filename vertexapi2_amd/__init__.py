@@ -145,6 +145,11 @@ def device_copy(dst: DeviceBuffer, src: DeviceBuffer) -> None:
     _check(lib().gasrt_stream_synchronize(None))
 
 
+def _want(want_values) -> int:
+    """Loader value flag: True / False, or -1 for whatever the file holds."""
+    return -1 if want_values == -1 else (1 if want_values else 0)
+
+
 class Edges:
     """Host edge list (gas_edges_t)."""
 
@@ -201,9 +206,39 @@ class Edges:
     @classmethod
     def load_mtx(cls, path: str, want_values: bool = False) -> "Edges":
         h = C.c_void_p()
-        _check(lib().gas_load_mtx(path.encode(), C.c_int(1 if want_values else 0),
+        _check(lib().gas_load_mtx(path.encode(), C.c_int(_want(want_values)),
                                   C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_edge(cls, path: str) -> "Edges":
+        """GraphLab SNAP edge list (0-based "src dst" lines, '#' comments)."""
+        h = C.c_void_p()
+        _check(lib().gas_load_edge(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_gr(cls, path: str, want_values: bool = False) -> "Edges":
+        """Lonestar binary CSR (.gr), expanded to an edge list in CSR order."""
+        h = C.c_void_p()
+        _check(lib().gas_load_gr(path.encode(), C.c_int(_want(want_values)),
+                                 C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load(cls, path: str, want_values: bool = False) -> "Edges":
+        """By extension (.edge, .mtx, .gr, optionally .gz), as loadGraph
+        (graphio.cpp:441-473)."""
+        h = C.c_void_p()
+        _check(lib().gas_load_graph(path.encode(), C.c_int(_want(want_values)),
+                                    C.byref(h)))
+        return cls(h)
+
+    def write_gr(self, path: str) -> None:
+        _check(lib().gas_write_gr(path.encode(), self._h))
+
+    def write_mtx(self, path: str) -> None:
+        _check(lib().gas_write_mtx(path.encode(), self._h))
 
     def view(self) -> EdgesView:
         v = EdgesView()

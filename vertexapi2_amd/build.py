@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libgas.so")
 
-SOURCES = ["gasrt.cpp", "graph.cpp", "apps.hip", "batched.hip", "server.cpp"]
+SOURCES = ["gasrt.cpp", "graph.cpp", "graphio.cpp", "apps.hip", "batched.hip", "server.cpp"]
 
 ARCH = os.environ.get("GAS_OFFLOAD_ARCH", "gfx950")
 
@@ -68,7 +68,7 @@ def build(verbose: bool = True) -> str:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-           "-lpthread"]
+           "-lpthread", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -1,5 +1,5 @@
-// Host graph toolkit: seeded synthetic generators, the MatrixMarket loader,
-// CSR/CSC construction and upload of a graph into HBM.
+// Host graph toolkit: seeded synthetic generators, CSR/CSC construction
+// and upload of a graph into HBM (graph files: graphio.cpp).
 //
 // CSR/CSC layout follows util.h:79-121 (offsets of V+1 entries, neighbour
 // ids sorted by key vertex, plus the original edge index of every slot);
@@ -361,91 +361,6 @@ int gas_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
       }
     });
   }
-  *out = e;
-  return GAS_OK;
-}
-
-int gas_load_mtx(const char *path, int want_values, gas_edges_t *out) {
-  if (!path || !out) return set_error(GAS_ERR_ARG, "gas_load_mtx: null");
-  std::ifstream f(path);
-  if (!f) return set_error(GAS_ERR_IO, "gas_load_mtx: cannot open %s", path);
-  std::string line;
-  if (!std::getline(f, line))
-    return set_error(GAS_ERR_IO, "gas_load_mtx: error reading header");
-  std::istringstream hs(line);
-  std::string banner, object, format, field, symmetry;
-  hs >> banner >> object >> format >> field >> symmetry;
-  if (object != "matrix")
-    return set_error(GAS_ERR_IO, "gas_load_mtx: unrecognized token %s",
-                     object.c_str());
-  if (format != "coordinate")
-    return set_error(GAS_ERR_IO,
-                     "gas_load_mtx: only coordinate format is supported");
-  bool values = want_values != 0;
-  if (field == "pattern") values = false;
-  if (field == "complex")
-    return set_error(GAS_ERR_IO, "gas_load_mtx: complex values unsupported");
-  enum { NONE, SYM, SKEW, HERM } st;
-  if (symmetry == "general") st = NONE;
-  else if (symmetry == "symmetric") st = SYM;
-  else if (symmetry == "skew-symmetric") st = SKEW;
-  else if (symmetry == "hermitian") st = HERM;
-  else
-    return set_error(GAS_ERR_IO, "gas_load_mtx: unrecognized symmetry '%s'",
-                     symmetry.c_str());
-  if (st == HERM && values)
-    return set_error(GAS_ERR_IO,
-                     "gas_load_mtx: hermitian values are not supported");
-  auto *e = new gas_edges;
-  int64_t max_vertex = 0;
-  bool first_data = true;
-  int64_t line_no = 1;
-  while (std::getline(f, line)) {
-    ++line_no;
-    size_t p = line.find_first_not_of(" \t\r\n");
-    if (p == std::string::npos) continue;  // blank line
-    if (line[0] == '%') continue;           // comment
-    if (first_data) {                       // "rows cols nnz"
-      first_data = false;
-      continue;
-    }
-    long s, d;
-    int nb = 0;
-    if (sscanf(line.c_str(), "%ld%ld%n", &s, &d, &nb) != 2) {
-      delete e;
-      return set_error(GAS_ERR_IO,
-                       "gas_load_mtx: error parsing src/dst at line %ld",
-                       (long)line_no);
-    }
-    --s;
-    --d;
-    if (s < 0 || d < 0 || s > INT32_MAX - 1 || d > INT32_MAX - 1) {
-      delete e;
-      return set_error(GAS_ERR_IO, "gas_load_mtx: bad vertex id at line %ld",
-                       (long)line_no);
-    }
-    if (s == d) {
-      // graphio.cpp:133-141: a self loop is dropped; with edge values the
-      // line is skipped before it can raise the maximum id.
-      if (values) continue;
-    } else {
-      e->src.push_back((int32_t)s);
-      e->dst.push_back((int32_t)d);
-      if (st != NONE) {
-        e->src.push_back((int32_t)d);
-        e->dst.push_back((int32_t)s);
-      }
-      if (values) {
-        float val = 0.f;
-        sscanf(line.c_str() + nb, "%f", &val);
-        e->w.push_back((int32_t)val);
-        if (st == SYM) e->w.push_back((int32_t)val);
-        if (st == SKEW) e->w.push_back((int32_t)(-val));
-      }
-    }
-    max_vertex = std::max(max_vertex, (int64_t)std::max(s, d));
-  }
-  e->nV = (int32_t)(max_vertex + 1);
   *out = e;
   return GAS_OK;
 }
