@@ -23,7 +23,11 @@
 //    kernel (both are per active vertex and nothing reads the activation
 //    flags before the superstep ends);
 //  * like the reference GPU engine, Program::scatter is not invoked
-//    (gpugas.h:666-668).
+//    (gpugas.h:666-668) -- unless the program declares
+//    `static constexpr bool scatterEdges = true`: then scatterActivate(true)
+//    and run() call it for every out-edge of each vertex whose apply
+//    returned true, as the reference CPU engine does (refgas.h:170-194), and
+//    getResults() also writes the edge data back to the caller's array.
 // Errors throw gas::Error carrying gas_last_error().
 #pragma once
 
@@ -177,6 +181,7 @@ class GASEngineGPU {
     if (m_own_graph) gas_graph_free(m_own_graph);
     m_own_graph = g;
     bindGraph(g, vertexData, edgeData);
+    m_ed_host = edgeData;
   }
 
   // Binds a graph already resident in HBM (shared between engines).
@@ -193,6 +198,16 @@ class GASEngineGPU {
       check(gasrt_memcpy_d2h(m_vd_host, m_dev.vdata,
                              sizeof(VertexData) * (size_t)m_nV, m_stream),
             "getResults");
+    if constexpr (scatters_edges<Program>::value && !kEmptyEdge) {
+      // edge data as the scatters left it, back in edge-list order
+      if (m_ed_host && m_nE > 0) {
+        std::vector<EdgeData> csc((size_t)m_nE);
+        check(gasrt_memcpy_d2h(csc.data(), m_edata.p,
+                               sizeof(EdgeData) * (size_t)m_nE, m_stream),
+              "getResults edges");
+        for (Int k = 0; k < m_nE; ++k) m_ed_host[m_csc_eid[(size_t)k]] = csc[(size_t)k];
+      }
+    }
   }
 
   // Forgets the caller's vertex-data array (getResults becomes a no-op
@@ -267,15 +282,30 @@ class GASEngineGPU {
 
   // Scatter of the step-by-step API: plain activation bytes, then the
   // ordered compaction, so the next frontier is an ascending list.
-  void scatterActivate(bool /*haveScatter*/ = true) {
+  void scatterActivate(bool haveScatter = true) {
     m_out_list = 0;
+    if constexpr (scatters_edges<Program>::value && !kEmptyEdge)
+      if (haveScatter && m_nE > 0)
+        klaunch(GAS_K_SCATTER, 1, k_edge_scatter<Program>, SCAN_BLOCKS);
     klaunch(GAS_K_SCATTER, 1, k_scatter_vp<Program>, SCAN_BLOCKS);
     endSuperstep(true);
   }
 
   Int nextIter() { return countActive(); }
 
-  void run() { runUntilEmpty(true, true); }
+  // refgas.h:215-222; programs that scatter to edges take the step-by-step
+  // supersteps (every scatter must see every apply of its superstep)
+  void run() {
+    if constexpr (scatters_edges<Program>::value && !kEmptyEdge) {
+      while (countActive()) {
+        gatherApply(true);
+        scatterActivate(true);
+        nextIter();
+      }
+    } else {
+      runUntilEmpty(true, true);
+    }
+  }
 
   // One whole superstep: gatherApply + scatterActivate, with the thread and
   // wave tiers of the activation fused into the apply kernel.  outList:
@@ -703,6 +733,21 @@ class GASEngineGPU {
                              sizeof(VertexData) * (size_t)m_nV, m_stream),
             "vertex data");
     const EdgeData *ed_dev = nullptr;
+    m_ed_host = nullptr;
+    if constexpr (scatters_edges<Program>::value && !kEmptyEdge) {
+      // scatters write the edge data: an engine-owned copy in CSC order,
+      // and the CSR slot -> CSC slot map of their out-edges
+      if (m_nE > 0 && !edgeData)
+        throw Error(GAS_ERR_ARG, "setGraph: program scatters to edge data it needs");
+      std::vector<int32_t> off((size_t)m_nV + 1), nb((size_t)m_nE), csr_eid((size_t)m_nE);
+      m_csc_eid.assign((size_t)m_nE, 0);
+      check(gas_graph_host_csc(g, off.data(), nb.data(), m_csc_eid.data()), "host csc");
+      check(gas_graph_host_csr(g, off.data(), nb.data(), csr_eid.data()), "host csr");
+      std::vector<int32_t> pos((size_t)m_nE), csr_csc((size_t)m_nE);
+      for (Int k = 0; k < m_nE; ++k) pos[(size_t)m_csc_eid[(size_t)k]] = k;
+      for (Int k = 0; k < m_nE; ++k) csr_csc[(size_t)k] = pos[(size_t)csr_eid[(size_t)k]];
+      m_csr_csc.upload(csr_csc, m_stream);
+    }
     if (!kEmptyEdge) {
       if (edgeData) {
         std::vector<EdgeData> sorted((size_t)m_nE);
@@ -774,6 +819,8 @@ class GASEngineGPU {
     d.csr_off = gv.csr_offsets;
     d.csr_dst = gv.csr_dsts;
     d.edata = ed_dev;
+    d.edata_rw = scatters_edges<Program>::value ? m_edata.p : nullptr;
+    d.csr_csc = m_csr_csc.p;
     d.vdata = m_vdata.p;
     d.vnext = m_vdata2.p;
     m_srcval.alloc(is_source_only<Program>::value ? (size_t)m_nV : 1);
@@ -908,6 +955,9 @@ class GASEngineGPU {
   std::vector<int32_t> m_h_csc_off;
   int m_par = 0;  // parity of the next superstep
   VertexData *m_vd_host = nullptr;
+  EdgeData *m_ed_host = nullptr;        // scatterEdges: caller's edge array
+  std::vector<int32_t> m_csc_eid;       // scatterEdges: edge-list index per CSC slot
+  DevBuf<int32_t> m_csr_csc;            // scatterEdges: CSR slot -> CSC slot
   DevBuf<VertexData> m_vdata, m_vdata2;  // state, double-buffered by k_fused
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;

@@ -139,6 +139,19 @@ template <class P>
 struct is_compensated<P, std::void_t<decltype(P::compensatedSum)>>
     : std::integral_constant<bool, P::compensatedSum> {};
 
+// Optional program trait: `static constexpr bool scatterEdges = true`
+// declares a Program::scatter that writes its edge.  The engine then calls
+// it for every out-edge of each vertex whose apply returned true, after
+// every apply of the superstep, whenever scatterActivate(true) runs (run()
+// included), as the reference CPU engine does (refgas.h:170-194).  Without
+// it scatter is never called, as in the reference GPU engine
+// (gpugas.h:666-668).
+template <class P, class = void>
+struct scatters_edges : std::false_type {};
+template <class P>
+struct scatters_edges<P, std::void_t<decltype(P::scatterEdges)>>
+    : std::integral_constant<bool, P::scatterEdges> {};
+
 // Optional program trait: `static constexpr bool gatherSourceOnly = true`
 // declares that gatherMap reads only its source vertex (not the
 // destination, not the edge).  The engine then keeps gatherMap(src) per
@@ -327,6 +340,8 @@ struct Dev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst;
   const ED *edata;  // CSC order
+  ED *edata_rw;     // the same array, written by scatterEdges programs
+  const int32_t *csr_csc;  // CSR slot -> CSC slot (scatterEdges programs)
   VD *vdata;        // current vertex state (read by gathers)
   VD *vnext;        // the other state buffer, written by k_fused's apply
   GR *srcval;       // gatherMap(v) per vertex (source-only programs)
@@ -1347,6 +1362,44 @@ __global__ __launch_bounds__(NT) void k_scatter_vp(Dev<P> d, int p) {
   });
   scattered = block_sum(scattered, lds);
   add_stat(d, 1, scattered);
+}
+
+// Program::scatter(src, dst, edge) over the out-edges of the slots whose
+// apply returned true (refgas.h:178-192), for scatterEdges programs.  An
+// edge is written only by its source's call, so no atomics; vertex data
+// is read after every apply of the superstep.  Out-degree <= WAVE: the
+// vertex's thread, edge by edge in CSR order; larger: the whole wave,
+// lanes striding over the edges.  Reads the input frontier without
+// clearing it (k_scatter_vp follows).
+template <class P>
+__global__ __launch_bounds__(NT) void k_edge_scatter(Dev<P> d, int p) {
+  __shared__ int32_t s_ids[BM_WORDS * 4];
+  __shared__ int lds[NT / WAVE];
+  if constexpr (scatters_edges<P>::value) {
+    const int lane = lane_id();
+    for_each_active(d, p, false, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
+      const bool ret = valid && d.rflag[i] != 0;
+      int so = 0, deg = 0;
+      if (ret) {
+        so = d.csr_off[v];
+        deg = d.csr_off[v + 1] - so;
+      }
+      if (ret && deg <= WAVE)
+        for (int k = so; k < so + deg; ++k)
+          P::scatter(d.vdata + v, d.vdata + d.csr_dst[k], d.edata_rw + d.csr_csc[k]);
+      uint64_t m = __ballot(ret && deg > WAVE);
+      while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const int vj = __shfl(v, j, WAVE);
+        const int oj = __shfl(so, j, WAVE);
+        const int dj = __shfl(deg, j, WAVE);
+        for (int k = lane; k < dj; k += WAVE)
+          P::scatter(d.vdata + vj, d.vdata + d.csr_dst[oj + k],
+                     d.edata_rw + d.csr_csc[oj + k]);
+      }
+    });
+  }
 }
 
 // Hub scatter: merge-path tiles over the scatter hubs' CSR out-edges.

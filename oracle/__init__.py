@@ -73,6 +73,18 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def ref_edge_trace(n, s, d, w, source):
+    """The reference engine's run() on the edge-scattering SSSP of
+    tests/template/edge_trace.hip: (dist, hits, last, supersteps)."""
+    s, d, w = _i32(s), _i32(d), _i32(w)
+    dist = np.zeros(n, np.int32)
+    hits = np.zeros(len(s), np.int32)
+    last = np.zeros(len(s), np.int32)
+    steps = ref().ref_edge_trace(C.c_int(n), C.c_int(len(s)), _p(s), _p(d), _p(w),
+                                 C.c_int(source), _p(dist), _p(hits), _p(last))
+    return dist, hits, last, steps
+
+
 _io = None
 
 

@@ -110,6 +110,41 @@ struct CC {
   static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
 };
 
+// SSSP whose scatter writes its edge: hits counts the scatters of the
+// edge, last the distance difference of its ends after the superstep's
+// applies (the test program of tests/template/edge_trace.hip, here for the
+// reference's own engine, whose run() calls scatter: refgas.h:178-192).
+struct EdgeTrace {
+  static const int kUnreached = INT_MAX - 100000;
+  struct VertexData {
+    int dist;
+  };
+  struct EdgeData {
+    int w;
+    int hits;
+    int last;
+  };
+  typedef int GatherResult;
+  static const int gatherZero = INT_MAX;
+  static int gatherReduce(const int &a, const int &b) { return std::min(a, b); }
+  static int gatherMap(const VertexData *, const VertexData *src,
+                       const EdgeData *e) {
+    return src->dist >= kUnreached ? INT_MAX : src->dist + e->w;
+  }
+  static bool apply(VertexData *v, GatherResult g) {
+    if (g < v->dist) {
+      v->dist = g;
+      return true;
+    }
+    return false;
+  }
+  static void scatter(const VertexData *src, const VertexData *dst,
+                      EdgeData *e) {
+    e->hits += 1;
+    e->last = src->dist - dst->dist;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -197,6 +232,33 @@ int64_t ref_pagerank(int nV, int nE, const int *srcs, const int *dsts,
   int64_t t1 = currentTime();
   for (int i = 0; i < nV; ++i) rank[i] = vd[i].rank;
   return t1 - t0;
+}
+
+// The reference engine's run() (refgas.h:215-222, scatter included) on
+// EdgeTrace, every vertex active first; returns the supersteps run.
+int ref_edge_trace(int nV, int nE, const int *srcs, const int *dsts,
+                   const int *w, int source, int *dist, int *hits, int *last) {
+  std::vector<EdgeTrace::VertexData> vd(nV);
+  for (int i = 0; i < nV; ++i) vd[i].dist = i == source ? 0 : EdgeTrace::kUnreached;
+  std::vector<EdgeTrace::EdgeData> ed(nE);
+  for (int i = 0; i < nE; ++i) ed[i] = {w[i], 0, 0};
+  GASEngineRef<EdgeTrace> engine;
+  engine.setGraph(nV, vd.data(), nE, ed.data(), srcs, dsts);
+  engine.setActive(0, nV);
+  int steps = 0;
+  while (engine.countActive()) {
+    engine.gatherApply();
+    engine.scatterActivate();
+    engine.nextIter();
+    ++steps;
+  }
+  engine.getResults();
+  for (int i = 0; i < nV; ++i) dist[i] = vd[i].dist;
+  for (int i = 0; i < nE; ++i) {
+    hits[i] = ed[i].hits;
+    last[i] = ed[i].last;
+  }
+  return steps;
 }
 
 // util.h:79-110, exposed for layout checks.
