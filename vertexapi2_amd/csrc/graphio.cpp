@@ -13,6 +13,8 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -85,7 +87,7 @@ bool blank(const std::string &s) {
 // directions (a skew-symmetric one negates the mirrored value).
 int read_edge_lines(GzFile &f, const TextRules &r, int64_t line_no,
                     gas_edges_t *out) {
-  auto *e = new gas_edges;
+  std::unique_ptr<gas_edges> e(new gas_edges);
   int64_t max_vertex = 0;
   bool size_line = r.skip_size_line;
   std::string line;
@@ -100,7 +102,6 @@ int read_edge_lines(GzFile &f, const TextRules &r, int64_t line_no,
     long s, d;
     int nb = 0;
     if (sscanf(line.c_str(), "%ld%ld%n", &s, &d, &nb) != 2) {
-      delete e;
       return set_error(GAS_ERR_IO, "%s: error parsing src/dst at line %ld",
                        r.who, (long)line_no);
     }
@@ -109,7 +110,6 @@ int read_edge_lines(GzFile &f, const TextRules &r, int64_t line_no,
       --d;
     }
     if (s < 0 || d < 0 || s > INT32_MAX - 1 || d > INT32_MAX - 1) {
-      delete e;
       return set_error(GAS_ERR_IO, "%s: bad vertex id at line %ld", r.who,
                        (long)line_no);
     }
@@ -133,12 +133,11 @@ int read_edge_lines(GzFile &f, const TextRules &r, int64_t line_no,
     max_vertex = std::max(max_vertex, (int64_t)std::max(s, d));
   }
   if (err) {
-    delete e;
     return set_error(GAS_ERR_IO, "%s: read error after line %ld", r.who,
                      (long)line_no);
   }
   e->nV = (int32_t)(max_vertex + 1);
-  *out = e;
+  *out = e.release();
   return GAS_OK;
 }
 
@@ -169,12 +168,25 @@ bool has_suffix(const char *path, const char *ext) {
   return strncmp(dot, ext, strlen(ext)) == 0;
 }
 
+// Host allocation failures and other exceptions become error codes at the
+// C boundary.
+template <class F>
+int io_guard(const char *who, F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return set_error(GAS_ERR_NOMEM, "%s: out of host memory", who);
+  } catch (const std::exception &x) {
+    return set_error(GAS_ERR_IO, "%s: %s", who, x.what());
+  }
+}
+
 }  // namespace
 }  // namespace gas
 
 using namespace gas;
 
-int gas_load_mtx(const char *path, int want_values, gas_edges_t *out) {
+static int load_mtx_impl(const char *path, int want_values, gas_edges_t *out) {
   if (!path || !out) return set_error(GAS_ERR_ARG, "gas_load_mtx: null");
   GzFile f(path);
   if (!f.f) return set_error(GAS_ERR_IO, "gas_load_mtx: cannot open %s", path);
@@ -208,7 +220,7 @@ int gas_load_mtx(const char *path, int want_values, gas_edges_t *out) {
   return read_edge_lines(f, r, 1, out);
 }
 
-int gas_load_edge(const char *path, gas_edges_t *out) {
+static int load_edge_impl(const char *path, gas_edges_t *out) {
   if (!path || !out) return set_error(GAS_ERR_ARG, "gas_load_edge: null");
   GzFile f(path);
   if (!f.f) return set_error(GAS_ERR_IO, "gas_load_edge: cannot open %s", path);
@@ -218,7 +230,7 @@ int gas_load_edge(const char *path, gas_edges_t *out) {
   return read_edge_lines(f, r, 0, out);
 }
 
-int gas_load_gr(const char *path, int want_values, gas_edges_t *out) {
+static int load_gr_impl(const char *path, int want_values, gas_edges_t *out) {
   if (!path || !out) return set_error(GAS_ERR_ARG, "gas_load_gr: null");
   GzFile f(path);
   if (!f.f) return set_error(GAS_ERR_IO, "gas_load_gr: cannot open %s", path);
@@ -235,7 +247,7 @@ int gas_load_gr(const char *path, int want_values, gas_edges_t *out) {
     return set_error(GAS_ERR_IO, "gas_load_gr: %llu vertices / %llu edges exceed int32",
                      (unsigned long long)nV, (unsigned long long)nE);
   std::vector<uint64_t> ends((size_t)nV);
-  auto *e = new gas_edges;
+  std::unique_ptr<gas_edges> e(new gas_edges);
   e->nV = (int32_t)nV;
   e->src.resize((size_t)nE);
   e->dst.resize((size_t)nE);
@@ -248,7 +260,6 @@ int gas_load_gr(const char *path, int want_values, gas_edges_t *out) {
     ok = nE == 0 || f.read(e->w.data(), 4 * (size_t)nE);
   }
   if (!ok) {
-    delete e;
     return set_error(GAS_ERR_IO, "gas_load_gr: error reading lonestar binary CSR file %s",
                      path);
   }
@@ -257,27 +268,24 @@ int gas_load_gr(const char *path, int want_values, gas_edges_t *out) {
   uint64_t j = 0;
   for (uint64_t v = 0; v < nV; ++v) {
     if (ends[v] < j || ends[v] > nE) {
-      delete e;
       return set_error(GAS_ERR_IO, "gas_load_gr: bad offset of vertex %llu",
                        (unsigned long long)v);
     }
     for (; j < ends[v]; ++j) e->src[j] = (int32_t)v;
   }
   if (j != nE) {
-    delete e;
     return set_error(GAS_ERR_IO, "gas_load_gr: offsets end at %llu of %llu edges",
                      (unsigned long long)j, (unsigned long long)nE);
   }
   for (int32_t d : e->dst)
     if (d < 0 || (uint64_t)d >= nV) {
-      delete e;
       return set_error(GAS_ERR_IO, "gas_load_gr: destination %d out of range", d);
     }
-  *out = e;
+  *out = e.release();
   return GAS_OK;
 }
 
-int gas_write_gr(const char *path, gas_edges_t e) {
+static int write_gr_impl(const char *path, gas_edges_t e) {
   if (!path || !e) return set_error(GAS_ERR_ARG, "gas_write_gr: null");
   std::vector<int64_t> off;
   std::vector<int32_t> order;
@@ -307,7 +315,7 @@ int gas_write_gr(const char *path, gas_edges_t e) {
   return ok ? GAS_OK : set_error(GAS_ERR_IO, "gas_write_gr: short write to %s", path);
 }
 
-int gas_write_mtx(const char *path, gas_edges_t e) {
+static int write_mtx_impl(const char *path, gas_edges_t e) {
   if (!path || !e) return set_error(GAS_ERR_ARG, "gas_write_mtx: null");
   FILE *f = fopen(path, "w");
   if (!f) return set_error(GAS_ERR_IO, "gas_write_mtx: unable to write to file %s", path);
@@ -333,4 +341,24 @@ int gas_load_graph(const char *path, int want_values, gas_edges_t *out) {
   if (has_suffix(path, ".gr")) return gas_load_gr(path, want_values, out);
   return set_error(GAS_ERR_IO, "gas_load_graph: unrecognized filetype extension of %s",
                    path);
+}
+
+int gas_load_mtx(const char *path, int want_values, gas_edges_t *out) {
+  return io_guard("gas_load_mtx", [&] { return load_mtx_impl(path, want_values, out); });
+}
+
+int gas_load_edge(const char *path, gas_edges_t *out) {
+  return io_guard("gas_load_edge", [&] { return load_edge_impl(path, out); });
+}
+
+int gas_load_gr(const char *path, int want_values, gas_edges_t *out) {
+  return io_guard("gas_load_gr", [&] { return load_gr_impl(path, want_values, out); });
+}
+
+int gas_write_gr(const char *path, gas_edges_t e) {
+  return io_guard("gas_write_gr", [&] { return write_gr_impl(path, e); });
+}
+
+int gas_write_mtx(const char *path, gas_edges_t e) {
+  return io_guard("gas_write_mtx", [&] { return write_mtx_impl(path, e); });
 }
