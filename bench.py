@@ -382,7 +382,7 @@ def served_lines(graphs, rates=(100, 1000, 5000, 20000), mix=(0.5, 0.3, 0.2)):
                            "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
                            "p99_ms": round(float(np.percentile(lat_p, 99)), 3)},
             "qps_vs_plain": round(res["qps"] / (n / end_p), 2)}
-        log(f"served rate {rate} done")
+        log(f"served rate {rate} ({out['mix']}) done")
     return out
 
 
