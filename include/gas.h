@@ -184,9 +184,18 @@ typedef struct {
   size_t device_bytes;
 } gas_graph_view;
 
-/* Builds CSC and CSR on the host (stable counting sort by key, the layout
- * of util.h:79-121) and uploads them. */
+/* Builds CSC and CSR (the layout of util.h:79-121: offsets, neighbour ids
+ * grouped by key vertex, equal keys in edge-list order) and keeps them in
+ * HBM, with host copies.  Lists of at least the device-build threshold of
+ * edges are sorted on the GPU (stable radix sort of (key, edge index)),
+ * smaller ones on the host (stable counting sort); both give the same
+ * layout. */
 int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
+/* Process-wide: edge lists of at least min_edges edges (default 2^20) are
+ * laid out on the GPU by gas_graph_create, smaller ones on the host; 0
+ * builds every graph on the GPU.  No counterpart in the reference, which
+ * sorts on the host (util.h:68). */
+int gas_graph_set_device_build(int64_t min_edges);
 int gas_graph_view_get(gas_graph_t g, gas_graph_view *view);
 /* Host copies for checking: arrays of V+1 / E int32 (any may be NULL). */
 int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,

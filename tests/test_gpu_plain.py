@@ -316,3 +316,45 @@ def test_device_result_entry_points(gpu):
     assert np.array_equal(rbuf.to_numpy().view(np.float64), res)
     g.ppr_device(src, buf.ptr, 10)
     assert np.array_equal(buf.to_numpy().view(np.float32), g.ppr(src, 10)[0])
+
+
+def _layout(g):
+    v = g.view()
+    n, m = v.n_vertices, v.n_edges
+    dev = [gas.download_i32(p, k) for p, k in (
+        (v.csc_offsets, n + 1), (v.csc_srcs, m), (v.csr_offsets, n + 1), (v.csr_dsts, m))]
+    if v.csc_weights:
+        dev += [gas.download_i32(v.csc_weights, m), gas.download_i32(v.csr_weights, m)]
+    return list(g.host_csc()) + list(g.host_csr()) + dev
+
+
+def test_device_graph_build_matches_host(gpu):
+    # the GPU radix-sort layout equals the host counting sort, equal keys
+    # in edge-list order: repeats, self loops, empty vertices, weights,
+    # no edges at all
+    rng = np.random.default_rng(21)
+    cases = []
+    n, m = 5000, 200000
+    s = rng.integers(0, n, m)
+    d = rng.integers(0, n, m)
+    cases.append((n, s, d, rng.integers(1, 255, m)))
+    n = 100000
+    cases.append((n, rng.integers(0, n, 3000), rng.integers(0, n, 3000), None))
+    cases.append((7, np.array([3, 3, 3, 0, 6, 6]), np.array([3, 1, 1, 6, 0, 0]),
+                  np.array([5, 4, 3, 2, 1, 0])))
+    cases.append((1, np.zeros(0, np.int32), np.zeros(0, np.int32), None))
+    el = workloads.road(64, 64, seed=2)
+    s, d, w = el.arrays()
+    cases.append((el.n_vertices, s, d, w))
+    try:
+        for n, s, d, w in cases:
+            e = gas.Edges.from_arrays(n, s, d, w)
+            gas.set_device_build(1 << 62)
+            host = _layout(gas.Graph(e))
+            gas.set_device_build(0)
+            dev = _layout(gas.Graph(e))
+            assert len(host) == len(dev)
+            for a, b in zip(host, dev):
+                assert np.array_equal(a, b), (n, len(s))
+    finally:
+        gas.set_device_build(1 << 20)

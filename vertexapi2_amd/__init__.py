@@ -116,6 +116,21 @@ def set_kernel_timing(enable: bool) -> None:
     _check(lib().gas_set_kernel_timing(1 if enable else 0))
 
 
+def set_device_build(min_edges: int) -> None:
+    """Graphs of at least `min_edges` edges get their CSC / CSR laid out on
+    the GPU (default 2**20; 0: every graph)."""
+    _check(lib().gas_graph_set_device_build(C.c_int64(min_edges)))
+
+
+def download_i32(ptr: int, n: int) -> np.ndarray:
+    """n int32 values from device memory (synchronous)."""
+    out = np.zeros(max(n, 0), np.int32)
+    if n > 0:
+        _check(lib().gasrt_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                                      C.c_size_t(4 * n), None))
+    return out
+
+
 class DeviceBuffer:
     """Device memory from gasrt_malloc (freed with the object)."""
 

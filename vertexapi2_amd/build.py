@@ -22,7 +22,8 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libgas.so")
 
-SOURCES = ["gasrt.cpp", "graph.cpp", "graphio.cpp", "apps.hip", "batched.hip", "server.cpp"]
+SOURCES = ["gasrt.cpp", "graph.cpp", "graph_build.hip", "graphio.cpp", "apps.hip", "batched.hip",
+           "server.cpp"]
 
 ARCH = os.environ.get("GAS_OFFLOAD_ARCH", "gfx950")
 

@@ -394,11 +394,92 @@ static void graph_release(gas_graph *g) {
   gasrt_free(g->csr_w);
 }
 
+static int64_t g_device_build_min_edges = (int64_t)1 << 20;
+
+int gas_graph_set_device_build(int64_t min_edges) {
+  if (min_edges < 0)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_device_build: negative edge count");
+  g_device_build_min_edges = min_edges;
+  return GAS_OK;
+}
+
+// CSC and CSR built on the device (graph_build.hip), then the host copies
+// of the layout downloaded.
+static int device_create(gas_edges_t edges, gas_graph *g) {
+  const int64_t E = g->nE;
+  const int32_t V = g->nV;
+  const size_t vb = ((size_t)V + 1) * 4, eb = (size_t)std::max<int64_t>(E, 1) * 4;
+  const bool wt = !edges->w.empty();
+  int32_t *d_src = nullptr, *d_dst = nullptr, *d_w = nullptr, *d_eid = nullptr;
+  uint32_t *d_keys = nullptr;
+  void *d_temp = nullptr;
+  const size_t temp_bytes = gas::device_build_temp_bytes(E);
+  int rc = 0;
+  auto dmalloc = [&](void **p, size_t bytes) {
+    if (!rc) rc = gasrt_malloc(p, bytes);
+  };
+  dmalloc((void **)&g->csc_off, vb);
+  dmalloc((void **)&g->csc_src, eb);
+  dmalloc((void **)&g->csr_off, vb);
+  dmalloc((void **)&g->csr_dst, eb);
+  if (wt) {
+    dmalloc((void **)&g->csc_w, eb);
+    dmalloc((void **)&g->csr_w, eb);
+  }
+  g->device_bytes = 2 * vb + (wt ? 4 : 2) * (size_t)E * 4;
+  dmalloc((void **)&d_src, eb);
+  dmalloc((void **)&d_dst, eb);
+  if (wt) dmalloc((void **)&d_w, eb);
+  dmalloc((void **)&d_eid, eb);
+  dmalloc((void **)&d_keys, eb);
+  dmalloc(&d_temp, std::max<size_t>(temp_bytes, 4));
+  if (!rc && E) rc = gasrt_memcpy_h2d(d_src, edges->src.data(), (size_t)E * 4, nullptr);
+  if (!rc && E) rc = gasrt_memcpy_h2d(d_dst, edges->dst.data(), (size_t)E * 4, nullptr);
+  if (!rc && E && wt) rc = gasrt_memcpy_h2d(d_w, edges->w.data(), (size_t)E * 4, nullptr);
+  auto layout = [&](const int32_t *keys, const int32_t *vals, int32_t *off,
+                    int32_t *out_vals, int32_t *out_w, std::vector<int32_t> &h_off,
+                    std::vector<int32_t> &h_vals, std::vector<int32_t> &h_eid) {
+    if (rc) return;
+    rc = gas::device_build_layout(V, E, keys, vals, d_w, off, out_vals, d_eid, out_w,
+                                  d_keys, d_temp, temp_bytes);
+    h_off.resize((size_t)V + 1);
+    h_vals.resize((size_t)E);
+    h_eid.resize((size_t)E);
+    if (!rc) rc = gasrt_memcpy_d2h(h_off.data(), off, vb, nullptr);
+    if (!rc && E) rc = gasrt_memcpy_d2h(h_vals.data(), out_vals, (size_t)E * 4, nullptr);
+    if (!rc && E) rc = gasrt_memcpy_d2h(h_eid.data(), d_eid, (size_t)E * 4, nullptr);
+  };
+  layout(d_dst, d_src, g->csc_off, g->csc_src, g->csc_w, g->h_csc_off, g->h_csc_src,
+         g->h_csc_eid);
+  layout(d_src, d_dst, g->csr_off, g->csr_dst, g->csr_w, g->h_csr_off, g->h_csr_dst,
+         g->h_csr_eid);
+  if (!rc) rc = gasrt_stream_synchronize(nullptr);
+  gasrt_free(d_src);
+  gasrt_free(d_dst);
+  gasrt_free(d_w);
+  gasrt_free(d_eid);
+  gasrt_free(d_keys);
+  gasrt_free(d_temp);
+  if (!rc && wt)
+    for (int32_t x : edges->w) g->max_w = std::max(g->max_w, x);
+  return rc;
+}
+
 int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
   if (!edges || !out) return set_error(GAS_ERR_ARG, "gas_graph_create: null");
   auto *g = new gas_graph;
   g->nV = edges->nV;
   g->nE = (int32_t)edges->src.size();
+  if ((int64_t)g->nE >= g_device_build_min_edges) {
+    const int rc = device_create(edges, g);
+    if (rc) {
+      graph_release(g);
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return GAS_OK;
+  }
   gas::build_compressed(g->nV, g->nE, edges->dst.data(), edges->src.data(),
                         g->h_csc_off, g->h_csc_src, g->h_csc_eid);
   gas::build_compressed(g->nV, g->nE, edges->src.data(), edges->dst.data(),
