@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <thread>
 #include <unordered_set>
@@ -131,6 +132,19 @@ static void dedupe_csr(int32_t nV, std::vector<int32_t> &off,
   adj.swap(nadj);
 }
 
+// Host allocation failures and other exceptions become error codes at the
+// C boundary.
+template <class F>
+int host_guard(const char *who, F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return set_error(GAS_ERR_NOMEM, "%s: out of host memory", who);
+  } catch (const std::exception &x) {
+    return set_error(GAS_ERR_STATE, "%s: %s", who, x.what());
+  }
+}
+
 }  // namespace gas
 
 using gas::set_error;
@@ -140,24 +154,26 @@ extern "C" {
 int gas_edges_from_arrays(int32_t n_vertices, int32_t n_edges,
                           const int32_t *srcs, const int32_t *dsts,
                           const int32_t *weights, gas_edges_t *out) {
-  if (!out || n_vertices < 0 || n_edges < 0 ||
-      (n_edges && (!srcs || !dsts)))
-    return set_error(GAS_ERR_ARG, "gas_edges_from_arrays: bad arguments");
-  for (int32_t k = 0; k < n_edges; ++k) {
-    if (srcs[k] < 0 || srcs[k] >= n_vertices || dsts[k] < 0 ||
-        dsts[k] >= n_vertices)
-      return set_error(GAS_ERR_ARG,
-                       "gas_edges_from_arrays: edge %d (%d -> %d) outside "
-                       "[0, %d)",
-                       k, srcs[k], dsts[k], n_vertices);
-  }
-  auto *e = new gas_edges;
-  e->nV = n_vertices;
-  e->src.assign(srcs, srcs + n_edges);
-  e->dst.assign(dsts, dsts + n_edges);
-  if (weights) e->w.assign(weights, weights + n_edges);
-  *out = e;
-  return GAS_OK;
+  return gas::host_guard("gas_edges_from_arrays", [&]() -> int {
+    if (!out || n_vertices < 0 || n_edges < 0 ||
+        (n_edges && (!srcs || !dsts)))
+      return set_error(GAS_ERR_ARG, "gas_edges_from_arrays: bad arguments");
+    for (int32_t k = 0; k < n_edges; ++k) {
+      if (srcs[k] < 0 || srcs[k] >= n_vertices || dsts[k] < 0 ||
+          dsts[k] >= n_vertices)
+        return set_error(GAS_ERR_ARG,
+                         "gas_edges_from_arrays: edge %d (%d -> %d) outside "
+                         "[0, %d)",
+                         k, srcs[k], dsts[k], n_vertices);
+    }
+    auto *e = new gas_edges;
+    e->nV = n_vertices;
+    e->src.assign(srcs, srcs + n_edges);
+    e->dst.assign(dsts, dsts + n_edges);
+    if (weights) e->w.assign(weights, weights + n_edges);
+    *out = e;
+    return GAS_OK;
+  });
 }
 
 int gas_edges_view_get(gas_edges_t e, gas_edges_view *view) {
@@ -176,210 +192,218 @@ int gas_edges_free(gas_edges_t e) {
 }
 
 int gas_gen_grid(int32_t rows, int32_t cols, gas_edges_t *out) {
-  if (!out || rows <= 0 || cols <= 0 || (int64_t)rows * cols > INT32_MAX)
-    return set_error(GAS_ERR_ARG, "gas_gen_grid: bad size %d x %d", rows,
-                     cols);
-  auto *e = new gas_edges;
-  e->nV = rows * cols;
-  int64_t links = (int64_t)rows * (cols - 1) + (int64_t)(rows - 1) * cols;
-  e->src.reserve(2 * links);
-  e->dst.reserve(2 * links);
-  for (int32_t r = 0; r < rows; ++r)
-    for (int32_t c = 0; c < cols; ++c) {
-      int32_t v = r * cols + c;
-      if (c + 1 < cols) {
-        e->src.push_back(v); e->dst.push_back(v + 1);
-        e->src.push_back(v + 1); e->dst.push_back(v);
+  return gas::host_guard("gas_gen_grid", [&]() -> int {
+    if (!out || rows <= 0 || cols <= 0 || (int64_t)rows * cols > INT32_MAX)
+      return set_error(GAS_ERR_ARG, "gas_gen_grid: bad size %d x %d", rows,
+                       cols);
+    auto *e = new gas_edges;
+    e->nV = rows * cols;
+    int64_t links = (int64_t)rows * (cols - 1) + (int64_t)(rows - 1) * cols;
+    e->src.reserve(2 * links);
+    e->dst.reserve(2 * links);
+    for (int32_t r = 0; r < rows; ++r)
+      for (int32_t c = 0; c < cols; ++c) {
+        int32_t v = r * cols + c;
+        if (c + 1 < cols) {
+          e->src.push_back(v); e->dst.push_back(v + 1);
+          e->src.push_back(v + 1); e->dst.push_back(v);
+        }
+        if (r + 1 < rows) {
+          e->src.push_back(v); e->dst.push_back(v + cols);
+          e->src.push_back(v + cols); e->dst.push_back(v);
+        }
       }
-      if (r + 1 < rows) {
-        e->src.push_back(v); e->dst.push_back(v + cols);
-        e->src.push_back(v + cols); e->dst.push_back(v);
-      }
-    }
-  e->w.assign(e->src.size(), 1);
-  *out = e;
-  return GAS_OK;
+    e->w.assign(e->src.size(), 1);
+    *out = e;
+    return GAS_OK;
+  });
 }
 
 int gas_gen_road(int32_t rows, int32_t cols, double drop_frac,
                  double long_frac, int32_t wmin, int32_t wmax, uint64_t seed,
                  gas_edges_t *out) {
-  if (!out || rows <= 1 || cols <= 1 || (int64_t)rows * cols > INT32_MAX ||
-      drop_frac < 0 || drop_frac > 1 || long_frac < 0 || wmin < 1 ||
-      wmax < wmin)
-    return set_error(GAS_ERR_ARG, "gas_gen_road: bad arguments");
-  const int64_t V = (int64_t)rows * cols;
-  const int64_t H = (int64_t)rows * (cols - 1);   // horizontal links
-  const int64_t L = H + (int64_t)(rows - 1) * cols;  // all grid links
-  const int64_t n_drop = (int64_t)std::floor(drop_frac * (double)L);
-  const int64_t n_long = (int64_t)std::floor(long_frac * (double)L);
-  if (n_long > V * (V - 1) / 2 - L)
-    return set_error(GAS_ERR_ARG, "gas_gen_road: too many long links");
-  gas::Xoshiro256 rng(seed);
-  // Partial Fisher-Yates: the first n_drop slots are the dropped links.
-  std::vector<int64_t> perm(L);
-  for (int64_t i = 0; i < L; ++i) perm[i] = i;
-  for (int64_t i = 0; i < n_drop; ++i) {
-    int64_t j = i + (int64_t)rng.below((uint64_t)(L - i));
-    std::swap(perm[i], perm[j]);
-  }
-  std::vector<uint8_t> dropped(L, 0);
-  for (int64_t i = 0; i < n_drop; ++i) dropped[perm[i]] = 1;
-  perm.clear();
-  perm.shrink_to_fit();
-
-  auto link_ends = [&](int64_t l, int32_t &u, int32_t &v) {
-    if (l < H) {
-      int64_t r = l / (cols - 1), c = l % (cols - 1);
-      u = (int32_t)(r * cols + c);
-      v = u + 1;
-    } else {
-      int64_t k = l - H;
-      u = (int32_t)k;  // (r, c) with r < rows-1, linear id r*cols + c
-      v = u + cols;
+  return gas::host_guard("gas_gen_road", [&]() -> int {
+    if (!out || rows <= 1 || cols <= 1 || (int64_t)rows * cols > INT32_MAX ||
+        drop_frac < 0 || drop_frac > 1 || long_frac < 0 || wmin < 1 ||
+        wmax < wmin)
+      return set_error(GAS_ERR_ARG, "gas_gen_road: bad arguments");
+    const int64_t V = (int64_t)rows * cols;
+    const int64_t H = (int64_t)rows * (cols - 1);   // horizontal links
+    const int64_t L = H + (int64_t)(rows - 1) * cols;  // all grid links
+    const int64_t n_drop = (int64_t)std::floor(drop_frac * (double)L);
+    const int64_t n_long = (int64_t)std::floor(long_frac * (double)L);
+    if (n_long > V * (V - 1) / 2 - L)
+      return set_error(GAS_ERR_ARG, "gas_gen_road: too many long links");
+    gas::Xoshiro256 rng(seed);
+    // Partial Fisher-Yates: the first n_drop slots are the dropped links.
+    std::vector<int64_t> perm(L);
+    for (int64_t i = 0; i < L; ++i) perm[i] = i;
+    for (int64_t i = 0; i < n_drop; ++i) {
+      int64_t j = i + (int64_t)rng.below((uint64_t)(L - i));
+      std::swap(perm[i], perm[j]);
     }
-  };
-  auto grid_adjacent = [&](int64_t a, int64_t b) {
-    if (a > b) std::swap(a, b);
-    if (b - a == cols) return true;
-    return b - a == 1 && (a % cols) != cols - 1;
-  };
+    std::vector<uint8_t> dropped(L, 0);
+    for (int64_t i = 0; i < n_drop; ++i) dropped[perm[i]] = 1;
+    perm.clear();
+    perm.shrink_to_fit();
 
-  auto *e = new gas_edges;
-  e->nV = (int32_t)V;
-  size_t m = 2 * (size_t)(L - n_drop + n_long);
-  e->src.reserve(m);
-  e->dst.reserve(m);
-  e->w.reserve(m);
-  const int32_t span = wmax - wmin + 1;
-  auto emit = [&](int32_t u, int32_t v) {
-    uint32_t a = (uint32_t)std::min(u, v), b = (uint32_t)std::max(u, v);
-    int32_t w = wmin + (int32_t)(gas::pair_hash(seed, a, b) % (uint64_t)span);
-    e->src.push_back(u); e->dst.push_back(v); e->w.push_back(w);
-    e->src.push_back(v); e->dst.push_back(u); e->w.push_back(w);
-  };
-  for (int64_t l = 0; l < L; ++l) {
-    if (dropped[l]) continue;
-    int32_t u, v;
-    link_ends(l, u, v);
-    emit(u, v);
-  }
-  std::unordered_set<uint64_t> added;
-  added.reserve((size_t)n_long * 2);
-  int64_t n_added = 0;
-  while (n_added < n_long) {
-    int64_t u = (int64_t)rng.below((uint64_t)V);
-    int64_t v = (int64_t)rng.below((uint64_t)V);
-    if (u == v || grid_adjacent(u, v)) continue;
-    uint64_t key = ((uint64_t)std::min(u, v) << 32) | (uint64_t)std::max(u, v);
-    if (!added.insert(key).second) continue;
-    emit((int32_t)u, (int32_t)v);
-    ++n_added;
-  }
-  *out = e;
-  return GAS_OK;
+    auto link_ends = [&](int64_t l, int32_t &u, int32_t &v) {
+      if (l < H) {
+        int64_t r = l / (cols - 1), c = l % (cols - 1);
+        u = (int32_t)(r * cols + c);
+        v = u + 1;
+      } else {
+        int64_t k = l - H;
+        u = (int32_t)k;  // (r, c) with r < rows-1, linear id r*cols + c
+        v = u + cols;
+      }
+    };
+    auto grid_adjacent = [&](int64_t a, int64_t b) {
+      if (a > b) std::swap(a, b);
+      if (b - a == cols) return true;
+      return b - a == 1 && (a % cols) != cols - 1;
+    };
+
+    auto *e = new gas_edges;
+    e->nV = (int32_t)V;
+    size_t m = 2 * (size_t)(L - n_drop + n_long);
+    e->src.reserve(m);
+    e->dst.reserve(m);
+    e->w.reserve(m);
+    const int32_t span = wmax - wmin + 1;
+    auto emit = [&](int32_t u, int32_t v) {
+      uint32_t a = (uint32_t)std::min(u, v), b = (uint32_t)std::max(u, v);
+      int32_t w = wmin + (int32_t)(gas::pair_hash(seed, a, b) % (uint64_t)span);
+      e->src.push_back(u); e->dst.push_back(v); e->w.push_back(w);
+      e->src.push_back(v); e->dst.push_back(u); e->w.push_back(w);
+    };
+    for (int64_t l = 0; l < L; ++l) {
+      if (dropped[l]) continue;
+      int32_t u, v;
+      link_ends(l, u, v);
+      emit(u, v);
+    }
+    std::unordered_set<uint64_t> added;
+    added.reserve((size_t)n_long * 2);
+    int64_t n_added = 0;
+    while (n_added < n_long) {
+      int64_t u = (int64_t)rng.below((uint64_t)V);
+      int64_t v = (int64_t)rng.below((uint64_t)V);
+      if (u == v || grid_adjacent(u, v)) continue;
+      uint64_t key = ((uint64_t)std::min(u, v) << 32) | (uint64_t)std::max(u, v);
+      if (!added.insert(key).second) continue;
+      emit((int32_t)u, (int32_t)v);
+      ++n_added;
+    }
+    *out = e;
+    return GAS_OK;
+  });
 }
 
 int gas_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
                  double c, uint64_t seed, int symmetrize, int32_t wmin,
                  int32_t wmax, gas_edges_t *out) {
-  if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 ||
-      b < 0 || c < 0 || a + b + c > 1.0 + 1e-12 ||
-      (wmax != 0 && (wmin < 1 || wmax < wmin)))
-    return set_error(GAS_ERR_ARG, "gas_gen_rmat: bad arguments");
-  const int64_t nV = (int64_t)1 << scale;
-  const int64_t m = (int64_t)edge_factor * nV;
-  if (m * (symmetrize ? 2 : 1) > INT32_MAX)
-    return set_error(GAS_ERR_ARG, "gas_gen_rmat: too many edges for int32");
-  std::vector<int32_t> us(m), vs(m);
-  const int64_t CH = 1 << 16;
-  const int64_t nch = (m + CH - 1) / CH;
-  const double ab = a + b, abc = a + b + c;
-  gas::parallel_for(nch, [&](int64_t cb, int64_t ce) {
-    for (int64_t ch = cb; ch < ce; ++ch) {
-      gas::Xoshiro256 rng(seed * 0x100000001B3ull + (uint64_t)ch);
-      int64_t e0 = ch * CH, e1 = std::min(m, e0 + CH);
-      for (int64_t k = e0; k < e1; ++k) {
-        uint32_t u = 0, v = 0;
-        for (int32_t lvl = 0; lvl < scale; ++lvl) {
-          uint32_t bit = 1u << (scale - 1 - lvl);
-          double r = rng.uniform();
-          if (r < a) {
-          } else if (r < ab) {
-            v |= bit;
-          } else if (r < abc) {
-            u |= bit;
-          } else {
-            u |= bit;
-            v |= bit;
+  return gas::host_guard("gas_gen_rmat", [&]() -> int {
+    if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 ||
+        b < 0 || c < 0 || a + b + c > 1.0 + 1e-12 ||
+        (wmax != 0 && (wmin < 1 || wmax < wmin)))
+      return set_error(GAS_ERR_ARG, "gas_gen_rmat: bad arguments");
+    const int64_t nV = (int64_t)1 << scale;
+    const int64_t m = (int64_t)edge_factor * nV;
+    if (m * (symmetrize ? 2 : 1) > INT32_MAX)
+      return set_error(GAS_ERR_ARG, "gas_gen_rmat: too many edges for int32");
+    std::vector<int32_t> us(m), vs(m);
+    const int64_t CH = 1 << 16;
+    const int64_t nch = (m + CH - 1) / CH;
+    const double ab = a + b, abc = a + b + c;
+    gas::parallel_for(nch, [&](int64_t cb, int64_t ce) {
+      for (int64_t ch = cb; ch < ce; ++ch) {
+        gas::Xoshiro256 rng(seed * 0x100000001B3ull + (uint64_t)ch);
+        int64_t e0 = ch * CH, e1 = std::min(m, e0 + CH);
+        for (int64_t k = e0; k < e1; ++k) {
+          uint32_t u = 0, v = 0;
+          for (int32_t lvl = 0; lvl < scale; ++lvl) {
+            uint32_t bit = 1u << (scale - 1 - lvl);
+            double r = rng.uniform();
+            if (r < a) {
+            } else if (r < ab) {
+              v |= bit;
+            } else if (r < abc) {
+              u |= bit;
+            } else {
+              u |= bit;
+              v |= bit;
+            }
           }
+          us[k] = (int32_t)u;
+          vs[k] = (int32_t)v;
         }
-        us[k] = (int32_t)u;
-        vs[k] = (int32_t)v;
-      }
-    }
-  });
-  // Drop self loops; optionally add reverse edges.
-  std::vector<int32_t> ks, vals;
-  ks.reserve(symmetrize ? 2 * m : m);
-  vals.reserve(symmetrize ? 2 * m : m);
-  for (int64_t k = 0; k < m; ++k) {
-    if (us[k] == vs[k]) continue;
-    ks.push_back(us[k]);
-    vals.push_back(vs[k]);
-    if (symmetrize) {
-      ks.push_back(vs[k]);
-      vals.push_back(us[k]);
-    }
-  }
-  us.clear(); us.shrink_to_fit();
-  vs.clear(); vs.shrink_to_fit();
-  std::vector<int32_t> off, adj, eid;
-  gas::build_compressed((int32_t)nV, (int32_t)ks.size(), ks.data(),
-                        vals.data(), off, adj, eid);
-  ks.clear(); ks.shrink_to_fit();
-  vals.clear(); vals.shrink_to_fit();
-  eid.clear(); eid.shrink_to_fit();
-  gas::dedupe_csr((int32_t)nV, off, adj);
-  auto *e = new gas_edges;
-  e->nV = (int32_t)nV;
-  int64_t E = off[nV];
-  e->src.resize(E);
-  e->dst.swap(adj);
-  gas::parallel_for(nV, [&](int64_t vb, int64_t ve) {
-    for (int64_t v = vb; v < ve; ++v)
-      for (int32_t k = off[v]; k < off[v + 1]; ++k) e->src[k] = (int32_t)v;
-  });
-  if (wmax != 0) {
-    e->w.resize(E);
-    const uint64_t span = (uint64_t)(wmax - wmin + 1);
-    gas::parallel_for(E, [&](int64_t kb, int64_t ke) {
-      for (int64_t k = kb; k < ke; ++k) {
-        uint32_t x = (uint32_t)e->src[k], y = (uint32_t)e->dst[k];
-        uint32_t lo = std::min(x, y), hi = std::max(x, y);
-        e->w[k] = wmin + (int32_t)(gas::pair_hash(seed, lo, hi) % span);
       }
     });
-  }
-  *out = e;
-  return GAS_OK;
+    // Drop self loops; optionally add reverse edges.
+    std::vector<int32_t> ks, vals;
+    ks.reserve(symmetrize ? 2 * m : m);
+    vals.reserve(symmetrize ? 2 * m : m);
+    for (int64_t k = 0; k < m; ++k) {
+      if (us[k] == vs[k]) continue;
+      ks.push_back(us[k]);
+      vals.push_back(vs[k]);
+      if (symmetrize) {
+        ks.push_back(vs[k]);
+        vals.push_back(us[k]);
+      }
+    }
+    us.clear(); us.shrink_to_fit();
+    vs.clear(); vs.shrink_to_fit();
+    std::vector<int32_t> off, adj, eid;
+    gas::build_compressed((int32_t)nV, (int32_t)ks.size(), ks.data(),
+                          vals.data(), off, adj, eid);
+    ks.clear(); ks.shrink_to_fit();
+    vals.clear(); vals.shrink_to_fit();
+    eid.clear(); eid.shrink_to_fit();
+    gas::dedupe_csr((int32_t)nV, off, adj);
+    auto *e = new gas_edges;
+    e->nV = (int32_t)nV;
+    int64_t E = off[nV];
+    e->src.resize(E);
+    e->dst.swap(adj);
+    gas::parallel_for(nV, [&](int64_t vb, int64_t ve) {
+      for (int64_t v = vb; v < ve; ++v)
+        for (int32_t k = off[v]; k < off[v + 1]; ++k) e->src[k] = (int32_t)v;
+    });
+    if (wmax != 0) {
+      e->w.resize(E);
+      const uint64_t span = (uint64_t)(wmax - wmin + 1);
+      gas::parallel_for(E, [&](int64_t kb, int64_t ke) {
+        for (int64_t k = kb; k < ke; ++k) {
+          uint32_t x = (uint32_t)e->src[k], y = (uint32_t)e->dst[k];
+          uint32_t lo = std::min(x, y), hi = std::max(x, y);
+          e->w[k] = wmin + (int32_t)(gas::pair_hash(seed, lo, hi) % span);
+        }
+      });
+    }
+    *out = e;
+    return GAS_OK;
+  });
 }
 
 int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,
                                    int32_t *degree) {
-  if (!e || !vertex)
-    return set_error(GAS_ERR_ARG, "gas_edges_max_outdegree_vertex: null");
-  std::vector<int32_t> deg(e->nV, 0);
-  for (int32_t s : e->src) deg[s]++;
-  int32_t best = -1, bv = -1;
-  for (int32_t v = 0; v < e->nV; ++v)
-    if (deg[v] > best) {
-      best = deg[v];
-      bv = v;
-    }
-  *vertex = bv;
-  if (degree) *degree = best;
-  return GAS_OK;
+  return gas::host_guard("gas_edges_max_outdegree_vertex", [&]() -> int {
+    if (!e || !vertex)
+      return set_error(GAS_ERR_ARG, "gas_edges_max_outdegree_vertex: null");
+    std::vector<int32_t> deg(e->nV, 0);
+    for (int32_t s : e->src) deg[s]++;
+    int32_t best = -1, bv = -1;
+    for (int32_t v = 0; v < e->nV; ++v)
+      if (deg[v] > best) {
+        best = deg[v];
+        bv = v;
+      }
+    *vertex = bv;
+    if (degree) *degree = best;
+    return GAS_OK;
+  });
 }
 
 static void graph_release(gas_graph *g) {
@@ -466,12 +490,45 @@ static int device_create(gas_edges_t edges, gas_graph *g) {
 }
 
 int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
-  if (!edges || !out) return set_error(GAS_ERR_ARG, "gas_graph_create: null");
-  auto *g = new gas_graph;
-  g->nV = edges->nV;
-  g->nE = (int32_t)edges->src.size();
-  if ((int64_t)g->nE >= g_device_build_min_edges) {
-    const int rc = device_create(edges, g);
+  return gas::host_guard("gas_graph_create", [&]() -> int {
+    if (!edges || !out) return set_error(GAS_ERR_ARG, "gas_graph_create: null");
+    auto *g = new gas_graph;
+    g->nV = edges->nV;
+    g->nE = (int32_t)edges->src.size();
+    if ((int64_t)g->nE >= g_device_build_min_edges) {
+      const int rc = device_create(edges, g);
+      if (rc) {
+        graph_release(g);
+        delete g;
+        return rc;
+      }
+      *out = g;
+      return GAS_OK;
+    }
+    gas::build_compressed(g->nV, g->nE, edges->dst.data(), edges->src.data(),
+                          g->h_csc_off, g->h_csc_src, g->h_csc_eid);
+    gas::build_compressed(g->nV, g->nE, edges->src.data(), edges->dst.data(),
+                          g->h_csr_off, g->h_csr_dst, g->h_csr_eid);
+    const size_t vb = ((size_t)g->nV + 1) * 4, eb = (size_t)g->nE * 4;
+    int rc = 0;
+    auto up = [&](int32_t *&dptr, const void *src, size_t bytes) {
+      if (rc) return;
+      rc = gasrt_malloc((void **)&dptr, bytes ? bytes : 4);
+      if (!rc) rc = gasrt_memcpy_h2d(dptr, src, bytes, nullptr);
+      g->device_bytes += bytes;
+    };
+    up(g->csc_off, g->h_csc_off.data(), vb);
+    up(g->csc_src, g->h_csc_src.data(), eb);
+    up(g->csr_off, g->h_csr_off.data(), vb);
+    up(g->csr_dst, g->h_csr_dst.data(), eb);
+    if (!edges->w.empty()) {
+      for (int32_t x : edges->w) g->max_w = std::max(g->max_w, x);
+      std::vector<int32_t> wc(g->nE);
+      for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csc_eid[k]];
+      up(g->csc_w, wc.data(), eb);
+      for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csr_eid[k]];
+      up(g->csr_w, wc.data(), eb);
+    }
     if (rc) {
       graph_release(g);
       delete g;
@@ -479,38 +536,7 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
     }
     *out = g;
     return GAS_OK;
-  }
-  gas::build_compressed(g->nV, g->nE, edges->dst.data(), edges->src.data(),
-                        g->h_csc_off, g->h_csc_src, g->h_csc_eid);
-  gas::build_compressed(g->nV, g->nE, edges->src.data(), edges->dst.data(),
-                        g->h_csr_off, g->h_csr_dst, g->h_csr_eid);
-  const size_t vb = ((size_t)g->nV + 1) * 4, eb = (size_t)g->nE * 4;
-  int rc = 0;
-  auto up = [&](int32_t *&dptr, const void *src, size_t bytes) {
-    if (rc) return;
-    rc = gasrt_malloc((void **)&dptr, bytes ? bytes : 4);
-    if (!rc) rc = gasrt_memcpy_h2d(dptr, src, bytes, nullptr);
-    g->device_bytes += bytes;
-  };
-  up(g->csc_off, g->h_csc_off.data(), vb);
-  up(g->csc_src, g->h_csc_src.data(), eb);
-  up(g->csr_off, g->h_csr_off.data(), vb);
-  up(g->csr_dst, g->h_csr_dst.data(), eb);
-  if (!edges->w.empty()) {
-    for (int32_t x : edges->w) g->max_w = std::max(g->max_w, x);
-    std::vector<int32_t> wc(g->nE);
-    for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csc_eid[k]];
-    up(g->csc_w, wc.data(), eb);
-    for (int32_t k = 0; k < g->nE; ++k) wc[k] = edges->w[g->h_csr_eid[k]];
-    up(g->csr_w, wc.data(), eb);
-  }
-  if (rc) {
-    graph_release(g);
-    delete g;
-    return rc;
-  }
-  *out = g;
-  return GAS_OK;
+  });
 }
 
 int gas_graph_view_get(gas_graph_t g, gas_graph_view *v) {
