@@ -415,10 +415,10 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
  * the call) in real time.  latency_ms[i] receives arrival-to-result time;
  * host_results[i], if non-NULL, receives the V int32 (BFS, SSSP) or float
  * (PPR) result.  Fails with GAS_ERR_FULL, serving nothing, if some query can
- * never fit the budget.  During the call a second host thread runs the
- * graphs' whole-run engines (gas_bfs_device, ...) for batches of one, so
- * the caller must not run whole-run entry points on the same graphs
- * concurrently. */
+ * never fit the budget.  During the call two more host threads run the
+ * graphs' whole-run engines (gas_bfs_device, ...) for batches of one, the
+ * second with its own set of them, so the caller must not run whole-run
+ * entry points on the same graphs concurrently. */
 int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
                       const int32_t *graph_ids, const int32_t *sources,
                       const double *arrival_s, void *const *host_results,
@@ -432,7 +432,8 @@ int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds);
 /* Batches of one: a query whose plain-engine time (measured per kind and
  * graph at the first replay) is at most max_ms, or at most idle_max_ms
  * while the server is lightly loaded, is answered on the plain engine by a
- * second host thread beside the batch rounds.  0, 0 = always batch.
+ * solo host thread beside the batch rounds (two under light load, one
+ * otherwise).  0, 0 = always batch.
  * Default 2 ms and 50 ms. */
 int gas_server_set_solo(gas_server_t s, double max_ms, double idle_max_ms);
 int gas_server_free(gas_server_t s);

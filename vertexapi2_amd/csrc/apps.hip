@@ -84,7 +84,8 @@ static int guarded(F f) {
 template <class P>
 static GASEngineGPU<P> &cached_engine(gas_graph_t g, int slot,
                                       typename P::VertexData *vd) {
-  if (!g->engine_cache[slot]) {
+  void *&cache = g->engine_cache[engine_set()][slot];
+  if (!cache) {
     auto *e = new GASEngineGPU<P>();
     try {
       e->setGraph(g, vd, nullptr);
@@ -92,15 +93,14 @@ static GASEngineGPU<P> &cached_engine(gas_graph_t g, int slot,
       delete e;
       throw;
     }
-    g->engine_cache[slot] = e;
-    g->engine_release[slot] = [](void *p) {
+    cache = e;
+    g->engine_release[engine_set()][slot] = [](void *p) {
       delete static_cast<GASEngineGPU<P> *>(p);
     };
   } else {
-    if (vd)
-      static_cast<GASEngineGPU<P> *>(g->engine_cache[slot])->resetVertexData(vd);
+    if (vd) static_cast<GASEngineGPU<P> *>(cache)->resetVertexData(vd);
   }
-  return *static_cast<GASEngineGPU<P> *>(g->engine_cache[slot]);
+  return *static_cast<GASEngineGPU<P> *>(cache);
 }
 
 // Times a whole run on the engine's stream with HIP events.
@@ -191,7 +191,7 @@ static int with_engine(gas_graph_t g, int program, F f) {
       case GAS_PROGRAM_BFS: {
         std::vector<BFS::VertexData> vd(g->nV, BFS::VertexData{-1});
         f(cached_engine<BFS>(g, program, vd.data()));
-        static_cast<GASEngineGPU<BFS> *>(g->engine_cache[program])->detachHost();
+        static_cast<GASEngineGPU<BFS> *>(g->engine_cache[engine_set()][program])->detachHost();
         break;
       }
       case GAS_PROGRAM_SSSP: {
@@ -199,7 +199,7 @@ static int with_engine(gas_graph_t g, int program, F f) {
           throw Error(GAS_ERR_ARG, "graph has no edge weights");
         std::vector<int> vd(g->nV, SSSP::gatherZero);
         f(cached_engine<SSSP>(g, program, vd.data()));
-        static_cast<GASEngineGPU<SSSP> *>(g->engine_cache[program])->detachHost();
+        static_cast<GASEngineGPU<SSSP> *>(g->engine_cache[engine_set()][program])->detachHost();
         break;
       }
       case GAS_PROGRAM_PAGERANK: {
@@ -208,7 +208,7 @@ static int with_engine(gas_graph_t g, int program, F f) {
           vd[v] = {PageRank::pageConst,
                    std::max(1, g->h_csr_off[v + 1] - g->h_csr_off[v])};
         f(cached_engine<PageRank>(g, program, vd.data()));
-        static_cast<GASEngineGPU<PageRank> *>(g->engine_cache[program])
+        static_cast<GASEngineGPU<PageRank> *>(g->engine_cache[engine_set()][program])
             ->detachHost();
         break;
       }
@@ -216,7 +216,7 @@ static int with_engine(gas_graph_t g, int program, F f) {
         std::vector<int> vd(g->nV);
         for (int32_t v = 0; v < g->nV; ++v) vd[v] = v;
         f(cached_engine<CC>(g, program, vd.data()));
-        static_cast<GASEngineGPU<CC> *>(g->engine_cache[program])->detachHost();
+        static_cast<GASEngineGPU<CC> *>(g->engine_cache[engine_set()][program])->detachHost();
         break;
       }
       default:

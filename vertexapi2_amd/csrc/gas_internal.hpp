@@ -44,12 +44,19 @@ struct gas_graph {
   // BFS pull divisor, SSSP distance step (gas_graph_autotune_batch)
   int32_t batch_param[2] = {0, 0};
   // Engines kept bound to this graph by the whole-run entry points
-  // (gas_bfs, ...), one per program, released with the graph.
-  void *engine_cache[8] = {};
-  void (*engine_release[8])(void *) = {};
+  // (gas_bfs, ...), one per program and engine set, released with the
+  // graph.  A host thread uses the set engine_set() names, so threads of
+  // different sets can run whole-run queries on one graph at once.
+  static constexpr int kEngineSets = 2;
+  void *engine_cache[kEngineSets][8] = {};
+  void (*engine_release[kEngineSets][8])(void *) = {};
 };
 
 namespace gas {
+
+// Engine set of the calling host thread's whole-run calls (0 unless the
+// thread chose another; the server's second solo thread uses 1).
+int &engine_set();
 
 // Device-side build of one layout (graph_build.hip): offsets (V+1), values
 // and weights (if d_w) in key order, the edge-list index of each slot;

@@ -17,6 +17,9 @@
 namespace gas {
 
 static thread_local std::string t_last_error;
+static thread_local int t_engine_set = 0;
+
+int &engine_set() { return t_engine_set; }
 
 int set_error(int code, const char *fmt, ...) {
   char buf[1024];

@@ -407,9 +407,10 @@ int gas_edges_max_outdegree_vertex(gas_edges_t e, int32_t *vertex,
 }
 
 static void graph_release(gas_graph *g) {
-  for (int i = 0; i < 8; ++i)
-    if (g->engine_cache[i] && g->engine_release[i])
-      g->engine_release[i](g->engine_cache[i]);
+  for (int k = 0; k < gas_graph::kEngineSets; ++k)
+    for (int i = 0; i < 8; ++i)
+      if (g->engine_cache[k][i] && g->engine_release[k][i])
+        g->engine_release[k][i](g->engine_cache[k][i]);
   gasrt_free(g->csc_off);
   gasrt_free(g->csc_src);
   gasrt_free(g->csr_off);

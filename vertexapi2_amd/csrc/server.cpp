@@ -2,8 +2,8 @@
 //
 // Waiting queries that are cheap on the plain engine for their kind and
 // graph, or that few others of their kind and graph wait beside, are queued
-// to a second host thread that answers them there one at a time (a batch of
-// one); the plain engine's latency-bound supersteps run beside the batch
+// to solo host threads that answer them there one at a time each (a batch
+// of one); the plain engine's latency-bound supersteps run beside the batch
 // rounds while the loop keeps driving the batch engines.
 //
 // One batch engine per (query kind, graph) -- BFS and SSSP lanes
@@ -65,6 +65,11 @@ constexpr double kSoloMaxMs = 2.0;
 constexpr double kSoloIdleMaxMs = 50.0;
 constexpr int kSoloBacklog = 4;
 constexpr double kSoloQueueMs = 8.0;
+// Solo threads: the plain engine's supersteps are latency-bound, so two of
+// them (each with its own plain engines and stream) overlap on the GPU.
+constexpr int kSoloThreads = 2;
+static_assert(kSoloThreads <= gas_graph::kEngineSets,
+              "each solo thread needs an engine set of its own");
 
 size_t query_bytes(int kind, int64_t V) {
   // lane state + the lane's result buffer
@@ -75,7 +80,7 @@ size_t query_bytes(int kind, int64_t V) {
   }
 }
 
-// The second host thread: queued queries one at a time on the plain engine
+// A solo host thread: queued queries one at a time on the plain engine
 // (gas_*_device into a per-graph buffer), each result optionally copied to
 // the caller on the thread's own stream.
 struct SoloWorker {
@@ -90,9 +95,12 @@ struct SoloWorker {
   double queued_ms = 0;  // loop side: expected plain-engine time handed over
   int pending = 0;       // loop side: queries handed over, not yet taken
 
+  // run(q) answers query q; the thread's whole-run calls use engine set
+  // `set` of each graph, so solo threads never share a plain engine.
   template <class F>
-  void start(F run) {
-    th = std::thread([this, run] {
+  void start(int set, F run) {
+    th = std::thread([this, run, set] {
+      engine_set() = set;
       for (;;) {
         int q;
         {
@@ -166,7 +174,8 @@ struct gas_server {
   std::vector<double> solo_ms;                   // plain-engine cost per key
   std::vector<double> batch1_ms;                 // batch-of-one cost per key
   bool calibrated = false;
-  std::vector<DevBuf<int32_t>> solo_out;         // [graph] plain results
+  // [solo thread][graph] plain results
+  std::vector<DevBuf<int32_t>> solo_out[kSoloThreads];
 
   Engine &engine(int kind, int graph) {
     auto &slot = engines[(size_t)kind * graphs.size() + graph];
@@ -233,7 +242,7 @@ int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
     s->engines.resize(3 * (size_t)n_graphs);
     s->solo_ms.assign(3 * (size_t)n_graphs, 1e30);
     s->batch1_ms.assign(3 * (size_t)n_graphs, 0.0);
-    s->solo_out.resize((size_t)n_graphs);
+    for (auto &v : s->solo_out) v.resize((size_t)n_graphs);
     *out = s.release();
   });
 }
@@ -284,22 +293,32 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
     // plain-engine cost of BFS and SSSP per graph, measured once on an idle
     // server (the first run of each builds that plain engine)
     if (!s->calibrated) {
-      for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
-        auto &buf = s->solo_out[gi];
-        if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
-        for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP, GAS_BATCH_PPR}) {
-          auto run = [&] {
-            return kind == GAS_BATCH_BFS
-                       ? gas_bfs_device(s->graphs[gi], 0, buf.p, nullptr)
-                   : kind == GAS_BATCH_SSSP
-                       ? gas_sssp_device(s->graphs[gi], 0, buf.p, nullptr)
-                       : gas_ppr_device(s->graphs[gi], 0, s->ppr_iterations,
-                                        reinterpret_cast<float *>(buf.p), nullptr);
-          };
-          if (run() != GAS_OK) continue;  // e.g. SSSP on an unweighted graph
-          const double ts = now_s();
-          if (run() != GAS_OK) continue;
-          s->solo_ms[(size_t)kind * s->graphs.size() + gi] = (now_s() - ts) * 1e3;
+      // every solo thread's plain engines (engine set = thread index) are
+      // built here, so no query pays for building one; set 0's second run
+      // is the timed one
+      struct SetGuard {
+        ~SetGuard() { engine_set() = 0; }
+      } set_guard;
+      for (int set = kSoloThreads - 1; set >= 0; --set) {
+        engine_set() = set;
+        for (size_t gi = 0; gi < s->graphs.size(); ++gi) {
+          auto &buf = s->solo_out[set][gi];
+          if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
+          for (int kind : {GAS_BATCH_BFS, GAS_BATCH_SSSP, GAS_BATCH_PPR}) {
+            auto run = [&] {
+              return kind == GAS_BATCH_BFS
+                         ? gas_bfs_device(s->graphs[gi], 0, buf.p, nullptr)
+                     : kind == GAS_BATCH_SSSP
+                         ? gas_sssp_device(s->graphs[gi], 0, buf.p, nullptr)
+                         : gas_ppr_device(s->graphs[gi], 0, s->ppr_iterations,
+                                          reinterpret_cast<float *>(buf.p), nullptr);
+            };
+            if (run() != GAS_OK) continue;  // e.g. SSSP on an unweighted graph
+            if (set != 0) continue;
+            const double ts = now_s();
+            if (run() != GAS_OK) continue;
+            s->solo_ms[(size_t)kind * s->graphs.size() + gi] = (now_s() - ts) * 1e3;
+          }
         }
       }
       // and a batch of one lane on each batch engine (from vertex 0)
@@ -332,28 +351,31 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
     int64_t rounds = 0, admissions_deferred = 0;
     int max_running = 0, solo = 0;
     const double t0 = now_s();
-    gasrt_stream_t copy_stream = nullptr;
-    check(gasrt_stream_create(&copy_stream), "stream");
     struct StreamGuard {
-      gasrt_stream_t s;
-      ~StreamGuard() { gasrt_stream_destroy(s); }
-    } copy_guard{copy_stream};
-    SoloWorker worker;  // joined before copy_stream is destroyed
-    worker.start([&](int q) {
-      const int gi = graph_ids[q];
-      auto &buf = s->solo_out[gi];
-      if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
-      int rc = kinds[q] == GAS_BATCH_BFS
-                   ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
-               : kinds[q] == GAS_BATCH_SSSP
-                   ? gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr)
-                   : gas_ppr_device(s->graphs[gi], sources[q], s->ppr_iterations,
-                                    reinterpret_cast<float *>(buf.p), nullptr);
-      if (rc == GAS_OK && host_results && host_results[q])
-        rc = gasrt_memcpy_d2h(host_results[q], buf.p, 4 * (size_t)s->nV[gi],
-                              copy_stream);
-      return rc;
-    });
+      gasrt_stream_t s = nullptr;
+      ~StreamGuard() {
+        if (s) gasrt_stream_destroy(s);
+      }
+    } copy_guard[kSoloThreads];
+    for (auto &c : copy_guard) check(gasrt_stream_create(&c.s), "stream");
+    SoloWorker workers[kSoloThreads];  // joined before the streams are destroyed
+    for (int w = 0; w < kSoloThreads; ++w) {
+      workers[w].start(w, [&, w](int q) {
+        const int gi = graph_ids[q];
+        auto &buf = s->solo_out[w][gi];
+        if (buf.n < (size_t)s->nV[gi]) buf.alloc((size_t)s->nV[gi]);
+        int rc = kinds[q] == GAS_BATCH_BFS
+                     ? gas_bfs_device(s->graphs[gi], sources[q], buf.p, nullptr)
+                 : kinds[q] == GAS_BATCH_SSSP
+                     ? gas_sssp_device(s->graphs[gi], sources[q], buf.p, nullptr)
+                     : gas_ppr_device(s->graphs[gi], sources[q], s->ppr_iterations,
+                                      reinterpret_cast<float *>(buf.p), nullptr);
+        if (rc == GAS_OK && host_results && host_results[q])
+          rc = gasrt_memcpy_d2h(host_results[q], buf.p, 4 * (size_t)s->nV[gi],
+                                copy_guard[w].s);
+        return rc;
+      });
+    }
     auto running_total = [&] {
       int r = 0;
       for (auto &ep : s->engines)
@@ -412,29 +434,33 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       // engines between rounds
       const double t = now_s() - t0;
       while (next < n && arrival_s[next] <= t) waiting.push_back(next++);
-      // queries the solo thread finished
-      for (const auto &f : worker.take()) {
-        const int q = f.first;
-        latency_ms[q] = (f.second - t0 - arrival_s[q]) * 1e3;
-        worker.queued_ms -= s->solo_ms[(size_t)kinds[q] * s->graphs.size() +
-                                       graph_ids[q]];
-        if (--worker.pending == 0) worker.queued_ms = 0;
-        ++done;
-        ++solo;
-        progressed = true;
-      }
-      // waiting queries go to the solo thread in arrival order when their
-      // plain-engine time is at most solo_max_ms, or at most solo_idle_max_ms
-      // while fewer than kSoloBacklog queries of their kind and graph wait
-      // (more would share a batch), and the solo queue would finish them no
-      // later than a batch of one (an idle solo thread takes one
-      // regardless).  BFS / SSSP results are exact on either engine; a PPR
+      // queries the solo threads finished
+      for (auto &worker : workers)
+        for (const auto &f : worker.take()) {
+          const int q = f.first;
+          latency_ms[q] = (f.second - t0 - arrival_s[q]) * 1e3;
+          worker.queued_ms -= s->solo_ms[(size_t)kinds[q] * s->graphs.size() +
+                                         graph_ids[q]];
+          if (--worker.pending == 0) worker.queued_ms = 0;
+          ++done;
+          ++solo;
+          progressed = true;
+        }
+      // waiting queries go to the solo thread with the shortest queue, in
+      // arrival order, when their plain-engine time is at most solo_max_ms,
+      // or at most solo_idle_max_ms while fewer than kSoloBacklog queries of
+      // their kind and graph wait (more would share a batch), and that queue
+      // would finish them no later than a batch of one (an idle solo thread
+      // takes one regardless).  Solo threads after the first take queries
+      // only while at most kSoloBacklog queries wait in all: under load the
+      // batches answer them for less GPU time each.  BFS / SSSP results are exact on either engine; a PPR
       // result from the plain engine is within the same 1e-5 of FP64 as a
       // batch lane but not bit-identical to one.
       {
         std::vector<int> backlog(s->engines.size(), 0);
         for (int q : waiting)
           ++backlog[(size_t)kinds[q] * s->graphs.size() + graph_ids[q]];
+        const int n_solo = waiting.size() <= (size_t)kSoloBacklog ? kSoloThreads : 1;
         for (auto it = waiting.begin(); it != waiting.end();) {
           const int q = *it;
           const size_t key = (size_t)kinds[q] * s->graphs.size() + graph_ids[q];
@@ -442,13 +468,18 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
           const double cap =
               backlog[key] < kSoloBacklog ? s->solo_idle_max_ms : s->solo_max_ms;
           const double budget = std::max(kSoloQueueMs, s->batch1_ms[key]);
-          if (ms > cap || (worker.pending > 0 && worker.queued_ms + ms > budget)) {
+          SoloWorker *wk = &workers[0];
+          for (int w = 1; w < n_solo; ++w)
+            if (workers[w].queued_ms < wk->queued_ms ||
+                (workers[w].pending == 0 && wk->pending > 0))
+              wk = &workers[w];
+          if (ms > cap || (wk->pending > 0 && wk->queued_ms + ms > budget)) {
             ++it;
             continue;
           }
-          worker.queued_ms += ms;
-          ++worker.pending;
-          worker.submit(q);
+          wk->queued_ms += ms;
+          ++wk->pending;
+          wk->submit(q);
           it = waiting.erase(it);
           progressed = true;
         }
@@ -523,7 +554,8 @@ int gas_server_replay(gas_server_t s, int32_t n, const int32_t *kinds,
       }
       max_running = std::max(max_running, running_total());
       if (!progressed) {
-        bool any_inflight = worker.pending > 0;
+        bool any_inflight = false;
+        for (auto &w : workers) any_inflight |= w.pending > 0;
         for (auto &ep : s->engines) any_inflight |= ep && ep->inflight;
         if (!any_inflight && waiting.empty() && next < n) {
           const double wait = arrival_s[next] - (now_s() - t0);
