@@ -358,3 +358,29 @@ def test_device_graph_build_matches_host(gpu):
                 assert np.array_equal(a, b), (n, len(s))
     finally:
         gas.set_device_build(1 << 20)
+
+
+def test_pagerank_degree_order(gpu):
+    # above 2**20 edges PageRank runs on a copy renumbered by in-degree and
+    # returns ranks in the caller's numbering; the graph as given agrees to
+    # FP32 rounding
+    el = workloads.rmat(17)
+    s, d, _ = el.arrays()
+    n = el.n_vertices
+    assert el.n_edges >= 1 << 20
+    ref = oracle.check_pagerank_fp64(n, s, d, 20)
+    g = gas.Graph(el)
+    r1, res1, _ = g.pagerank(20)
+    buf = gas.DeviceBuffer(4 * n)
+    g.pagerank_device(buf.ptr, 20)
+    r1d = buf.to_numpy().view(np.float32)
+    g.set_pagerank_order(0)
+    r0, res0, _ = g.pagerank(20)
+    assert rel_l1(r1, ref) <= PR_REL_L1 and rel_l1(r0, ref) <= PR_REL_L1
+    assert np.array_equal(r1, r1d)
+    assert np.allclose(res1, res0, rtol=1e-5)
+    g.set_pagerank_order(1)
+    tol1, _, _ = g.pagerank(0)   # tolerance-driven run on the ordered copy
+    g.set_pagerank_order(0)
+    tol0, _, _ = g.pagerank(0)
+    assert rel_l1(tol1, tol0) <= 1e-3

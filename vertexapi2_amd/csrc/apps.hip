@@ -149,10 +149,12 @@ __global__ void k_pr_init(Dev<PageRank> d) {
   }
 }
 
-__global__ void k_pr_rank_out(Dev<PageRank> d, float *out) {
+// Ranks in the caller's numbering: out[v] = rank of v's id in the engine's
+// graph (new_of = NULL: the same numbering).
+__global__ void k_pr_rank_out(Dev<PageRank> d, const int32_t *new_of, float *out) {
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
-    out[v] = d.vdata[v].rank;
+    out[v] = d.vdata[new_of ? new_of[v] : v].rank;
 }
 
 // CC's initial labels, label[v] = v (connected_component.cu:112-114).
@@ -203,12 +205,13 @@ static int with_engine(gas_graph_t g, int program, F f) {
         break;
       }
       case GAS_PROGRAM_PAGERANK: {
-        std::vector<PageRank::VertexData> vd(g->nV);
-        for (int32_t v = 0; v < g->nV; ++v)
+        gas_graph_t pg = pagerank_graph(g);
+        std::vector<PageRank::VertexData> vd(pg->nV);
+        for (int32_t v = 0; v < pg->nV; ++v)
           vd[v] = {PageRank::pageConst,
-                   std::max(1, g->h_csr_off[v + 1] - g->h_csr_off[v])};
-        f(cached_engine<PageRank>(g, program, vd.data()));
-        static_cast<GASEngineGPU<PageRank> *>(g->engine_cache[engine_set()][program])
+                   std::max(1, pg->h_csr_off[v + 1] - pg->h_csr_off[v])};
+        f(cached_engine<PageRank>(pg, program, vd.data()));
+        static_cast<GASEngineGPU<PageRank> *>(pg->engine_cache[engine_set()][program])
             ->detachHost();
         break;
       }
@@ -392,12 +395,13 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
   if (!g || !rank_out || iterations < 0)
     return set_error(GAS_ERR_ARG, "gas_pagerank: bad arguments");
   return guarded([&] {
-    std::vector<PageRank::VertexData> vd(g->nV);
-    for (int32_t v = 0; v < g->nV; ++v) {
+    gas_graph_t pg = pagerank_graph(g);  // g or its degree-ordered copy
+    std::vector<PageRank::VertexData> vd(pg->nV);
+    for (int32_t v = 0; v < pg->nV; ++v) {
       vd[v].rank = PageRank::pageConst;
-      vd[v].numOutEdges = g->h_csr_off[v + 1] - g->h_csr_off[v];
+      vd[v].numOutEdges = pg->h_csr_off[v + 1] - pg->h_csr_off[v];
     }
-    auto &e = cached_engine<PageRank>(g, GAS_PROGRAM_PAGERANK, vd.data());
+    auto &e = cached_engine<PageRank>(pg, GAS_PROGRAM_PAGERANK, vd.data());
     e.resetStats();
     DevBuf<double> hist;
     hist.alloc(iterations > 0 ? (size_t)iterations : 1);
@@ -415,7 +419,8 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
     }
     double ms = tm.stop();
     e.getResults();
-    for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[v].rank;
+    for (int32_t v = 0; v < g->nV; ++v)
+      rank_out[v] = vd[pg == g ? v : g->pr_new_of[v]].rank;
     if (residual_out && iterations > 0)
       check(gasrt_memcpy_d2h(residual_out, hist.p,
                              sizeof(double) * (size_t)iterations, e.stream()),
@@ -461,7 +466,8 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
   if (!g || !rank_dev || iterations < 1)
     return set_error(GAS_ERR_ARG, "gas_pagerank_device: bad arguments");
   return guarded([&] {
-    auto &e = cached_engine<PageRank>(g, GAS_PROGRAM_PAGERANK, nullptr);
+    gas_graph_t pg = pagerank_graph(g);  // g or its degree-ordered copy
+    auto &e = cached_engine<PageRank>(pg, GAS_PROGRAM_PAGERANK, nullptr);
     e.resetStats();
     DevBuf<double> hist;
     if (!residual_dev) hist.alloc((size_t)iterations);
@@ -476,7 +482,9 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
       e.gatherApply(true);
       launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
     }
-    launch(k_pr_rank_out, grid, e.stream(), e.dev(), rank_dev);
+    launch(k_pr_rank_out, grid, e.stream(), e.dev(),
+           pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_pr_new_of,
+           rank_dev);
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);

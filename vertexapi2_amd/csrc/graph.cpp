@@ -12,6 +12,8 @@
 #include <fstream>
 #include <new>
 #include <sstream>
+#include <stdexcept>
+#include <string>
 #include <thread>
 #include <unordered_set>
 
@@ -143,6 +145,53 @@ int host_guard(const char *who, F f) {
   } catch (const std::exception &x) {
     return set_error(GAS_ERR_STATE, "%s: %s", who, x.what());
   }
+}
+
+}  // namespace gas
+
+namespace gas {
+
+// Vertices renumbered by descending in-degree (ties by id): hub
+// destinations and, on skewed graphs, the hottest sources share cache lines,
+// and each in-list stays sorted by (new) source id.  The edge list is laid
+// out by new source, so the stable CSC sort keeps every in-list ascending.
+gas_graph *pagerank_graph(gas_graph *g) {
+  if (!g->pr_order || (int64_t)g->nE < kPageRankOrderEdges) return g;
+  if (g->pr_twin) return g->pr_twin;
+  const int32_t V = g->nV;
+  std::vector<int32_t> order((size_t)V);
+  for (int32_t v = 0; v < V; ++v) order[(size_t)v] = v;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return g->h_csc_off[(size_t)a + 1] - g->h_csc_off[(size_t)a] >
+           g->h_csc_off[(size_t)b + 1] - g->h_csc_off[(size_t)b];
+  });
+  std::vector<int32_t> new_of((size_t)V);
+  for (int32_t k = 0; k < V; ++k) new_of[(size_t)order[(size_t)k]] = k;
+  gas_edges e;
+  e.nV = V;
+  e.src.reserve((size_t)g->nE);
+  e.dst.reserve((size_t)g->nE);
+  for (int32_t k = 0; k < V; ++k) {
+    const int32_t u = order[(size_t)k];
+    for (int32_t j = g->h_csr_off[(size_t)u]; j < g->h_csr_off[(size_t)u + 1]; ++j) {
+      e.src.push_back(k);
+      e.dst.push_back(new_of[(size_t)g->h_csr_dst[(size_t)j]]);
+    }
+  }
+  gas_graph_t twin = nullptr;
+  int rc = gas_graph_create(&e, &twin);
+  int32_t *d_new = nullptr;
+  if (!rc) rc = gasrt_malloc((void **)&d_new, sizeof(int32_t) * std::max<size_t>(V, 1));
+  if (!rc) rc = gasrt_memcpy_h2d(d_new, new_of.data(), sizeof(int32_t) * (size_t)V, nullptr);
+  if (rc) {
+    if (twin) gas_graph_free(twin);
+    gasrt_free(d_new);
+    throw std::runtime_error(std::string("PageRank vertex order: ") + gas_last_error());
+  }
+  g->pr_twin = twin;
+  g->pr_new_of.swap(new_of);
+  g->d_pr_new_of = d_new;
+  return twin;
 }
 
 }  // namespace gas
@@ -411,6 +460,10 @@ static void graph_release(gas_graph *g) {
     for (int i = 0; i < 8; ++i)
       if (g->engine_cache[k][i] && g->engine_release[k][i])
         g->engine_release[k][i](g->engine_cache[k][i]);
+  if (g->pr_twin) gas_graph_free(g->pr_twin);
+  g->pr_twin = nullptr;
+  gasrt_free(g->d_pr_new_of);
+  g->d_pr_new_of = nullptr;
   gasrt_free(g->csc_off);
   gasrt_free(g->csc_src);
   gasrt_free(g->csr_off);
@@ -425,6 +478,19 @@ int gas_graph_set_device_build(int64_t min_edges) {
   if (min_edges < 0)
     return set_error(GAS_ERR_ARG, "gas_graph_set_device_build: negative edge count");
   g_device_build_min_edges = min_edges;
+  return GAS_OK;
+}
+
+int gas_graph_set_pagerank_order(gas_graph_t g, int32_t mode) {
+  if (!g || mode < 0 || mode > 1)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_pagerank_order: bad arguments");
+  if (mode != g->pr_order && g->pr_twin) {
+    gas_graph_free(g->pr_twin);
+    g->pr_twin = nullptr;
+    gasrt_free(g->d_pr_new_of);
+    g->d_pr_new_of = nullptr;
+  }
+  g->pr_order = mode;
   return GAS_OK;
 }
 
