@@ -196,15 +196,17 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
  * builds every graph on the GPU.  No counterpart in the reference, which
  * sorts on the host (util.h:68). */
 int gas_graph_set_device_build(int64_t min_edges);
-/* PageRank's whole-run entry points (gas_pagerank, gas_pagerank_device,
- * and gas_graph_autotune / gas_graph_set_tiers for GAS_PROGRAM_PAGERANK) on
- * a graph of at least 2^20 edges run on a copy of it with the vertices
- * renumbered by descending in-degree (mode 1, default; built at first use,
- * results returned in the caller's numbering), or on the graph as given
- * (mode 0).  The order puts hub destinations together and, on skewed
- * graphs, the most-read sources in shared cache lines; sums differ from
- * mode 0 only by FP32 rounding order.  No counterpart in the reference. */
-int gas_graph_set_pagerank_order(gas_graph_t g, int32_t mode);
+/* PageRank's and CC's whole-run entry points (gas_pagerank[_device],
+ * gas_cc[_device], and gas_graph_autotune / gas_graph_set_tiers for those
+ * programs) on a graph of at least 2^20 edges run on a copy of it with the
+ * vertices renumbered by descending in-degree (mode 1, default; built at
+ * first use, results returned in the caller's numbering, CC labels are the
+ * caller's ids), or on the graph as given (mode 0).  The order puts hub
+ * destinations together and, on skewed graphs, the most-read sources in
+ * shared cache lines.  CC labels are identical in both modes; PageRank
+ * sums differ only by FP32 rounding order.  No counterpart in the
+ * reference. */
+int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode);
 int gas_graph_view_get(gas_graph_t g, gas_graph_view *view);
 /* Host copies for checking: arrays of V+1 / E int32 (any may be NULL). */
 int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,

@@ -374,13 +374,32 @@ def test_pagerank_degree_order(gpu):
     buf = gas.DeviceBuffer(4 * n)
     g.pagerank_device(buf.ptr, 20)
     r1d = buf.to_numpy().view(np.float32)
-    g.set_pagerank_order(0)
+    g.set_vertex_order(0)
     r0, res0, _ = g.pagerank(20)
     assert rel_l1(r1, ref) <= PR_REL_L1 and rel_l1(r0, ref) <= PR_REL_L1
     assert np.array_equal(r1, r1d)
     assert np.allclose(res1, res0, rtol=1e-5)
-    g.set_pagerank_order(1)
+    g.set_vertex_order(1)
     tol1, _, _ = g.pagerank(0)   # tolerance-driven run on the ordered copy
-    g.set_pagerank_order(0)
+    g.set_vertex_order(0)
     tol0, _, _ = g.pagerank(0)
     assert rel_l1(tol1, tol0) <= 1e-3
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_cc_degree_order(gpu, sym):
+    # CC on the degree-ordered copy starts from the caller's ids as labels:
+    # labels identical to the graph as given and to the oracle
+    el = workloads.rmat(17, symmetrize=sym)
+    s, d, _ = el.arrays()
+    n = el.n_vertices
+    assert el.n_edges >= 1 << 20
+    ref = oracle.cc(n, s, d)[0]
+    g = gas.Graph(el)
+    lab1, _ = g.cc()
+    buf = gas.DeviceBuffer(4 * n)
+    g.cc_device(buf.ptr)
+    g.set_vertex_order(0)
+    lab0, _ = g.cc()
+    assert np.array_equal(lab1, ref) and np.array_equal(lab0, ref)
+    assert np.array_equal(buf.to_numpy(), ref)

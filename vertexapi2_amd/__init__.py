@@ -402,10 +402,11 @@ class Graph:
         _check(lib().gas_cc_device(self._h, C.c_void_p(out_ptr), C.byref(st)))
         return st
 
-    def set_pagerank_order(self, mode: int) -> None:
-        """1 (default): PageRank runs on a copy renumbered by descending
-        in-degree (graphs of 2**20+ edges); 0: on the graph as given."""
-        _check(lib().gas_graph_set_pagerank_order(self._h, C.c_int32(mode)))
+    def set_vertex_order(self, mode: int) -> None:
+        """1 (default): PageRank and CC run on a copy renumbered by
+        descending in-degree (graphs of 2**20+ edges); 0: on the graph as
+        given."""
+        _check(lib().gas_graph_set_vertex_order(self._h, C.c_int32(mode)))
 
     def pagerank_device(self, out_ptr: int, iterations: int = 20, residual_ptr=None):
         """Fixed-sweep PageRank with the ranks (and, if given, the per-sweep

@@ -157,11 +157,20 @@ __global__ void k_pr_rank_out(Dev<PageRank> d, const int32_t *new_of, float *out
     out[v] = d.vdata[new_of ? new_of[v] : v].rank;
 }
 
-// CC's initial labels, label[v] = v (connected_component.cu:112-114).
-__global__ void k_cc_init(Dev<CC> d) {
+// CC's initial labels, label[v] = v (connected_component.cu:112-114), in
+// the caller's numbering: old_of[v] is v's original id (NULL: v itself).
+__global__ void k_cc_init(Dev<CC> d, const int32_t *old_of) {
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
-    d.vdata[v] = (int)v;
+    d.vdata[v] = old_of ? old_of[v] : (int)v;
+}
+
+// Labels in the caller's numbering: out[v] = label of v's id in the
+// engine's graph (new_of = NULL: the same numbering).
+__global__ void k_cc_label_out(Dev<CC> d, const int32_t *new_of, int32_t *out) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
+       v += (int64_t)gridDim.x * NT)
+    out[v] = d.vdata[new_of ? new_of[v] : v];
 }
 
 __global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, float *out) {
@@ -205,7 +214,7 @@ static int with_engine(gas_graph_t g, int program, F f) {
         break;
       }
       case GAS_PROGRAM_PAGERANK: {
-        gas_graph_t pg = pagerank_graph(g);
+        gas_graph_t pg = ordered_graph(g);
         std::vector<PageRank::VertexData> vd(pg->nV);
         for (int32_t v = 0; v < pg->nV; ++v)
           vd[v] = {PageRank::pageConst,
@@ -216,10 +225,12 @@ static int with_engine(gas_graph_t g, int program, F f) {
         break;
       }
       case GAS_PROGRAM_CC: {
-        std::vector<int> vd(g->nV);
-        for (int32_t v = 0; v < g->nV; ++v) vd[v] = v;
-        f(cached_engine<CC>(g, program, vd.data()));
-        static_cast<GASEngineGPU<CC> *>(g->engine_cache[engine_set()][program])->detachHost();
+        gas_graph_t cg = ordered_graph(g);
+        std::vector<int> vd(cg->nV);
+        for (int32_t v = 0; v < cg->nV; ++v) vd[v] = cg == g ? v : g->old_of[v];
+        f(cached_engine<CC>(cg, program, vd.data()));
+        static_cast<GASEngineGPU<CC> *>(cg->engine_cache[engine_set()][program])
+            ->detachHost();
         break;
       }
       default:
@@ -372,9 +383,11 @@ int gas_sssp_device(gas_graph_t g, int32_t source, int32_t *dist_dev,
 int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats) {
   if (!g || !label_out) return set_error(GAS_ERR_ARG, "gas_cc: bad arguments");
   return guarded([&] {
-    std::vector<int> vd(g->nV);
-    for (int32_t v = 0; v < g->nV; ++v) vd[v] = v;
-    auto &e = cached_engine<CC>(g, GAS_PROGRAM_CC, vd.data());
+    // labels start as the caller's ids, also on the degree-ordered copy
+    gas_graph_t cg = ordered_graph(g);
+    std::vector<int> vd(cg->nV);
+    for (int32_t v = 0; v < cg->nV; ++v) vd[v] = cg == g ? v : g->old_of[v];
+    auto &e = cached_engine<CC>(cg, GAS_PROGRAM_CC, vd.data());
     e.resetStats();
     RunTimer tm(e.stream());
     e.setActive(0, g->nV);
@@ -382,7 +395,7 @@ int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats) {
     e.runUntilEmpty(true, true);
     double ms = tm.stop();
     e.getResults();
-    std::copy(vd.begin(), vd.end(), label_out);
+    for (int32_t v = 0; v < g->nV; ++v) label_out[v] = vd[cg == g ? v : g->new_of[v]];
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
@@ -395,7 +408,7 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
   if (!g || !rank_out || iterations < 0)
     return set_error(GAS_ERR_ARG, "gas_pagerank: bad arguments");
   return guarded([&] {
-    gas_graph_t pg = pagerank_graph(g);  // g or its degree-ordered copy
+    gas_graph_t pg = ordered_graph(g);  // g or its degree-ordered copy
     std::vector<PageRank::VertexData> vd(pg->nV);
     for (int32_t v = 0; v < pg->nV; ++v) {
       vd[v].rank = PageRank::pageConst;
@@ -420,7 +433,7 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v)
-      rank_out[v] = vd[pg == g ? v : g->pr_new_of[v]].rank;
+      rank_out[v] = vd[pg == g ? v : g->new_of[v]].rank;
     if (residual_out && iterations > 0)
       check(gasrt_memcpy_d2h(residual_out, hist.p,
                              sizeof(double) * (size_t)iterations, e.stream()),
@@ -440,19 +453,21 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
 int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats) {
   if (!g || !label_dev) return set_error(GAS_ERR_ARG, "gas_cc_device: bad arguments");
   return guarded([&] {
-    auto &e = cached_engine<CC>(g, GAS_PROGRAM_CC, nullptr);
+    gas_graph_t cg = ordered_graph(g);  // g or its degree-ordered copy
+    const bool same = cg == g;
+    auto &e = cached_engine<CC>(cg, GAS_PROGRAM_CC, nullptr);
     e.resetStats();
     RunTimer tm(e.stream());
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
-    launch(k_cc_init, grid, e.stream(), e.dev());
+    launch(k_cc_init, grid, e.stream(), e.dev(),
+           same ? (const int32_t *)nullptr : (const int32_t *)g->d_old_of);
     e.stateChanged();
     e.setActive(0, g->nV);
     e.setIteration(0);
     e.runUntilEmpty(true, true);
-    check(gasrt_memcpy_d2d(label_dev, e.deviceVertexData(),
-                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
-          "result");
+    launch(k_cc_label_out, grid, e.stream(), e.dev(),
+           same ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, label_dev);
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -466,7 +481,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
   if (!g || !rank_dev || iterations < 1)
     return set_error(GAS_ERR_ARG, "gas_pagerank_device: bad arguments");
   return guarded([&] {
-    gas_graph_t pg = pagerank_graph(g);  // g or its degree-ordered copy
+    gas_graph_t pg = ordered_graph(g);  // g or its degree-ordered copy
     auto &e = cached_engine<PageRank>(pg, GAS_PROGRAM_PAGERANK, nullptr);
     e.resetStats();
     DevBuf<double> hist;
@@ -483,7 +498,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
       launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
     }
     launch(k_pr_rank_out, grid, e.stream(), e.dev(),
-           pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_pr_new_of,
+           pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of,
            rank_dev);
     double ms = tm.stop();
     if (stats) {

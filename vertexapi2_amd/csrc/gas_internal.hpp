@@ -50,13 +50,14 @@ struct gas_graph {
   static constexpr int kEngineSets = 2;
   void *engine_cache[kEngineSets][8] = {};
   void (*engine_release[kEngineSets][8])(void *) = {};
-  // PageRank's copy of the graph with vertices renumbered by descending
-  // in-degree (gas_graph_set_pagerank_order), built at first use, and
-  // each vertex's id in it (host and device)
-  int32_t pr_order = 1;
-  gas_graph *pr_twin = nullptr;
-  std::vector<int32_t> pr_new_of;
-  int32_t *d_pr_new_of = nullptr;
+  // Copy of the graph with the vertices renumbered by descending in-degree
+  // (gas_graph_set_vertex_order), built at first use by PageRank and CC,
+  // with each vertex's id in it (new_of) and the original id of each of
+  // its vertices (old_of), on the host and the device
+  int32_t vertex_order = 1;
+  gas_graph *ordered = nullptr;
+  std::vector<int32_t> new_of, old_of;
+  int32_t *d_new_of = nullptr, *d_old_of = nullptr;
 };
 
 namespace gas {
@@ -65,11 +66,11 @@ namespace gas {
 // thread chose another; the server's second solo thread uses 1).
 int &engine_set();
 
-// The graph PageRank's whole-run entry points run on: g itself, or its
-// degree-ordered copy (built here at first use) when g->pr_order is set and
-// g has at least kPageRankOrderEdges edges.
-constexpr int64_t kPageRankOrderEdges = (int64_t)1 << 20;
-gas_graph *pagerank_graph(gas_graph *g);
+// The graph PageRank's and CC's whole-run entry points run on: g itself,
+// or its degree-ordered copy (built here at first use) when
+// g->vertex_order is set and g has at least kOrderEdges edges.
+constexpr int64_t kOrderEdges = (int64_t)1 << 20;
+gas_graph *ordered_graph(gas_graph *g);
 
 // Device-side build of one layout (graph_build.hip): offsets (V+1), values
 // and weights (if d_w) in key order, the edge-list index of each slot;

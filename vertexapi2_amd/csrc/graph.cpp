@@ -155,9 +155,9 @@ namespace gas {
 // destinations and, on skewed graphs, the hottest sources share cache lines,
 // and each in-list stays sorted by (new) source id.  The edge list is laid
 // out by new source, so the stable CSC sort keeps every in-list ascending.
-gas_graph *pagerank_graph(gas_graph *g) {
-  if (!g->pr_order || (int64_t)g->nE < kPageRankOrderEdges) return g;
-  if (g->pr_twin) return g->pr_twin;
+gas_graph *ordered_graph(gas_graph *g) {
+  if (!g->vertex_order || (int64_t)g->nE < kOrderEdges) return g;
+  if (g->ordered) return g->ordered;
   const int32_t V = g->nV;
   std::vector<int32_t> order((size_t)V);
   for (int32_t v = 0; v < V; ++v) order[(size_t)v] = v;
@@ -180,17 +180,23 @@ gas_graph *pagerank_graph(gas_graph *g) {
   }
   gas_graph_t twin = nullptr;
   int rc = gas_graph_create(&e, &twin);
-  int32_t *d_new = nullptr;
-  if (!rc) rc = gasrt_malloc((void **)&d_new, sizeof(int32_t) * std::max<size_t>(V, 1));
+  int32_t *d_new = nullptr, *d_old = nullptr;
+  const size_t vb = sizeof(int32_t) * std::max<size_t>(V, 1);
+  if (!rc) rc = gasrt_malloc((void **)&d_new, vb);
+  if (!rc) rc = gasrt_malloc((void **)&d_old, vb);
   if (!rc) rc = gasrt_memcpy_h2d(d_new, new_of.data(), sizeof(int32_t) * (size_t)V, nullptr);
+  if (!rc) rc = gasrt_memcpy_h2d(d_old, order.data(), sizeof(int32_t) * (size_t)V, nullptr);
   if (rc) {
     if (twin) gas_graph_free(twin);
     gasrt_free(d_new);
-    throw std::runtime_error(std::string("PageRank vertex order: ") + gas_last_error());
+    gasrt_free(d_old);
+    throw std::runtime_error(std::string("vertex order: ") + gas_last_error());
   }
-  g->pr_twin = twin;
-  g->pr_new_of.swap(new_of);
-  g->d_pr_new_of = d_new;
+  g->ordered = twin;
+  g->new_of.swap(new_of);
+  g->old_of.swap(order);
+  g->d_new_of = d_new;
+  g->d_old_of = d_old;
   return twin;
 }
 
@@ -460,10 +466,11 @@ static void graph_release(gas_graph *g) {
     for (int i = 0; i < 8; ++i)
       if (g->engine_cache[k][i] && g->engine_release[k][i])
         g->engine_release[k][i](g->engine_cache[k][i]);
-  if (g->pr_twin) gas_graph_free(g->pr_twin);
-  g->pr_twin = nullptr;
-  gasrt_free(g->d_pr_new_of);
-  g->d_pr_new_of = nullptr;
+  if (g->ordered) gas_graph_free(g->ordered);
+  g->ordered = nullptr;
+  gasrt_free(g->d_new_of);
+  gasrt_free(g->d_old_of);
+  g->d_new_of = g->d_old_of = nullptr;
   gasrt_free(g->csc_off);
   gasrt_free(g->csc_src);
   gasrt_free(g->csr_off);
@@ -481,16 +488,19 @@ int gas_graph_set_device_build(int64_t min_edges) {
   return GAS_OK;
 }
 
-int gas_graph_set_pagerank_order(gas_graph_t g, int32_t mode) {
+int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode) {
   if (!g || mode < 0 || mode > 1)
-    return set_error(GAS_ERR_ARG, "gas_graph_set_pagerank_order: bad arguments");
-  if (mode != g->pr_order && g->pr_twin) {
-    gas_graph_free(g->pr_twin);
-    g->pr_twin = nullptr;
-    gasrt_free(g->d_pr_new_of);
-    g->d_pr_new_of = nullptr;
+    return set_error(GAS_ERR_ARG, "gas_graph_set_vertex_order: bad arguments");
+  if (mode != g->vertex_order && g->ordered) {
+    gas_graph_free(g->ordered);
+    g->ordered = nullptr;
+    gasrt_free(g->d_new_of);
+    gasrt_free(g->d_old_of);
+    g->d_new_of = g->d_old_of = nullptr;
+    g->new_of.clear();
+    g->old_of.clear();
   }
-  g->pr_order = mode;
+  g->vertex_order = mode;
   return GAS_OK;
 }
 
