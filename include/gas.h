@@ -198,8 +198,9 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
 int gas_graph_set_device_build(int64_t min_edges);
 /* PageRank's and CC's whole-run entry points (gas_pagerank[_device],
  * gas_cc[_device], and gas_graph_autotune / gas_graph_set_tiers for those
- * programs) on a graph of at least 2^20 edges run on a copy of it with the
- * vertices renumbered by descending in-degree (mode 1, default; built at
+ * programs) on a skewed graph (at least 2^20 edges, largest in-degree at
+ * least 32 times the mean) run on a copy of it with the vertices
+ * renumbered by descending in-degree (mode 1, default; built at
  * first use, results returned in the caller's numbering, CC labels are the
  * caller's ids), or on the graph as given (mode 0).  The order puts hub
  * destinations together and, on skewed graphs, the most-read sources in
@@ -207,6 +208,13 @@ int gas_graph_set_device_build(int64_t min_edges);
  * sums differ only by FP32 rounding order.  No counterpart in the
  * reference. */
 int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode);
+/* The graph the batch engines run on: g's degree-ordered copy (built if
+ * needed) with each of g's vertices' id in it (device and host arrays of
+ * n_vertices, valid while g lives and its order mode is unchanged), or g
+ * itself and NULLs when g is not reordered (mode 0, or not skewed).  Batch
+ * sources and results stay in g's numbering. */
+int gas_graph_ordered_view(gas_graph_t g, gas_graph_t *ordered,
+                           const int32_t **new_of_dev, const int32_t **new_of_host);
 int gas_graph_view_get(gas_graph_t g, gas_graph_view *view);
 /* Host copies for checking: arrays of V+1 / E int32 (any may be NULL). */
 int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,
@@ -388,9 +396,11 @@ int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor);
  * re-relaxations than Bellman-Ford rounds, same (unique) distances.
  * 0 = no bound. */
 int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta);
-/* Batch autotuner: times a 64-lane batch from a fixed sample of sources on
- * this graph for each candidate setting of `kind` and keeps the fastest in
- * the graph -- GAS_BATCH_BFS: pull divisor in {0, 2, 8, 32};
+/* Batch autotuner: times 64-lane batches from six fixed samples of
+ * sources (uniform over the vertices with out-edges) on this graph for each
+ * candidate setting of `kind` and keeps in the graph the fastest in total
+ * among the default (the first candidate) and the settings no slower than
+ * 1.1x the default on any sample -- GAS_BATCH_BFS: pull divisor in {0, 2, 8, 32};
  * GAS_BATCH_SSSP: distance step in {0, w/4, w/2, w} for the largest edge
  * weight w.  Batch engines created on the graph afterwards (gas_batch_create,
  * the server's engines) start with it; param_out (may be NULL) receives it. */

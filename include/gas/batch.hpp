@@ -69,6 +69,7 @@ struct BatchCtl {
 struct BatchDev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst, *csr_w;
+  const int32_t *new_of;  // caller's vertex -> the engine graph's, or NULL
   BatchCtl *ctl;
   uint64_t *vn;                         // BFS [V][2]: visited, next bits
                                         // (one 16-byte pair per vertex)
@@ -550,7 +551,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
                                                      int32_t *out) {
   const uint16_t *row = d.depth16 + (size_t)s * d.rowV;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
-    const uint16_t x = row[v];
+    const uint16_t x = row[d.new_of ? d.new_of[v] : v];
     out[v] = x == 0xffff ? -1 : (int32_t)x;
   }
 }
@@ -696,7 +697,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
 static __global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
                                                       int32_t *out) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
-    out[v] = d.dist[(size_t)v * BW + s];
+    out[v] = d.dist[(size_t)(d.new_of ? d.new_of[v] : v) * BW + s];
 }
 
 // Results of every lane in `lanes` at once, lane s into outs.p[s] (V int32).
@@ -708,9 +709,10 @@ struct LaneOutputs {
 static __global__ __launch_bounds__(NT) void kb_bfs_extract_many(
     BatchDev d, uint64_t lanes, LaneOutputs outs) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
+    const int u = d.new_of ? d.new_of[v] : v;
     for (uint64_t m = lanes; m; m &= m - 1) {
       const int s = __ffsll((unsigned long long)m) - 1;
-      const uint16_t x = d.depth16[(size_t)s * d.rowV + v];
+      const uint16_t x = d.depth16[(size_t)s * d.rowV + u];
       outs.p[s][v] = x == 0xffff ? -1 : (int32_t)x;
     }
   }
@@ -728,7 +730,8 @@ static __global__ __launch_bounds__(NT) void kb_sssp_extract_many(
     const int nv = min(64, d.nV - v0);
     for (int i = threadIdx.x; i < 64 * BW; i += NT) {
       const int j = i / BW, s = i % BW;
-      tile[j][s] = j < nv ? d.dist[(size_t)(v0 + j) * BW + s] : 0;
+      const int u = j < nv ? (d.new_of ? d.new_of[v0 + j] : v0 + j) : 0;
+      tile[j][s] = j < nv ? d.dist[(size_t)u * BW + s] : 0;
     }
     __syncthreads();
     int r = 0;
@@ -749,9 +752,16 @@ class BatchEngine {
  public:
   enum Kind { BFS_KIND = 0, SSSP_KIND = 1 };
 
-  BatchEngine(gas_graph_t g, Kind kind) : m_kind(kind) {
+  // On a skewed graph the engine runs on its degree-ordered copy
+  // (gas_graph_ordered_view); sources and results stay in the caller's
+  // numbering.
+  BatchEngine(gas_graph_t caller, Kind kind) : m_kind(kind) {
     check(gasrt_stream_create(&m_stream), "stream");
     check(gasrt_host_alloc((void **)&m_hctl, sizeof(BatchCtl) * 2), "pinned");
+    gas_graph_t g = nullptr;
+    const int32_t *dnew = nullptr;
+    check(gas_graph_ordered_view(caller, &g, &dnew, &m_hnew), "vertex order");
+    m_dev.new_of = dnew;
     gas_graph_view gv;
     check(gas_graph_view_get(g, &gv), "graph view");
     if (kind == SSSP_KIND && !gv.csr_weights && gv.n_edges)
@@ -831,7 +841,7 @@ class BatchEngine {
     check(gasrt_stream_synchronize(m_stream), "sync");
     // the graph's tuned setting (gas_graph_autotune_batch)
     int32_t param = 0;
-    check(gas_graph_batch_param(g, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP,
+    check(gas_graph_batch_param(caller, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP,
                                 &param),
           "batch param");
     if (kind == BFS_KIND)
@@ -877,8 +887,10 @@ class BatchEngine {
       if ((m_used | reset) & bit) throw Error(GAS_ERR_STATE, "lane in use");
       reset |= bit;
     }
+    int32_t mapped[BW];
+    for (int i = 0; i < n; ++i) mapped[i] = m_hnew ? m_hnew[sources[i]] : sources[i];
     check(gasrt_memcpy_h2d(m_lanes.p, lanes, 4 * (size_t)n, m_stream), "lanes");
-    check(gasrt_memcpy_h2d(m_sources.p, sources, 4 * (size_t)n, m_stream),
+    check(gasrt_memcpy_h2d(m_sources.p, mapped, 4 * (size_t)n, m_stream),
           "sources");
     const unsigned grid = gridFor(m_nV);
     for (int i = 0; i < n; ++i) m_hstart[lanes[i]] = m_round;
@@ -1026,6 +1038,7 @@ class BatchEngine {
   gasrt_event_t m_ev = nullptr;
   BatchCtl *m_hctl = nullptr;  // [0] synchronous reads, [1] snapshots
   int32_t m_nV = 0, m_nE = 0;
+  const int32_t *m_hnew = nullptr;  // caller's vertex -> engine graph's (host)
   int m_par = 0;
   int m_round = 0;
   uint64_t m_used = 0;
@@ -1071,6 +1084,7 @@ struct PprDev {
   int32_t rot;                    // sweep mod 3: nz[rot] is this sweep's input
   float *part_s, *part_c;         // [slots][BW]
   const int32_t *src_lane;        // [BW]
+  const int32_t *new_of;          // caller's vertex -> the engine graph's, or NULL
 };
 
 __device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
@@ -1198,14 +1212,17 @@ static __global__ void kp_seed(PprDev d, int p, const int32_t *lanes,
 static __global__ __launch_bounds__(NT) void kp_extract(PprDev d, int p, int s,
                                                  float *out) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
-    out[v] = d.rank[p][(size_t)v * BW + s];
+    out[v] = d.rank[p][(size_t)(d.new_of ? d.new_of[v] : v) * BW + s];
 }
 
 class PprBatch {
  public:
-  PprBatch(gas_graph_t g, int iterations) : m_iters(iterations) {
+  PprBatch(gas_graph_t caller, int iterations) : m_iters(iterations) {
     if (iterations < 1) throw Error(GAS_ERR_ARG, "PPR needs >= 1 iteration");
     check(gasrt_stream_create(&m_stream), "stream");
+    gas_graph_t g = nullptr;
+    const int32_t *dnew = nullptr;
+    check(gas_graph_ordered_view(caller, &g, &dnew, &m_hnew), "vertex order");
     gas_graph_view gv;
     check(gas_graph_view_get(g, &gv), "graph view");
     m_nV = gv.n_vertices;
@@ -1255,6 +1272,7 @@ class PprBatch {
     PprDev &d = m_dev;
     d.nV = m_nV;
     d.nE = gv.n_edges;
+    d.new_of = dnew;
     d.csc_off = gv.csc_offsets;
     d.csc_src = gv.csc_srcs;
     d.outdeg = m_outdeg.p;
@@ -1298,12 +1316,14 @@ class PprBatch {
       const uint64_t bit = 1ull << lanes[i];
       if ((m_used | reset) & bit) throw Error(GAS_ERR_STATE, "lane in use");
       reset |= bit;
-      m_hsrc[lanes[i]] = sources[i];
+      m_hsrc[lanes[i]] = m_hnew ? m_hnew[sources[i]] : sources[i];
       m_done[lanes[i]] = 0;
     }
+    int32_t mapped[BW];
+    for (int i = 0; i < n; ++i) mapped[i] = m_hnew ? m_hnew[sources[i]] : sources[i];
     check(gasrt_memcpy_h2d(m_src.p, m_hsrc, sizeof(m_hsrc), m_stream), "src");
     check(gasrt_memcpy_h2d(m_lanes.p, lanes, 4 * (size_t)n, m_stream), "lanes");
-    check(gasrt_memcpy_h2d(m_sources.p, sources, 4 * (size_t)n, m_stream),
+    check(gasrt_memcpy_h2d(m_sources.p, mapped, 4 * (size_t)n, m_stream),
           "sources");
     launch(kp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, m_par, reset);
     launch(kp_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
@@ -1386,6 +1406,7 @@ class PprBatch {
   gasrt_stream_t m_stream = nullptr;
   gasrt_event_t m_ev = nullptr;
   int32_t m_nV = 0;
+  const int32_t *m_hnew = nullptr;  // caller's vertex -> engine graph's (host)
   int m_par = 0;
   uint64_t m_used = 0;
   int32_t m_hsrc[BW];

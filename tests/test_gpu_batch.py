@@ -288,3 +288,38 @@ def test_batch_autotune(gpu):
     nw = gas.Graph(gas.Edges.from_arrays(3, [0, 1], [1, 2]))
     with pytest.raises(gas.GasError):
         nw.autotune_batch(gas.BATCH_SSSP)  # no weights
+
+
+@pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP, gas.BATCH_PPR])
+def test_batch_degree_order(gpu, kind):
+    # on a skewed graph of 2**20+ edges the batch engines run on the
+    # degree-ordered copy; sources and results stay in the caller's ids
+    el = workloads.rmat(16, symmetrize=True, weights=True)
+    s, d, w = el.arrays()
+    n = el.n_vertices
+    assert el.n_edges >= 1 << 20
+    src = _sources(np.random.default_rng(4), el, 64)
+    src[0] = 0
+    out = []
+    for mode in (1, 0):
+        g = gas.Graph(el)
+        g.set_vertex_order(mode)
+        b = gas.Batch(g, kind, ppr_iterations=10)
+        b.add(np.arange(64), src)
+        b.run()
+        out.append([b.result(lane) for lane in range(64)])
+    for lane in range(64):
+        if kind == gas.BATCH_PPR:
+            assert rel_l1(out[0][lane], out[1][lane]) <= 1e-5, lane
+        else:
+            assert np.array_equal(out[0][lane], out[1][lane]), lane
+    for lane in (0, 33):
+        if kind == gas.BATCH_BFS:
+            want = oracle.bfs(n, s, d, int(src[lane]))[0]
+        elif kind == gas.BATCH_SSSP:
+            want = oracle.sssp(n, s, d, w, int(src[lane]))[0]
+        else:
+            ref = oracle.check_ppr_fp64(n, s, d, int(src[lane]), 10)
+            assert rel_l1(out[0][lane], ref) <= 1e-5
+            continue
+        assert np.array_equal(out[0][lane], want), lane

@@ -404,7 +404,7 @@ class Graph:
 
     def set_vertex_order(self, mode: int) -> None:
         """1 (default): PageRank and CC run on a copy renumbered by
-        descending in-degree (graphs of 2**20+ edges); 0: on the graph as
+        descending in-degree (skewed graphs of 2**20+ edges); 0: on the graph as
         given."""
         _check(lib().gas_graph_set_vertex_order(self._h, C.c_int32(mode)))
 

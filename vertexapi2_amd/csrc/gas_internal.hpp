@@ -68,8 +68,10 @@ int &engine_set();
 
 // The graph PageRank's and CC's whole-run entry points run on: g itself,
 // or its degree-ordered copy (built here at first use) when
-// g->vertex_order is set and g has at least kOrderEdges edges.
+// g->vertex_order is set, g has at least kOrderEdges edges and its largest
+// in-degree is at least kOrderSkew times the mean.
 constexpr int64_t kOrderEdges = (int64_t)1 << 20;
+constexpr int64_t kOrderSkew = 32;
 gas_graph *ordered_graph(gas_graph *g);
 
 // Device-side build of one layout (graph_build.hip): offsets (V+1), values

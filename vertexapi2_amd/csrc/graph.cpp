@@ -159,6 +159,13 @@ gas_graph *ordered_graph(gas_graph *g) {
   if (!g->vertex_order || (int64_t)g->nE < kOrderEdges) return g;
   if (g->ordered) return g->ordered;
   const int32_t V = g->nV;
+  // only skewed graphs: on near-regular ones (road networks, meshes) the
+  // numbering as given is the local one
+  int32_t max_in = 0;
+  for (int32_t v = 0; v < V; ++v)
+    max_in = std::max(max_in, g->h_csc_off[(size_t)v + 1] - g->h_csc_off[(size_t)v]);
+  if ((int64_t)max_in < kOrderSkew * std::max<int64_t>(1, (int64_t)g->nE / std::max(V, 1)))
+    return g;
   std::vector<int32_t> order((size_t)V);
   for (int32_t v = 0; v < V; ++v) order[(size_t)v] = v;
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
@@ -167,15 +174,24 @@ gas_graph *ordered_graph(gas_graph *g) {
   });
   std::vector<int32_t> new_of((size_t)V);
   for (int32_t k = 0; k < V; ++k) new_of[(size_t)order[(size_t)k]] = k;
+  // edge weights (kept on the device only) in CSR order
+  std::vector<int32_t> w;
+  if (g->csr_w && g->nE) {
+    w.resize((size_t)g->nE);
+    if (gasrt_memcpy_d2h(w.data(), g->csr_w, sizeof(int32_t) * (size_t)g->nE, nullptr))
+      throw std::runtime_error(std::string("vertex order: ") + gas_last_error());
+  }
   gas_edges e;
   e.nV = V;
   e.src.reserve((size_t)g->nE);
   e.dst.reserve((size_t)g->nE);
+  if (!w.empty()) e.w.reserve((size_t)g->nE);
   for (int32_t k = 0; k < V; ++k) {
     const int32_t u = order[(size_t)k];
     for (int32_t j = g->h_csr_off[(size_t)u]; j < g->h_csr_off[(size_t)u + 1]; ++j) {
       e.src.push_back(k);
       e.dst.push_back(new_of[(size_t)g->h_csr_dst[(size_t)j]]);
+      if (!w.empty()) e.w.push_back(w[(size_t)j]);
     }
   }
   gas_graph_t twin = nullptr;
@@ -486,6 +502,19 @@ int gas_graph_set_device_build(int64_t min_edges) {
     return set_error(GAS_ERR_ARG, "gas_graph_set_device_build: negative edge count");
   g_device_build_min_edges = min_edges;
   return GAS_OK;
+}
+
+int gas_graph_ordered_view(gas_graph_t g, gas_graph_t *ordered,
+                           const int32_t **new_of_dev, const int32_t **new_of_host) {
+  if (!g || !ordered || !new_of_dev || !new_of_host)
+    return set_error(GAS_ERR_ARG, "gas_graph_ordered_view: null");
+  return gas::host_guard("gas_graph_ordered_view", [&]() -> int {
+    gas_graph *og = gas::ordered_graph(g);
+    *ordered = og;
+    *new_of_dev = og == g ? nullptr : g->d_new_of;
+    *new_of_host = og == g ? nullptr : g->new_of.data();
+    return GAS_OK;
+  });
 }
 
 int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode) {
