@@ -196,17 +196,18 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
  * builds every graph on the GPU.  No counterpart in the reference, which
  * sorts on the host (util.h:68). */
 int gas_graph_set_device_build(int64_t min_edges);
-/* PageRank's and CC's whole-run entry points (gas_pagerank[_device],
- * gas_cc[_device], and gas_graph_autotune / gas_graph_set_tiers for those
- * programs) on a skewed graph (at least 2^20 edges, largest in-degree at
+/* The whole-run entry points of BFS, SSSP, PageRank and CC
+ * (gas_bfs[_device], gas_sssp[_device], gas_pagerank[_device],
+ * gas_cc[_device], and gas_graph_autotune / gas_graph_set_tiers for them)
+ * and the batch engines on a skewed graph (at least 2^20 edges, largest in-degree at
  * least 32 times the mean) run on a copy of it with the vertices
  * renumbered by descending in-degree (mode 1, default; built at
  * first use, results returned in the caller's numbering, CC labels are the
- * caller's ids), or on the graph as given (mode 0).  The order puts hub
- * destinations together and, on skewed graphs, the most-read sources in
- * shared cache lines.  CC labels are identical in both modes; PageRank
- * sums differ only by FP32 rounding order.  No counterpart in the
- * reference. */
+ * caller's ids, sources are mapped), or on the graph as given (mode 0).
+ * The order puts hub destinations together and, on skewed graphs, the
+ * most-read sources in shared cache lines.  Integer results are identical
+ * in both modes; PageRank sums differ only by FP32 rounding order.  No
+ * counterpart in the reference. */
 int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode);
 /* The graph the batch engines run on: g's degree-ordered copy (built if
  * needed) with each of g's vertices' id in it (device and host arrays of

@@ -387,19 +387,28 @@ def test_pagerank_degree_order(gpu):
 
 
 @pytest.mark.parametrize("sym", [False, True])
-def test_cc_degree_order(gpu, sym):
-    # CC on the degree-ordered copy starts from the caller's ids as labels:
-    # labels identical to the graph as given and to the oracle
-    el = workloads.rmat(17, symmetrize=sym)
-    s, d, _ = el.arrays()
+def test_degree_order_integer_programs(gpu, sym):
+    # BFS, SSSP and CC on the degree-ordered copy: sources mapped, results
+    # in the caller's numbering, CC labels starting as the caller's ids --
+    # identical to the graph as given and to the oracle
+    el = workloads.rmat(17, symmetrize=sym, weights=True)
+    s, d, w = el.arrays()
     n = el.n_vertices
     assert el.n_edges >= 1 << 20
-    ref = oracle.cc(n, s, d)[0]
     g = gas.Graph(el)
-    lab1, _ = g.cc()
+    g0 = gas.Graph(el)
+    g0.set_vertex_order(0)
     buf = gas.DeviceBuffer(4 * n)
+    ref = oracle.cc(n, s, d)[0]
     g.cc_device(buf.ptr)
-    g.set_vertex_order(0)
-    lab0, _ = g.cc()
-    assert np.array_equal(lab1, ref) and np.array_equal(lab0, ref)
+    assert np.array_equal(g.cc()[0], ref) and np.array_equal(g0.cc()[0], ref)
     assert np.array_equal(buf.to_numpy(), ref)
+    for src in (0, 777, n - 1):
+        want = oracle.bfs(n, s, d, src)[0]
+        assert np.array_equal(g.bfs(src)[0], want) and np.array_equal(g0.bfs(src)[0], want)
+        g.bfs_device(src, buf.ptr)
+        assert np.array_equal(buf.to_numpy(), want)
+        want = oracle.sssp(n, s, d, w, src)[0]
+        assert np.array_equal(g.sssp(src)[0], want) and np.array_equal(g0.sssp(src)[0], want)
+        g.sssp_device(src, buf.ptr)
+        assert np.array_equal(buf.to_numpy(), want)

@@ -165,6 +165,27 @@ __global__ void k_cc_init(Dev<CC> d, const int32_t *old_of) {
     d.vdata[v] = old_of ? old_of[v] : (int)v;
 }
 
+// A per-vertex int32 result in the caller's numbering: out[v] = in[new_of[v]].
+__global__ void k_gather_result(const int32_t *in, const int32_t *new_of, int32_t *out,
+                                int32_t n) {
+  for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * NT)
+    out[v] = in[new_of[v]];
+}
+
+// Copies a per-vertex int32 result of the engine on `og` (g or its
+// degree-ordered copy) to the caller's device array, in g's numbering.
+static void result_to_caller(gas_graph_t g, gas_graph_t og, const int32_t *in,
+                             int32_t *out, gasrt_stream_t s) {
+  if (og == g) {
+    check(gasrt_memcpy_d2d(out, in, sizeof(int32_t) * (size_t)g->nV, s), "result");
+    return;
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>(
+      2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
+  launch(k_gather_result, grid, s, in, (const int32_t *)g->d_new_of, out, g->nV);
+}
+
 // Labels in the caller's numbering: out[v] = label of v's id in the
 // engine's graph (new_of = NULL: the same numbering).
 __global__ void k_cc_label_out(Dev<CC> d, const int32_t *new_of, int32_t *out) {
@@ -200,17 +221,21 @@ static int with_engine(gas_graph_t g, int program, F f) {
   return guarded([&] {
     switch (program) {
       case GAS_PROGRAM_BFS: {
+        gas_graph_t og = ordered_graph(g);
         std::vector<BFS::VertexData> vd(g->nV, BFS::VertexData{-1});
-        f(cached_engine<BFS>(g, program, vd.data()));
-        static_cast<GASEngineGPU<BFS> *>(g->engine_cache[engine_set()][program])->detachHost();
+        f(cached_engine<BFS>(og, program, vd.data()));
+        static_cast<GASEngineGPU<BFS> *>(og->engine_cache[engine_set()][program])
+            ->detachHost();
         break;
       }
       case GAS_PROGRAM_SSSP: {
         if (!g->csc_w && g->nE > 0)
           throw Error(GAS_ERR_ARG, "graph has no edge weights");
+        gas_graph_t og = ordered_graph(g);
         std::vector<int> vd(g->nV, SSSP::gatherZero);
-        f(cached_engine<SSSP>(g, program, vd.data()));
-        static_cast<GASEngineGPU<SSSP> *>(g->engine_cache[engine_set()][program])->detachHost();
+        f(cached_engine<SSSP>(og, program, vd.data()));
+        static_cast<GASEngineGPU<SSSP> *>(og->engine_cache[engine_set()][program])
+            ->detachHost();
         break;
       }
       case GAS_PROGRAM_PAGERANK: {
@@ -289,17 +314,20 @@ int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
   if (!g || !depth_out || source < 0 || source >= g->nV)
     return set_error(GAS_ERR_ARG, "gas_bfs: bad arguments");
   return guarded([&] {
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
+    const int32_t src = og == g ? source : g->new_of[source];
     std::vector<BFS::VertexData> vd(g->nV);
     for (auto &x : vd) x.depth = -1;
-    auto &e = cached_engine<BFS>(g, GAS_PROGRAM_BFS, vd.data());
+    auto &e = cached_engine<BFS>(og, GAS_PROGRAM_BFS, vd.data());
     e.resetStats();
     RunTimer tm(e.stream());
-    e.setActive(source, source + 1);
+    e.setActive(src, src + 1);
     e.setIteration(0);
     e.runUntilEmpty(false, false);
     double ms = tm.stop();
     e.getResults();
-    for (int32_t v = 0; v < g->nV; ++v) depth_out[v] = vd[v].depth;
+    for (int32_t v = 0; v < g->nV; ++v)
+      depth_out[v] = vd[og == g ? v : g->new_of[v]].depth;
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
@@ -312,16 +340,17 @@ int gas_bfs_device(gas_graph_t g, int32_t source, int32_t *depth_dev,
   if (!g || !depth_dev || source < 0 || source >= g->nV)
     return set_error(GAS_ERR_ARG, "gas_bfs_device: bad arguments");
   return guarded([&] {
-    auto &e = cached_engine<BFS>(g, GAS_PROGRAM_BFS, nullptr);
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
+    const int32_t src = og == g ? source : g->new_of[source];
+    auto &e = cached_engine<BFS>(og, GAS_PROGRAM_BFS, nullptr);
     e.resetStats();
     RunTimer tm(e.stream());
     e.fillVertexData(BFS::VertexData{-1}, -1, BFS::VertexData{-1});
-    e.setActive(source, source + 1);
+    e.setActive(src, src + 1);
     e.setIteration(0);
     e.runUntilEmpty(false, false);
-    check(gasrt_memcpy_d2d(depth_dev, e.deviceVertexData(),
-                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
-          "result");
+    result_to_caller(g, og, reinterpret_cast<const int32_t *>(e.deviceVertexData()),
+                     depth_dev, e.stream());
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -337,9 +366,10 @@ int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
   if (!g->csc_w && g->nE > 0)
     return set_error(GAS_ERR_ARG, "gas_sssp: graph has no edge weights");
   return guarded([&] {
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
     std::vector<int> vd(g->nV, SSSP::gatherZero);
-    vd[source] = 0;
-    auto &e = cached_engine<SSSP>(g, GAS_PROGRAM_SSSP, vd.data());
+    vd[og == g ? source : g->new_of[source]] = 0;
+    auto &e = cached_engine<SSSP>(og, GAS_PROGRAM_SSSP, vd.data());
     e.resetStats();
     RunTimer tm(e.stream());
     e.setActive(0, g->nV);
@@ -347,7 +377,7 @@ int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
     e.runUntilEmpty(true, true);
     double ms = tm.stop();
     e.getResults();
-    std::copy(vd.begin(), vd.end(), dist_out);
+    for (int32_t v = 0; v < g->nV; ++v) dist_out[v] = vd[og == g ? v : g->new_of[v]];
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
@@ -362,16 +392,15 @@ int gas_sssp_device(gas_graph_t g, int32_t source, int32_t *dist_dev,
   if (!g->csc_w && g->nE > 0)
     return set_error(GAS_ERR_ARG, "gas_sssp_device: graph has no edge weights");
   return guarded([&] {
-    auto &e = cached_engine<SSSP>(g, GAS_PROGRAM_SSSP, nullptr);
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
+    auto &e = cached_engine<SSSP>(og, GAS_PROGRAM_SSSP, nullptr);
     e.resetStats();
     RunTimer tm(e.stream());
-    e.fillVertexData(SSSP::gatherZero, source, 0);
+    e.fillVertexData(SSSP::gatherZero, og == g ? source : g->new_of[source], 0);
     e.setActive(0, g->nV);
     e.setIteration(0);
     e.runUntilEmpty(true, true);
-    check(gasrt_memcpy_d2d(dist_dev, e.deviceVertexData(),
-                           sizeof(int32_t) * (size_t)g->nV, e.stream()),
-          "result");
+    result_to_caller(g, og, e.deviceVertexData(), dist_dev, e.stream());
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
