@@ -206,8 +206,9 @@ int gas_graph_set_device_build(int64_t min_edges);
  * caller's ids, sources are mapped), or on the graph as given (mode 0).
  * The order puts hub destinations together and, on skewed graphs, the
  * most-read sources in shared cache lines.  Integer results are identical
- * in both modes; PageRank sums differ only by FP32 rounding order.  No
- * counterpart in the reference. */
+ * in both modes; PageRank sums differ only by FP32 rounding order.
+ * Changing the mode frees the ordered copy: free the batch engines and
+ * servers created on g before.  No counterpart in the reference. */
 int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode);
 /* The graph the batch engines run on: g's degree-ordered copy (built if
  * needed) with each of g's vertices' id in it (device and host arrays of
