@@ -55,6 +55,7 @@ struct gas_graph {
   // with each vertex's id in it (new_of) and the original id of each of
   // its vertices (old_of), on the host and the device
   int32_t vertex_order = 1;
+  bool order_unskewed = false;  // checked once: too regular to reorder
   gas_graph *ordered = nullptr;
   std::vector<int32_t> new_of, old_of;
   int32_t *d_new_of = nullptr, *d_old_of = nullptr;

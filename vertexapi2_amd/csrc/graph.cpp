@@ -156,7 +156,7 @@ namespace gas {
 // and each in-list stays sorted by (new) source id.  The edge list is laid
 // out by new source, so the stable CSC sort keeps every in-list ascending.
 gas_graph *ordered_graph(gas_graph *g) {
-  if (!g->vertex_order || (int64_t)g->nE < kOrderEdges) return g;
+  if (!g->vertex_order || (int64_t)g->nE < kOrderEdges || g->order_unskewed) return g;
   if (g->ordered) return g->ordered;
   const int32_t V = g->nV;
   // only skewed graphs: on near-regular ones (road networks, meshes) the
@@ -164,8 +164,10 @@ gas_graph *ordered_graph(gas_graph *g) {
   int32_t max_in = 0;
   for (int32_t v = 0; v < V; ++v)
     max_in = std::max(max_in, g->h_csc_off[(size_t)v + 1] - g->h_csc_off[(size_t)v]);
-  if ((int64_t)max_in < kOrderSkew * std::max<int64_t>(1, (int64_t)g->nE / std::max(V, 1)))
+  if ((int64_t)max_in < kOrderSkew * std::max<int64_t>(1, (int64_t)g->nE / std::max(V, 1))) {
+    g->order_unskewed = true;  // the per-query entry points ask every call
     return g;
+  }
   std::vector<int32_t> order((size_t)V);
   for (int32_t v = 0; v < V; ++v) order[(size_t)v] = v;
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
