@@ -313,6 +313,15 @@ def test_batch_degree_order(gpu, kind):
             assert rel_l1(out[0][lane], out[1][lane]) <= 1e-5, lane
         else:
             assert np.array_equal(out[0][lane], out[1][lane]), lane
+    if kind == gas.BATCH_PPR:
+        # the plain engine's PPR on the ordered copy, host and device results
+        g = gas.Graph(el)
+        ref = oracle.check_ppr_fp64(n, s, d, int(src[5]), 10)
+        r, _ = g.ppr(int(src[5]), 10)
+        assert rel_l1(r, ref) <= 1e-5
+        buf = gas.DeviceBuffer(4 * n)
+        g.ppr_device(int(src[5]), buf.ptr, 10)
+        assert rel_l1(buf.to_numpy().view(np.float32), ref) <= 1e-5
     for lane in (0, 33):
         if kind == gas.BATCH_BFS:
             want = oracle.bfs(n, s, d, int(src[lane]))[0]

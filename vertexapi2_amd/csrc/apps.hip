@@ -194,10 +194,12 @@ __global__ void k_cc_label_out(Dev<CC> d, const int32_t *new_of, int32_t *out) {
     out[v] = d.vdata[new_of ? new_of[v] : v];
 }
 
-__global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, float *out) {
+// Ranks in the caller's numbering (new_of = NULL: the same numbering).
+__global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, const int32_t *new_of,
+                               float *out) {
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
-    out[v] = d.vdata[v].rank;
+    out[v] = d.vdata[new_of ? new_of[v] : v].rank;
 }
 
 }  // namespace gas
@@ -546,13 +548,15 @@ int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
     return set_error(GAS_ERR_ARG, "gas_ppr: bad arguments");
   return guarded([&] {
     typedef PersonalizedPageRank PPR;
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
+    const int32_t src = og == g ? source : g->new_of[source];
     std::vector<PPR::VertexData> vd(g->nV);
     for (int32_t v = 0; v < g->nV; ++v) {
-      vd[v].rank = v == source ? 0.15f : 0.0f;
-      vd[v].numOutEdges = g->h_csr_off[v + 1] - g->h_csr_off[v];
-      vd[v].isSource = v == source;
+      vd[v].rank = v == src ? 0.15f : 0.0f;
+      vd[v].numOutEdges = og->h_csr_off[v + 1] - og->h_csr_off[v];
+      vd[v].isSource = v == src;
     }
-    auto &e = cached_engine<PPR>(g, 4, vd.data());
+    auto &e = cached_engine<PPR>(og, 4, vd.data());
     e.resetStats();
     RunTimer tm(e.stream());
     for (int it = 0; it < iterations; ++it) {
@@ -561,7 +565,7 @@ int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
     }
     double ms = tm.stop();
     e.getResults();
-    for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[v].rank;
+    for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[og == g ? v : g->new_of[v]].rank;
     if (stats) {
       e.stats(stats);
       stats->ms = ms;
@@ -577,18 +581,20 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
     return set_error(GAS_ERR_ARG, "gas_ppr_device: bad arguments");
   return guarded([&] {
     typedef PersonalizedPageRank PPR;
-    auto &e = cached_engine<PPR>(g, 4, nullptr);
+    gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
+    auto &e = cached_engine<PPR>(og, 4, nullptr);
     e.resetStats();
     RunTimer tm(e.stream());
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
-    launch(k_ppr_init, grid, e.stream(), e.dev(), source);
+    launch(k_ppr_init, grid, e.stream(), e.dev(), og == g ? source : g->new_of[source]);
     e.stateChanged();
     for (int it = 0; it < iterations; ++it) {
       e.setActive(0, g->nV);
       e.gatherApply(true);
     }
-    launch(k_ppr_rank_out, grid, e.stream(), e.dev(), rank_dev);
+    launch(k_ppr_rank_out, grid, e.stream(), e.dev(),
+           og == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, rank_dev);
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
