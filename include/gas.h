@@ -196,10 +196,10 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out);
  * builds every graph on the GPU.  No counterpart in the reference, which
  * sorts on the host (util.h:68). */
 int gas_graph_set_device_build(int64_t min_edges);
-/* The whole-run entry points of BFS, SSSP, PageRank and CC
- * (gas_bfs[_device], gas_sssp[_device], gas_pagerank[_device],
- * gas_cc[_device], and gas_graph_autotune / gas_graph_set_tiers for them)
- * and the batch engines on a skewed graph (at least 2^20 edges, largest in-degree at
+/* The whole-run entry points of BFS, SSSP, PageRank, CC and personalised
+ * PageRank (gas_bfs[_device], gas_sssp[_device], gas_pagerank[_device],
+ * gas_cc[_device], gas_ppr[_device], and gas_graph_autotune /
+ * gas_graph_set_tiers for them) and the batch engines on a skewed graph (at least 2^20 edges, largest in-degree at
  * least 32 times the mean) run on a copy of it with the vertices
  * renumbered by descending in-degree (mode 1, default; built at
  * first use, results returned in the caller's numbering, CC labels are the
