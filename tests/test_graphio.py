@@ -259,7 +259,7 @@ def test_loaders_match_reference(tmp_path, want_values):
     with open(SMALL, "rb") as f, gzip.open(gz, "wb") as g:
         shutil.copyfileobj(f, g)
     paths.append(str(gz))
-    # a .gr with values written by this build, read back by the reference
+    # a .gr with values written by gas_write_gr, read back by the reference
     gr = str(tmp_path / "small.gr")
     gas.Edges.load_mtx(SMALL, True).write_gr(gr)
     paths.append(gr)
