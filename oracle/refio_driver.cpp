@@ -1,6 +1,7 @@
 // C entry points over the reference's own graph loaders and writers
 // (graphio.cpp, compiled from its source by oracle/Makefile), so the tests
-// can compare the build's loaders with them.  Test infrastructure only.
+// can compare the gas_load_* / gas_write_* entry points with them.  Test
+// infrastructure only.
 // The reference's loaders exit(1) on malformed input: call them with
 // well-formed files only.
 #include <cstdlib>
