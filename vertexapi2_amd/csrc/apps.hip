@@ -451,8 +451,10 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
     hist.alloc(iterations > 0 ? (size_t)iterations : 1);
     RunTimer tm(e.stream());
     if (iterations > 0) {
+      // every vertex stays active: one setActive serves all sweeps (a
+      // gatherApply without a scatter leaves the input as it was)
+      e.setActive(0, g->nV);
       for (int it = 0; it < iterations; ++it) {
-        e.setActive(0, g->nV);
         e.gatherApply(true);
         launch(k_residual_sum, 1, e.stream(), e.dev().rpart, hist.p, it);
       }
@@ -523,8 +525,8 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_pr_init, grid, e.stream(), e.dev());
     e.stateChanged();
+    e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
     for (int it = 0; it < iterations; ++it) {
-      e.setActive(0, g->nV);
       e.gatherApply(true);
       launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
     }
@@ -559,10 +561,8 @@ int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
     auto &e = cached_engine<PPR>(og, 4, vd.data());
     e.resetStats();
     RunTimer tm(e.stream());
-    for (int it = 0; it < iterations; ++it) {
-      e.setActive(0, g->nV);
-      e.gatherApply(true);
-    }
+    e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
+    for (int it = 0; it < iterations; ++it) e.gatherApply(true);
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[og == g ? v : g->new_of[v]].rank;
@@ -589,10 +589,8 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_ppr_init, grid, e.stream(), e.dev(), og == g ? source : g->new_of[source]);
     e.stateChanged();
-    for (int it = 0; it < iterations; ++it) {
-      e.setActive(0, g->nV);
-      e.gatherApply(true);
-    }
+    e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
+    for (int it = 0; it < iterations; ++it) e.gatherApply(true);
     launch(k_ppr_rank_out, grid, e.stream(), e.dev(),
            og == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, rank_dev);
     double ms = tm.stop();
