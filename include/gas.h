@@ -328,8 +328,9 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
                  double *residual_out, gas_run_stats *stats);
 /* The same two runs with the result left in HBM (label_dev: n_vertices
  * int32; rank_dev: n_vertices floats, residual_dev: `iterations` doubles or
- * NULL); the initial state is written on the device.  PageRank here runs
- * fixed sweeps only (iterations >= 1). */
+ * NULL, in which case the per-sweep residual is not reduced); the initial
+ * state is written on the device.  PageRank here runs fixed sweeps only
+ * (iterations >= 1). */
 int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats);
 int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
                         double *residual_dev, gas_run_stats *stats);

@@ -517,9 +517,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
     gas_graph_t pg = ordered_graph(g);  // g or its degree-ordered copy
     auto &e = cached_engine<PageRank>(pg, GAS_PROGRAM_PAGERANK, nullptr);
     e.resetStats();
-    DevBuf<double> hist;
-    if (!residual_dev) hist.alloc((size_t)iterations);
-    double *res = residual_dev ? residual_dev : hist.p;
+    double *res = residual_dev;  // residual sums only when asked for
     RunTimer tm(e.stream());
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
@@ -528,7 +526,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
     e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
     for (int it = 0; it < iterations; ++it) {
       e.gatherApply(true);
-      launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
+      if (res) launch(k_residual_sum, 1, e.stream(), e.dev().rpart, res, it);
     }
     launch(k_pr_rank_out, grid, e.stream(), e.dev(),
            pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of,
