@@ -211,6 +211,60 @@ def check_ppr_fp64(nV, srcs, dsts, source, iterations):
     return out
 
 
+def check_pagerank_tol_fp64(nV, srcs, dsts, tol=0.01):
+    """The tolerance-driven PageRank run (pagerank.cu:85-101: every vertex
+    active, then refgas.h:215-222's loop -- gather over in-edges, apply
+    rank = 0.15 + 0.85 sum, activate the out-neighbours of vertices whose rank
+    moved by >= tol, next frontier) evaluated in FP64 with numpy: (ranks,
+    supersteps).  Independent of the C restatement; where its activation
+    decisions match the FP32 engines' (equal superstep counts), it is the
+    exact value their FP32 arithmetic approximates."""
+    s = np.asarray(srcs, np.int64)
+    d = np.asarray(dsts, np.int64)
+    outdeg = np.bincount(s, minlength=nV).astype(np.float64)
+    inv = np.zeros(nV)
+    inv[outdeg > 0] = 1.0 / outdeg[outdeg > 0]
+    rank = np.full(nV, 0.15)
+    active = np.arange(nV)
+    steps = 0
+    while len(active):
+        sums = np.bincount(d, weights=(rank * inv)[s], minlength=nV)
+        new = 0.15 + 0.85 * sums[active]
+        ret = np.abs(new - rank[active]) >= tol
+        rank[active] = new
+        returned = np.zeros(nV, bool)
+        returned[active[ret]] = True
+        flags = np.zeros(nV, bool)
+        flags[d[returned[s]]] = True
+        active = np.nonzero(flags)[0]
+        steps += 1
+    return rank, steps
+
+
+def edge_list_to_csc(nV, srcs, dsts):
+    """CSC of util.h:114-121: (offsets, sources, edge index) by destination,
+    equal destinations in edge-list order."""
+    s, d = _i32(srcs), _i32(dsts)
+    off = np.zeros(nV + 1, np.int32)
+    out = np.zeros(len(s), np.int32)
+    eid = np.zeros(len(s), np.int32)
+    lib().gasref_edge_list_to_csc(nV, len(s), _p(s), _p(d), _p(off), _p(out), _p(eid))
+    return off, out, eid
+
+
+def cert_bfs(nV, srcs, dsts, source, depth) -> bool:
+    """True iff depth is exactly the BFS depth from source (one pass)."""
+    s, d, x = _i32(srcs), _i32(dsts), _i32(depth)
+    return bool(lib().gasref_cert_bfs(nV, len(s), _p(s), _p(d), source, _p(x)))
+
+
+def cert_sssp(nV, srcs, dsts, weights, source, dist) -> bool:
+    """True iff dist is exactly the shortest-path distance from source
+    (unreached 2147383647; positive weights), in one pass."""
+    s, d, w, x = _i32(srcs), _i32(dsts), _i32(weights), _i32(dist)
+    return bool(lib().gasref_cert_sssp(nV, len(s), _p(s), _p(d), _p(w), source, _p(x)))
+
+
 def edge_list_to_csr(nV, srcs, dsts):
     s, d = _i32(srcs), _i32(dsts)
     off = np.zeros(nV + 1, np.int32)
@@ -247,6 +301,45 @@ def ref_pagerank(nV, srcs, dsts, iterations):
     out = np.zeros(nV, np.float32)
     us = ref().ref_pagerank(nV, len(s), _p(s), _p(d), iterations, _p(out, C.c_float))
     return out, us
+
+
+def ref_pagerank_run(nV, srcs, dsts):
+    """The reference engine's tolerance-driven PageRank (run(), tol 0.01):
+    (ranks, supersteps)."""
+    s, d = _i32(srcs), _i32(dsts)
+    out = np.zeros(nV, np.float32)
+    steps = ref().ref_pagerank_run(nV, len(s), _p(s), _p(d), _p(out, C.c_float))
+    return out, steps
+
+
+class RefPrepared:
+    """An engine of the reference (compiled refgas.h) bound to a graph once
+    and reused query after query: kind 0 BFS, 1 SSSP, 2 personalised
+    PageRank (`iterations` sweeps)."""
+
+    def __init__(self, kind, nV, srcs, dsts, weights=None):
+        r = ref()
+        r.ref_prepare.restype = C.c_void_p
+        r.ref_prepared_query.restype = C.c_int64
+        r.ref_prepared_query.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        r.ref_prepared_free.argtypes = [C.c_void_p]
+        s, d = _i32(srcs), _i32(dsts)
+        w = _i32(weights) if weights is not None else None
+        self.kind, self.nV = kind, nV
+        self.h = r.ref_prepare(kind, nV, len(s), _p(s), _p(d),
+                               _p(w) if w is not None else None)
+
+    def query(self, source, iterations=10):
+        """(result, microseconds of the reference's Took window)."""
+        out = np.zeros(self.nV, np.float32 if self.kind == 2 else np.int32)
+        us = ref().ref_prepared_query(self.h, int(source), int(iterations),
+                                      out.ctypes.data_as(C.c_void_p))
+        return out, us
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_prepared_free(self.h)
+            self.h = None
 
 
 def ref_edge_list_to_csr(nV, srcs, dsts):

@@ -490,3 +490,56 @@ int gasref_check_ppr_fp64(int nV, int nE, const int32_t *srcs,
   if (source < 0 || source >= nV) return 1;
   return power_iteration(nV, nE, srcs, dsts, source, iterations, rank);
 }
+
+/* ---- certificates (one pass over the edge list, no search) --------------- */
+/* 1 iff depth is the BFS depth from source: depth[source] = 0; every edge
+ * out of a reached vertex u ends at a reached vertex of depth <= depth[u] +
+ * 1; every other reached vertex has an in-edge from depth - 1.  Any array
+ * that passes is the unique BFS depth vector (induction on depth). */
+int gasref_cert_bfs(int nV, int nE, const int32_t *srcs, const int32_t *dsts,
+                    int source, const int32_t *depth) {
+  if (source < 0 || source >= nV || depth[source] != 0) return 0;
+  unsigned char *parent = (unsigned char *)calloc((size_t)(nV ? nV : 1), 1);
+  int ok = 1;
+  for (int k = 0; k < nE && ok; ++k) {
+    int32_t du = depth[srcs[k]], dv = depth[dsts[k]];
+    if (du < 0) continue;
+    if (dv < 0 || dv > du + 1) ok = 0;
+    else if (dv == du + 1) parent[dsts[k]] = 1;
+  }
+  for (int v = 0; v < nV && ok; ++v) {
+    if (depth[v] < -1) ok = 0;
+    else if (v != source && depth[v] >= 0 && !parent[v]) ok = 0;
+  }
+  free(parent);
+  return ok;
+}
+
+/* 1 iff dist is the shortest-path distance from source under positive
+ * weights with 2147383647 for unreached vertices: dist[source] = 0; no edge
+ * out of a reached vertex can lower its head; every other reached vertex
+ * has a tight in-edge (dist[v] = dist[u] + w).  With positive weights the
+ * tight edges form a tree rooted at source, so a passing array is the
+ * unique distance vector. */
+int gasref_cert_sssp(int nV, int nE, const int32_t *srcs, const int32_t *dsts,
+                     const int32_t *weights, int source, const int32_t *dist) {
+  const int32_t inf = INT_MAX - 100000;
+  if (source < 0 || source >= nV || dist[source] != 0) return 0;
+  unsigned char *parent = (unsigned char *)calloc((size_t)(nV ? nV : 1), 1);
+  int ok = 1;
+  for (int k = 0; k < nE && ok; ++k) {
+    int32_t du = dist[srcs[k]];
+    if (du >= inf) continue;
+    if (weights[k] <= 0) { ok = 0; break; }
+    int64_t cand = (int64_t)du + weights[k];
+    int64_t dv = dist[dsts[k]];
+    if (dv > cand) ok = 0;
+    else if (dv == cand) parent[dsts[k]] = 1;
+  }
+  for (int v = 0; v < nV && ok; ++v) {
+    if (dist[v] < 0 || dist[v] > inf) ok = 0;
+    else if (v != source && dist[v] < inf && !parent[v]) ok = 0;
+  }
+  free(parent);
+  return ok;
+}

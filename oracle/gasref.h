@@ -71,6 +71,14 @@ int gasref_check_ppr_fp64(int nV, int nE, const int32_t *srcs,
                           const int32_t *dsts, int source, int iterations,
                           double *rank);
 
+/* Certificates: 1 iff the array is the exact BFS depth / shortest-path
+ * distance (unreached 2147383647) from source, checked in one pass over the
+ * edge list (properties that fix the answer uniquely; no search). */
+int gasref_cert_bfs(int nV, int nE, const int32_t *srcs, const int32_t *dsts,
+                    int source, const int32_t *depth);
+int gasref_cert_sssp(int nV, int nE, const int32_t *srcs, const int32_t *dsts,
+                     const int32_t *weights, int source, const int32_t *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "refgas.h"
@@ -108,6 +109,48 @@ struct CC {
     return changed;
   }
   static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+// Personalised PageRank (no counterpart in the reference): the PageRank
+// struct above with the teleport 0.15 paid only at the source, run as fixed
+// all-active sweeps -- the CPU side of the served stream's PPR queries.
+struct PPR {
+  struct VertexData {
+    float rank;
+    int numOutEdges;
+    int isSource;
+  };
+  struct EdgeData {};
+  typedef float GatherResult;
+  static constexpr float gatherZero = 0.0f;
+  static float gatherMap(const VertexData *, const VertexData *src,
+                         const EdgeData *) {
+    return src->rank / src->numOutEdges;
+  }
+  static float gatherReduce(const float &left, const float &right) {
+    return left + right;
+  }
+  static bool apply(VertexData *v, const float &sum) {
+    v->rank = (v->isSource ? 0.15f : 0.0f) + 0.85f * sum;
+    return false;
+  }
+  static void scatter(const VertexData *, const VertexData *, EdgeData *) {}
+};
+
+// One engine of the reference bound to a graph once (setGraph outside any
+// timed window) and reused query after query: the vertex array it retains
+// (refgas.h:90) is re-initialised in place before each run.
+struct Prepared {
+  int kind = 0;  // 0 BFS, 1 SSSP, 2 PPR
+  int nV = 0;
+  std::vector<int> weights;
+  std::vector<int> outdeg;
+  std::vector<BFS::VertexData> bfs_vd;
+  std::vector<int> sssp_vd;
+  std::vector<PPR::VertexData> ppr_vd;
+  GASEngineRef<BFS> bfs;
+  GASEngineRef<SSSP> sssp;
+  GASEngineRef<PPR> ppr;
 };
 
 // SSSP whose scatter writes its edge: hits counts the scatters of the
@@ -234,6 +277,33 @@ int64_t ref_pagerank(int nV, int nE, const int *srcs, const int *dsts,
   return t1 - t0;
 }
 
+// pagerank.cu:85-101 with pagerank.cu:126-132, the tolerance-driven run:
+// refgas.h:215-222's run() loop (gatherApply, scatterActivate, nextIter
+// while countActive) with its supersteps counted; returns the supersteps.
+int ref_pagerank_run(int nV, int nE, const int *srcs, const int *dsts,
+                     float *rank) {
+  std::vector<int> srcOffsets(nV + 1);
+  edgeListToCSR<int>(nV, nE, srcs, dsts, &srcOffsets[0], 0, 0);
+  std::vector<PageRank::VertexData> vd(nV);
+  for (int i = 0; i < nV; ++i) {
+    vd[i].numOutEdges = srcOffsets[i + 1] - srcOffsets[i];
+    vd[i].rank = PageRank::pageConst;
+  }
+  GASEngineRef<PageRank> engine;
+  engine.setGraph(nV, vd.data(), nE, 0, srcs, dsts);
+  engine.setActive(0, nV);
+  int steps = 0;
+  while (engine.countActive()) {
+    engine.gatherApply();
+    engine.scatterActivate();
+    engine.nextIter();
+    ++steps;
+  }
+  engine.getResults();
+  for (int i = 0; i < nV; ++i) rank[i] = vd[i].rank;
+  return steps;
+}
+
 // The reference engine's run() (refgas.h:215-222, scatter included) on
 // EdgeTrace, every vertex active first; returns the supersteps run.
 int ref_edge_trace(int nV, int nE, const int *srcs, const int *dsts,
@@ -260,6 +330,82 @@ int ref_edge_trace(int nV, int nE, const int *srcs, const int *dsts,
   }
   return steps;
 }
+
+// A prepared engine of `kind` (0 BFS, 1 SSSP, 2 personalised PageRank)
+// on the graph; weights are needed for SSSP only.
+void *ref_prepare(int kind, int nV, int nE, const int *srcs, const int *dsts,
+                  const int *weights) {
+  Prepared *p = new Prepared;
+  p->kind = kind;
+  p->nV = nV;
+  if (kind == 0) {
+    p->bfs_vd.assign(nV, BFS::VertexData{-1});
+    p->bfs.setGraph(nV, p->bfs_vd.data(), nE, 0, srcs, dsts);
+  } else if (kind == 1) {
+    p->weights.assign(weights, weights + nE);
+    p->sssp_vd.assign(nV, SSSP::gatherZero);
+    p->sssp.setGraph(nV, p->sssp_vd.data(), nE, p->weights.data(), srcs, dsts);
+  } else {
+    std::vector<int> off(nV + 1);
+    edgeListToCSR<int>(nV, nE, srcs, dsts, &off[0], 0, 0);
+    p->outdeg.resize(nV);
+    for (int i = 0; i < nV; ++i) p->outdeg[i] = off[i + 1] - off[i];
+    p->ppr_vd.resize(nV);
+    p->ppr.setGraph(nV, p->ppr_vd.data(), nE, 0, srcs, dsts);
+  }
+  return p;
+}
+
+// One query on a prepared engine: the driver's initialisation and loop
+// (bfs.cu:91-112; sssp.cu:67-82; `iterations` PPR sweeps); the result goes
+// to out (int32 or float per vertex).  Returns microseconds of the
+// reference's Took window (initialisation of the frontier through results).
+int64_t ref_prepared_query(void *h, int source, int iterations, void *out) {
+  Prepared *p = static_cast<Prepared *>(h);
+  const int nV = p->nV;
+  int64_t t0 = 0, t1 = 0;
+  if (p->kind == 0) {
+    for (auto &x : p->bfs_vd) x.depth = -1;
+    t0 = currentTime();
+    p->bfs.setActive(source, source + 1);
+    int iter = 0;
+    g_iterationCount = iter;
+    while (p->bfs.countActive()) {
+      p->bfs.gatherApply(false);
+      p->bfs.scatterActivate(false);
+      p->bfs.nextIter();
+      g_iterationCount = ++iter;
+    }
+    p->bfs.getResults();
+    t1 = currentTime();
+    int *o = static_cast<int *>(out);
+    for (int i = 0; i < nV; ++i) o[i] = p->bfs_vd[i].depth;
+  } else if (p->kind == 1) {
+    for (auto &x : p->sssp_vd) x = SSSP::gatherZero;
+    p->sssp_vd[source] = 0;
+    t0 = currentTime();
+    p->sssp.setActive(0, nV);
+    p->sssp.run();
+    p->sssp.getResults();
+    t1 = currentTime();
+    std::copy(p->sssp_vd.begin(), p->sssp_vd.end(), static_cast<int *>(out));
+  } else {
+    for (int i = 0; i < nV; ++i)
+      p->ppr_vd[i] = {i == source ? 0.15f : 0.0f, p->outdeg[i], i == source ? 1 : 0};
+    t0 = currentTime();
+    for (int it = 0; it < iterations; ++it) {
+      p->ppr.setActive(0, nV);
+      p->ppr.gatherApply();
+    }
+    p->ppr.getResults();
+    t1 = currentTime();
+    float *o = static_cast<float *>(out);
+    for (int i = 0; i < nV; ++i) o[i] = p->ppr_vd[i].rank;
+  }
+  return t1 - t0;
+}
+
+void ref_prepared_free(void *h) { delete static_cast<Prepared *>(h); }
 
 // util.h:79-110, exposed for layout checks.
 void ref_edge_list_to_csr(int nV, int nE, const int *srcs, const int *dsts,

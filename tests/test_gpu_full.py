@@ -1,8 +1,10 @@
 """Parity at the full configuration sizes: R-MAT-21 PageRank (20 sweeps) and
 symmetrised CC against the C oracle's FP64 power iteration and minimum
-reaching id; R-MAT-22 64-source batches through certificates that pin BFS
-depths and shortest-path distances exactly (size-independent properties);
-a served stream over road1024 + R-MAT-22 against the plain engine."""
+reaching id; R-MAT-22 64-source batches, every lane through the oracle's
+certificates that pin BFS depths and shortest-path distances exactly
+(size-independent properties); a served stream over road1024 + R-MAT-22,
+every result through the certificates or the FP64 personalised PageRank and
+against the same query answered alone."""
 import numpy as np
 import pytest
 
@@ -15,61 +17,6 @@ pytestmark = pytest.mark.gpu
 
 def rel_l1(x, ref):
     return float(np.abs(x.astype(np.float64) - ref).sum() / np.abs(ref).sum())
-
-
-def bfs_certificate(n, s, d, src, depth):
-    """True iff depth is the BFS depth from src: depth[src] = 0; along every
-    edge from a reached vertex the head is reached with depth <= tail + 1;
-    every reached vertex but src has an in-edge from depth - 1."""
-    if depth[src] != 0:
-        return False
-    ds, dt = depth[s], depth[d]
-    reached = ds >= 0
-    if np.any(dt[reached] < 0) or np.any(dt[reached] > ds[reached] + 1):
-        return False
-    tight = reached & (dt == ds + 1)
-    has_parent = np.zeros(n, bool)
-    has_parent[d[tight]] = True
-    need = depth >= 0
-    need[src] = False
-    return bool(np.all(has_parent[need]))
-
-
-def sssp_certificate(n, s, d, w, src, dist):
-    """True iff dist is the shortest-path distance from src (positive
-    weights): dist[src] = 0; no edge from a reached vertex can be relaxed;
-    every reached vertex but src has a tight in-edge."""
-    inf = oracle.SSSP_UNREACHED
-    if dist[src] != 0:
-        return False
-    ds = dist[s].astype(np.int64)
-    dt = dist[d].astype(np.int64)
-    reached = ds < inf
-    if np.any(dt[reached] > ds[reached] + w[reached]):
-        return False
-    tight = reached & (dt == ds + w)
-    has_parent = np.zeros(n, bool)
-    has_parent[d[tight]] = True
-    need = dist < inf
-    need[src] = False
-    return bool(np.all(has_parent[need]))
-
-
-def test_certificates_catch_mistakes():
-    # the certificates themselves: exact answers pass, perturbed ones fail
-    el = workloads.road(32, 32, seed=3)
-    s, d, w = el.arrays()
-    n = el.n_vertices
-    depth = oracle.check_bfs(n, s, d, 5)
-    dist = oracle.check_dijkstra(n, s, d, w, 5)
-    assert bfs_certificate(n, s, d, 5, depth)
-    assert sssp_certificate(n, s, d, w, 5, dist)
-    bad = depth.copy()
-    bad[np.argmax(depth)] += 1
-    assert not bfs_certificate(n, s, d, 5, bad)
-    bad = dist.copy()
-    bad[np.argmax(dist)] -= 1
-    assert not sssp_certificate(n, s, d, w, 5, bad)
 
 
 @pytest.fixture(scope="module")
@@ -118,7 +65,7 @@ def test_rmat22s_batch_full(gpu, rmat22s, kind):
     b.add(np.arange(64), src)
     b.run()
     res = b.results(range(64))
-    # every lane against the plain engine, two lanes by certificate
+    # every ninth lane against the plain engine, every lane by certificate
     buf = gas.DeviceBuffer(4 * n)
     for lane in range(0, 64, 9):
         if kind == gas.BATCH_BFS:
@@ -126,24 +73,38 @@ def test_rmat22s_batch_full(gpu, rmat22s, kind):
         else:
             g.sssp_device(int(src[lane]), buf.ptr)
         assert np.array_equal(res[lane], buf.to_numpy()), lane
-    for lane in (0, 63):
+    import concurrent.futures as cf
+
+    def cert(lane):
         if kind == gas.BATCH_BFS:
-            assert bfs_certificate(n, s, d, int(src[lane]), res[lane]), lane
-        else:
-            assert sssp_certificate(n, s, d, w, int(src[lane]), res[lane]), lane
+            return oracle.cert_bfs(n, s, d, int(src[lane]), res[lane])
+        return oracle.cert_sssp(n, s, d, w, int(src[lane]), res[lane])
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        ok = list(ex.map(cert, range(64)))
+    assert all(ok), [lane for lane in range(64) if not ok[lane]]
 
 
 def test_served_full_size(gpu, rmat22s):
-    el, g22 = rmat22s
-    road = gas.Graph(workloads.road())
+    # a served stream over both full-size graphs: every result checked by
+    # the oracle (BFS / SSSP certificates that fix the answer exactly, PPR
+    # within 1e-5 rel L1 of the FP64 power iteration) and against the same
+    # query answered alone by the plain engine
+    import concurrent.futures as cf
+    el22, g22 = rmat22s
+    el_road = workloads.road()
+    road = gas.Graph(el_road)
     graphs = [road, g22]
+    arrays = [el_road.arrays(), el22.arrays()]
+    nvs = [el_road.n_vertices, el22.n_vertices]
     srv = gas.Server(graphs, memory_budget=32 << 30)
     st = serve.make_stream(2000, 48, graphs, seed=77)
     keep = list(range(len(st["kinds"])))
     lat, results, stats = srv.replay(st["kinds"], st["graph_ids"], st["sources"],
                                      st["arrivals"], keep=keep)
     assert np.all(lat > 0)
-    for i, (k, gi, src) in enumerate(zip(st["kinds"], st["graph_ids"], st["sources"])):
+    queries = list(zip(st["kinds"], st["graph_ids"], st["sources"]))
+    for i, (k, gi, src) in enumerate(queries):
         gg = graphs[gi]
         if k == gas.BATCH_BFS:
             assert np.array_equal(results[i], gg.bfs(int(src))[0]), i
@@ -152,3 +113,21 @@ def test_served_full_size(gpu, rmat22s):
         else:
             plain, _ = gg.ppr(int(src), 10)
             assert rel_l1(results[i], plain.astype(np.float64)) <= 1e-5, i
+
+    def check(i):
+        k, gi, src = queries[i]
+        s, d, w = arrays[gi]
+        n = nvs[gi]
+        if k == gas.BATCH_BFS:
+            return oracle.cert_bfs(n, s, d, int(src), results[i])
+        if k == gas.BATCH_SSSP:
+            return oracle.cert_sssp(n, s, d, w, int(src), results[i])
+        ref = oracle.check_ppr_fp64(n, s, d, int(src), 10)
+        return rel_l1(results[i], ref) <= 1e-5
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        ok = list(ex.map(check, range(len(queries))))
+    bad = [(i, queries[i]) for i, good in enumerate(ok) if not good]
+    assert not bad, bad
+    kinds = {int(k) for k, _, _ in queries}
+    assert kinds == {gas.BATCH_BFS, gas.BATCH_SSSP, gas.BATCH_PPR}

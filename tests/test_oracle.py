@@ -90,6 +90,15 @@ def test_csr_matches_reference_layout():
         a, b = slice(off[v], off[v + 1]), slice(roff[v], roff[v + 1])
         assert sorted(dst[a]) == sorted(rdst[b])
         assert sorted(eid[a]) == sorted(reid[b])
+    # the oracle's order is the stable one: edge indices ascend within a key
+    for v in range(500):
+        assert np.all(np.diff(eid[off[v]:off[v + 1]]) > 0)
+    # CSC is CSR of the reversed edges (util.h:114-121)
+    coff, csrc, ceid = oracle.edge_list_to_csc(500, s, d)
+    roff2, rsrc, reid2 = oracle.edge_list_to_csr(500, d, s)
+    assert np.array_equal(coff, roff2) and np.array_equal(csrc, rsrc)
+    assert np.array_equal(ceid, reid2)
+    assert np.array_equal(coff, np.concatenate([[0], np.cumsum(np.bincount(d, minlength=500))]))
 
 
 @needs_ref
@@ -215,6 +224,143 @@ def test_checks_catch_planted_mistakes():
     s2, d2, _ = random_graph(rng, 200, 600)
     assert not np.array_equal(oracle.bfs(200, s2, d2, 0)[0],
                               oracle.check_bfs(200, d2, s2, 0))
-    # an off-by-one in the PageRank damping is caught by the FP64 check
-    r64 = oracle.check_pagerank_fp64(200, s2, d2, 5)
-    assert not np.allclose(r64 * 1.0001, r64, rtol=1e-6)
+
+
+def _power_iteration_np(n, s, d, iters, damping=0.85, outdeg_of=None, sweeps_less=0):
+    """A numpy power iteration of pagerank.cu:50, 62 with knobs for planted
+    mistakes: damping, whose degree divides a contribution, sweeps run."""
+    outdeg = np.bincount(s if outdeg_of is None else outdeg_of, minlength=n)
+    r = np.full(n, 0.15)
+    for _ in range(iters - sweeps_less):
+        contrib = r[s] / np.maximum(outdeg[s], 1)
+        r = 0.15 + damping * np.bincount(d, weights=contrib, minlength=n)
+    return r
+
+
+def test_pagerank_checks_catch_planted_mistakes():
+    # the FP64 checker and the 1e-5 rel-L1 rule of the GPU tests: a numpy
+    # power iteration matches the checker to rounding, and each planted
+    # mistake in it (damping 0.85 -> 0.851, one sweep short, the
+    # destination's degree, reversed edges) lands outside the rule
+    rng = np.random.default_rng(5)
+    n = 300
+    s, d, _ = random_graph(rng, n, 2400)
+    r64 = oracle.check_pagerank_fp64(n, s, d, 20)
+
+    def rel(x):
+        return float(np.abs(x - r64).sum() / np.abs(r64).sum())
+
+    assert rel(_power_iteration_np(n, s, d, 20)) <= 1e-13
+    assert rel(_power_iteration_np(n, s, d, 20, damping=0.851)) > 1e-5
+    assert rel(_power_iteration_np(n, s, d, 20, sweeps_less=1)) > 1e-5
+    assert rel(_power_iteration_np(n, s, d, 20, outdeg_of=d)) > 1e-5
+    assert rel(_power_iteration_np(n, d, s, 20)) > 1e-5
+    # the FP32 restatement of the engine passes the same rule
+    assert rel(oracle.pagerank(n, s, d, 20)[0].astype(np.float64)) <= 1e-5
+
+
+def test_certificates_catch_mistakes():
+    # the one-pass BFS / SSSP certificates: exact answers pass; a depth or
+    # distance moved by one at any reached vertex, an unreached vertex
+    # marked reached, a reached one marked unreached and a wrong source fail
+    for el in (workloads.road(32, 32, seed=3),
+               workloads.rmat(10, symmetrize=True, weights=True)):
+        s, d, w = el.arrays()
+        n = el.n_vertices
+        src = int(el.max_outdegree_vertex()[0])
+        depth = oracle.check_bfs(n, s, d, src)
+        dist = oracle.check_dijkstra(n, s, d, w, src)
+        assert oracle.cert_bfs(n, s, d, src, depth)
+        assert oracle.cert_sssp(n, s, d, w, src, dist)
+        assert not oracle.cert_bfs(n, s, d, (src + 1) % n, depth)
+        assert not oracle.cert_sssp(n, s, d, w, (src + 1) % n, dist)
+        rng = np.random.default_rng(n)
+        reached = np.nonzero((depth > 0))[0]
+        for v in rng.choice(reached, min(40, len(reached)), replace=False):
+            for delta in (-1, 1):
+                bad = depth.copy()
+                bad[v] += delta
+                assert not oracle.cert_bfs(n, s, d, src, bad), (v, delta)
+                bad = dist.copy()
+                bad[v] += delta
+                assert not oracle.cert_sssp(n, s, d, w, src, bad), (v, delta)
+            bad = depth.copy()
+            bad[v] = -1
+            assert not oracle.cert_bfs(n, s, d, src, bad)
+            bad = dist.copy()
+            bad[v] = oracle.SSSP_UNREACHED
+            assert not oracle.cert_sssp(n, s, d, w, src, bad)
+        unreached = np.nonzero(depth < 0)[0]
+        if len(unreached):
+            bad = depth.copy()
+            bad[unreached[0]] = 1
+            assert not oracle.cert_bfs(n, s, d, src, bad)
+            bad = dist.copy()
+            bad[unreached[0]] = 1
+            assert not oracle.cert_sssp(n, s, d, w, src, bad)
+    # a graph with unreached vertices, the certificates against the textbook
+    rng = np.random.default_rng(9)
+    s, d, w = random_graph(rng, 500, 700)
+    for src in (0, 7, 499):
+        assert oracle.cert_bfs(500, s, d, src, oracle.check_bfs(500, s, d, src))
+        assert oracle.cert_sssp(500, s, d, w, src, oracle.check_dijkstra(500, s, d, w, src))
+
+
+@needs_ref
+def test_reference_tolerance_pagerank_supersteps():
+    # the restated engine's tolerance-driven run takes the reference's
+    # supersteps (compiled refgas.h run(), pagerank.cu:85-101)
+    for sc in (8, 10, 12):
+        el = workloads.rmat(sc)
+        s, d, _ = el.arrays()
+        n = el.n_vertices
+        mine, st = oracle.pagerank(n, s, d, 0)
+        ref, steps = oracle.ref_pagerank_run(n, s, d)
+        assert st.supersteps == steps
+        assert np.array_equal(ref, oracle.ref_pagerank(n, s, d, 0)[0])
+
+
+@needs_ref
+def test_pagerank_tol_fp64_matches_reference():
+    # the FP64 evaluation of the tolerance-driven run takes the reference
+    # engine's supersteps and agrees with its FP32 ranks to FP32 rounding;
+    # on a directed n-cycle sweep k moves every rank by 0.15 * 0.85^k, so the
+    # run stops after the first sweep whose step is below tol = 0.01:
+    # 0.15 * 0.85^k < 0.01 first at k = 17, i.e. 17 supersteps, leaving
+    # r = 0.15 * sum_{j<=17} 0.85^j
+    rng = np.random.default_rng(23)
+    for trial in range(4):
+        n = int(rng.integers(2, 800))
+        s, d, _ = random_graph(rng, n, int(rng.integers(n, 5 * n)))
+        ref, steps = oracle.ref_pagerank_run(n, s, d)
+        r64, st64 = oracle.check_pagerank_tol_fp64(n, s, d)
+        assert st64 == steps
+        assert np.abs(ref - r64).sum() / r64.sum() <= 1e-6
+    n = 9
+    s = np.arange(n, dtype=np.int32)
+    r64, st64 = oracle.check_pagerank_tol_fp64(n, s, (s + 1) % n)
+    assert st64 == 17
+    assert np.allclose(r64, 0.15 * sum(0.85 ** j for j in range(18)), rtol=1e-14)
+    assert oracle.ref_pagerank_run(n, s, (s + 1) % n)[1] == 17
+
+
+def test_ref_prepared_engines():
+    # one reference engine bound once, many queries: each equals the
+    # textbook answer (BFS, Dijkstra) or the FP64 personalised PageRank
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    el = workloads.road(48, 48, seed=3)
+    s, d, w = el.arrays()
+    n = el.n_vertices
+    for kind in (0, 1, 2):
+        eng = oracle.RefPrepared(kind, n, s, d, w)
+        for src in (0, 77, n - 1):
+            out, us = eng.query(src, 10)
+            assert us >= 0
+            if kind == 0:
+                assert np.array_equal(out, oracle.check_bfs(n, s, d, src))
+            elif kind == 1:
+                assert np.array_equal(out, oracle.check_dijkstra(n, s, d, w, src))
+            else:
+                ref = oracle.check_ppr_fp64(n, s, d, src, 10)
+                assert np.abs(out - ref).sum() / np.abs(ref).sum() <= 1e-6
