@@ -78,12 +78,12 @@ gas_graph *ordered_graph(gas_graph *g);
 // Device-side build of one layout (graph_build.hip): offsets (V+1), values
 // and weights (if d_w) in key order, the edge-list index of each slot;
 // d_sorted_keys (E) and d_temp (device_build_temp_bytes(E)) are scratch.
-// Enqueued on the null stream.
+// Enqueued on `stream`.
 int device_build_layout(int32_t nV, int64_t nE, const int32_t *d_keys,
                         const int32_t *d_vals, const int32_t *d_w,
                         int32_t *d_off, int32_t *d_out_vals, int32_t *d_eid,
                         int32_t *d_out_w, uint32_t *d_sorted_keys, void *d_temp,
-                        size_t temp_bytes);
+                        size_t temp_bytes, gasrt_stream_t stream);
 size_t device_build_temp_bytes(int64_t nE);
 
 // Stable counting sort of edges by key: offsets (V+1), values permuted into

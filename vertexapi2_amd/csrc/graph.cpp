@@ -546,7 +546,8 @@ static int device_create(gas_edges_t edges, gas_graph *g) {
   uint32_t *d_keys = nullptr;
   void *d_temp = nullptr;
   const size_t temp_bytes = gas::device_build_temp_bytes(E);
-  int rc = 0;
+  gasrt_stream_t stream = nullptr;
+  int rc = gasrt_stream_create(&stream);
   auto dmalloc = [&](void **p, size_t bytes) {
     if (!rc) rc = gasrt_malloc(p, bytes);
   };
@@ -565,33 +566,38 @@ static int device_create(gas_edges_t edges, gas_graph *g) {
   dmalloc((void **)&d_eid, eb);
   dmalloc((void **)&d_keys, eb);
   dmalloc(&d_temp, std::max<size_t>(temp_bytes, 4));
-  if (!rc && E) rc = gasrt_memcpy_h2d(d_src, edges->src.data(), (size_t)E * 4, nullptr);
-  if (!rc && E) rc = gasrt_memcpy_h2d(d_dst, edges->dst.data(), (size_t)E * 4, nullptr);
-  if (!rc && E && wt) rc = gasrt_memcpy_h2d(d_w, edges->w.data(), (size_t)E * 4, nullptr);
+  if (!rc && E) rc = gasrt_memcpy_h2d(d_src, edges->src.data(), (size_t)E * 4, stream);
+  if (!rc && E) rc = gasrt_memcpy_h2d(d_dst, edges->dst.data(), (size_t)E * 4, stream);
+  if (!rc && E && wt) rc = gasrt_memcpy_h2d(d_w, edges->w.data(), (size_t)E * 4, stream);
   auto layout = [&](const int32_t *keys, const int32_t *vals, int32_t *off,
                     int32_t *out_vals, int32_t *out_w, std::vector<int32_t> &h_off,
                     std::vector<int32_t> &h_vals, std::vector<int32_t> &h_eid) {
     if (rc) return;
     rc = gas::device_build_layout(V, E, keys, vals, d_w, off, out_vals, d_eid, out_w,
-                                  d_keys, d_temp, temp_bytes);
+                                  d_keys, d_temp, temp_bytes, stream);
     h_off.resize((size_t)V + 1);
     h_vals.resize((size_t)E);
     h_eid.resize((size_t)E);
-    if (!rc) rc = gasrt_memcpy_d2h(h_off.data(), off, vb, nullptr);
-    if (!rc && E) rc = gasrt_memcpy_d2h(h_vals.data(), out_vals, (size_t)E * 4, nullptr);
-    if (!rc && E) rc = gasrt_memcpy_d2h(h_eid.data(), d_eid, (size_t)E * 4, nullptr);
+    if (!rc) rc = gasrt_memcpy_d2h(h_off.data(), off, vb, stream);
+    if (!rc && E) rc = gasrt_memcpy_d2h(h_vals.data(), out_vals, (size_t)E * 4, stream);
+    if (!rc && E) rc = gasrt_memcpy_d2h(h_eid.data(), d_eid, (size_t)E * 4, stream);
   };
   layout(d_dst, d_src, g->csc_off, g->csc_src, g->csc_w, g->h_csc_off, g->h_csc_src,
          g->h_csc_eid);
   layout(d_src, d_dst, g->csr_off, g->csr_dst, g->csr_w, g->h_csr_off, g->h_csr_dst,
          g->h_csr_eid);
-  if (!rc) rc = gasrt_stream_synchronize(nullptr);
+  if (stream) {
+    // the stream drains before its buffers are freed, also after a failure
+    const int rs = gasrt_stream_synchronize(stream);
+    if (!rc) rc = rs;
+  }
   gasrt_free(d_src);
   gasrt_free(d_dst);
   gasrt_free(d_w);
   gasrt_free(d_eid);
   gasrt_free(d_keys);
   gasrt_free(d_temp);
+  if (stream) gasrt_stream_destroy(stream);
   if (!rc && wt)
     for (int32_t x : edges->w) g->max_w = std::max(g->max_w, x);
   return rc;
