@@ -7,38 +7,55 @@ graph (3% of links dropped, 1% long links, int32 weights U[1,64]).  One step
 = one BFS run plus one SSSP run of the plain engine to completion on the
 graph already resident in HBM: initial vertex state written on the device,
 supersteps until the frontier is empty, result copied into a device buffer
-(gas_bfs_device / gas_sssp_device; the reference's "Took" window is the run
-alone, sssp.cu:77-81, bfs.cu:100-111).  Degree tiers are autotuned once per
+(gas_bfs_device / gas_sssp_device).  Degree tiers are autotuned once per
 program before the warmup.
 
 value = traversed edges per second in GTEPS: (E_r(BFS) + E_r(SSSP)) / time,
 with E_r the summed out-degree of the vertices the run reaches, and time the
 host wall clock around the K timed steps (each step ends synchronised).
+results_on_host: the same K steps through gas_bfs / gas_sssp, the
+reference's "Took" window (frontier initialisation through results in host
+memory, sssp.cu:77-81, bfs.cu:100-111): host-side initial state, upload,
+run, download.
 
 roofline: after the timed steps, the same K steps run again with every
-engine kernel launch bracketed by HIP events on the engine's stream (the
-event packets roughly double a road step, so that pass is not the value);
-the dominant kernel (largest summed time) is reported with its algorithmic
-bytes per launch (model below) over its average launch duration.  traffic: HBM bytes per launch of that kernel
-from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept under profiles/
-(pmc_<workload>.json), or null when none matches.
+engine kernel launch bracketed by HIP events on the engine's stream; the
+dominant kernel (largest summed time) is reported with its algorithmic bytes
+per launch (model below, unit counts from the engine's device statistics)
+over its average event-bracketed duration (frac).  Event brackets include
+dispatch latency, so the entry also gives the same bytes over the kernel's
+average duration in the rocprofv3 kernel-trace summary of this workload
+(profiles/kstats_<workload>.json: frac_rocprof).  traffic: HBM bytes per
+launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+(profiles/pmc_<workload>.json), or null.
+roofline_8d: the whole step against the compulsory bytes of the scope
+table: BFS 20 V_r + 8 E_r, SSSP 20 V_r + 12 E_r (V_r reached vertices, each
+datum once, re-relaxations not credited), PageRank (8 E + 16 V) per sweep,
+CC sum over supersteps of 16 A + 8 G + 4 S + 8 R.
 cpu_baseline: the reference's own CPU engine (refgas.h built from its
-sources into oracle/_ref) on the same workload, one host core.
+sources into oracle/_ref) on the same workload, one host core, with the CPU
+model.  The batched and served lines add an all-core figure: one prepared
+reference engine per core of the GPU's CPU share (16 threads), all running
+at once, each answering queries of one (kind, graph) for a bounded time.
 
 Further lines under "apps" (unless --no-extra): grid BFS (64x64), R-MAT-21
-PageRank (20 sweeps) and CC (symmetrised), each with its dominant-kernel
-roofline; "batched": 64-source BFS and SSSP batches on symmetrised weighted
-R-MAT-22 against the plain engine answering the same sources one at a time;
-"served": Poisson query streams (50/30/20 BFS/SSSP/PPR over road1024 and
-R-MAT-22) replayed through the continuous-batching server, with QPS, p50
-and p99 against a FIFO of the plain engine's measured service times.
+PageRank (20 sweeps, with its error against an FP64 power iteration
+computed here with numpy) and CC (symmetrised), each with its dominant
+kernel's roofline and roofline_8d; "batched": 64-source BFS and SSSP batches
+on symmetrised weighted R-MAT-22 against the plain engine answering the same
+sources one at a time; "served": Poisson query streams (50/30/20 BFS/SSSP/PPR
+over road1024 and R-MAT-22) at 100 ... 20,000 queries/s replayed through the
+continuous-batching server and through the plain engine answering the same
+stream itself one query at a time, with QPS, p50 and p99.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -51,6 +68,7 @@ from vertexapi2_amd import serve, workloads  # noqa: E402
 
 METRIC = "GTEPS + % HBM roofline per app; batched-served QPS & p99 vs plain 1-source"
 HBM_PEAK_GBS = 8000.0
+SERVED_RATES = (100, 200, 500, 1000, 2000, 5000, 10000, 20000)
 
 # Bytes of one vertex's state, one edge's value and one gather result.
 VD = {"bfs": 4, "sssp": 4, "cc": 4, "pagerank": 8, "ppr": 12}
@@ -66,20 +84,54 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def reached_edges(values, csr_off, unreached):
+def reached(values, csr_off, unreached):
+    """(V_r, E_r): reached vertices and their summed out-degree."""
     deg = np.diff(csr_off).astype(np.int64)
-    return int(deg[values != unreached].sum())
+    m = values != unreached
+    return int(m.sum()), int(deg[m].sum())
+
+
+def bytes_8d(app, vr, er):
+    """Compulsory bytes of one BFS / SSSP run (scope-table model)."""
+    return 20 * vr + (12 if app == "sssp" else 8) * er
+
+
+def frac_entry(byts, seconds):
+    gbs = byts / seconds / 1e9
+    return {"bytes": int(byts), "seconds": round(seconds, 6), "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_share():
+    """Worker processes for all-core figures: the CPU share of one GPU on
+    the box (16 threads), fewer if fewer are available."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 # ---------------------------------------------------------------------------
 # Algorithmic bytes per kernel (each datum counted once, lower bounds).
 #   k_gather_vp   per gathered vertex: 8 (CSC offset pair) + 4 (result);
 #                 per in-edge: 4 (source id) + edge value + source read (SRC)
+#   k_hgather     the same for the hub vertices and their in-edges (the
+#                 engine counts both: hub_vertices_total, hub_edges_total)
 #   k_apply       per active vertex: 4 (slot id) + 4 (gather result, when
 #                 there is a gather) + state; per scattered out-edge: 4
 #                 (destination id) + 1 (activation byte)
-#   k_hgather     per hub in-edge: 4 + edge value + source state; per hub:
-#                 8 + 4
 #   k_compact1    V (activation bytes); k_compact2: V + 4 per listed id
 #   k_fused       per active vertex: 8 (offsets) + state read + state write;
 #                 per in-edge as k_gather_vp; per scattered edge 5
@@ -87,14 +139,11 @@ def reached_edges(values, csr_off, unreached):
 #                 vertex: 4 (list id) + 8 (CSR offsets) + state read + write,
 #                 with a gather + 8 (CSC offsets); per in-edge as
 #                 k_gather_vp; per scattered edge 5
-def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
-    """{kernel: total algorithmic bytes over the run}.  hub_split =
-    (hub_vertices, hub_edges) per sweep for all-active runs, splitting the
-    gather between k_gather_vp and k_hgather; without it every gathered
-    edge is booked to k_gather_vp (graphs whose in-degrees stay under the
-    hub tier)."""
+def kernel_models(app, st, n_vertices, have_gather):
+    """{kernel: total algorithmic bytes over the run}."""
     vd, ed, sv = VD[app], ED.get(app, 0), SRC[app]
     A, Eg, Es = st.active_total, st.gathered_edges, st.scattered_edges
+    Hv, He = st.hub_vertices_total, st.hub_edges_total
     out = {}
     ks = st.kernels()
     if set(ks) <= {"k_small"}:
@@ -106,14 +155,8 @@ def kernel_models(app, st, n_vertices, have_gather, hub_split=None):
         out["k_fused"] = A * (8 + 2 * vd) + Eg * (4 + ed + sv) + Es * 5
         return out
     if have_gather:
-        if hub_split is not None:
-            sweeps = st.supersteps
-            hv, he = hub_split
-            out["k_gather_vp"] = ((A - sweeps * hv) * (8 + GR) +
-                                  (Eg - sweeps * he) * (4 + ed + sv))
-            out["k_hgather"] = sweeps * (hv * (8 + GR) + he * (4 + ed + sv))
-        else:
-            out["k_gather_vp"] = A * (8 + GR) + Eg * (4 + ed + sv)
+        out["k_gather_vp"] = (A - Hv) * (8 + GR) + (Eg - He) * (4 + ed + sv)
+        out["k_hgather"] = Hv * (8 + GR) + He * (4 + ed + sv)
     out["k_apply"] = A * (4 + (GR if have_gather else 0) + vd) + Es * 5
     launches = st.kernels().get("k_compact1", (0, 0))[1]
     out["k_compact1"] = launches * n_vertices
@@ -148,21 +191,63 @@ def roofline_table(stats_list, models_list):
     return dom, table
 
 
+def _profile(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def roofline_entry(dom, table, workload):
     e = table[dom]
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"pmc_{workload}.json")
-    if os.path.exists(pmc):
-        with open(pmc) as f:
-            rec = json.load(f)
+    rec = _profile(f"pmc_{workload}.json")
+    if rec is not None:
         k = rec.get("kernels", {}).get(dom)
         if k is not None:
             traffic = k.get("hbm_bytes_per_launch")
     achieved = e.get("achieved_gbs")
-    return {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": e.get("frac"), "traffic": traffic,
-            "avg_launch_us": e["avg_launch_us"], "launches": e["launches"],
-            "algorithmic_bytes_per_launch": e.get("algorithmic_bytes_per_launch")}
+    out = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": e.get("frac"), "traffic": traffic,
+           "avg_launch_us": e["avg_launch_us"], "launches": e["launches"],
+           "algorithmic_bytes_per_launch": e.get("algorithmic_bytes_per_launch")}
+    ks = _profile(f"kstats_{workload}.json")
+    b = e.get("algorithmic_bytes_per_launch")
+    if ks is not None and b and dom in ks.get("kernels", {}):
+        us = ks["kernels"][dom]["avg_us"]
+        gbs = b / (us * 1e-6) / 1e9
+        out.update({"rocprof_avg_us": us, "achieved_rocprof": round(gbs, 2),
+                    "frac_rocprof": round(gbs / HBM_PEAK_GBS, 5)})
+    return out
+
+
+def fp64_pagerank(n, s, d, iterations, source=-1):
+    """FP64 power iteration of rank = tele + 0.85 sum rank/outdeg over
+    in-edges (pagerank.cu:50, 62), tele = 0.15 everywhere (source < 0) or at
+    the source only (personalised), written here with numpy."""
+    s = np.asarray(s, np.int64)
+    d = np.asarray(d, np.int64)
+    outdeg = np.bincount(s, minlength=n).astype(np.float64)
+    tele = np.full(n, 0.15) if source < 0 else np.zeros(n)
+    if source >= 0:
+        tele[source] = 0.15
+    r = tele.copy()
+    inv = np.zeros(n)
+    nz = outdeg > 0
+    inv[nz] = 1.0 / outdeg[nz]
+    for _ in range(iterations):
+        r = tele + 0.85 * np.bincount(d, weights=(r * inv)[s], minlength=n)
+    return r
+
+
+def fp_errors(x, ref):
+    x = np.asarray(x, np.float64)
+    diff = np.abs(x - ref)
+    nz = ref != 0
+    return {"rel_l1": float(diff.sum() / np.abs(ref).sum()),
+            "max_rel": float((diff[nz] / np.abs(ref[nz])).max(initial=0.0)),
+            "max_abs": float(diff.max(initial=0.0))}
 
 
 # ---------------------------------------------------------------------------
@@ -179,6 +264,14 @@ def headline(g, steps, warmup):
         sb0 = g.bfs_device(0, dbuf.ptr)
         ss0 = g.sssp_device(0, sbuf.ptr)
     total = time.perf_counter() - t0
+    # results on the host (the reference's Took window)
+    g.bfs(0)
+    g.sssp(0)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        g.bfs(0)
+        g.sssp(0)
+    total_host = time.perf_counter() - t1
     # the same steps again with every engine launch bracketed by HIP events
     # (the event packets slow the step down, so this pass is not the value)
     gas.set_kernel_timing(True)
@@ -194,16 +287,16 @@ def headline(g, steps, warmup):
         gas.set_kernel_timing(False)
     depth, dist = dbuf.to_numpy(), sbuf.to_numpy()
     csr_off = g.host_csr()[0]
-    er = (reached_edges(depth, csr_off, -1) +
-          reached_edges(dist, csr_off, gas.SSSP_UNREACHED))
+    vb, eb = reached(depth, csr_off, -1)
+    vs, es = reached(dist, csr_off, gas.SSSP_UNREACHED)
     stats, models = [], []
     for sb, ss in sts:
         stats += [sb, ss]
         models += [kernel_models("bfs", sb, V, False), kernel_models("sssp", ss, V, True)]
     dom, table = roofline_table(stats, models)
-    sb, ss = sb0, ss0
-    return {"total_s": total, "total_timed_s": total_timed, "er": er, "tiers": tiers,
-            "dom": dom, "table": table, "bfs": sb, "sssp": ss}
+    return {"total_s": total, "total_host_s": total_host, "total_timed_s": total_timed,
+            "er": eb + es, "bytes_8d": bytes_8d("bfs", vb, eb) + bytes_8d("sssp", vs, es),
+            "tiers": tiers, "dom": dom, "table": table, "bfs": sb0, "sssp": ss0}
 
 
 def cpu_baseline(el, budget_s=12.0):
@@ -221,15 +314,78 @@ def cpu_baseline(el, budget_s=12.0):
         depth, us1 = oracle.ref_bfs(n, s, d, 0)
         dist, us2 = oracle.ref_sssp(n, s, d, w, 0)
         us_total += us1 + us2
-        er_total += reached_edges(depth, off, -1) + reached_edges(dist, off, gas.SSSP_UNREACHED)
+        er_total += reached(depth, off, -1)[1] + reached(dist, off, gas.SSSP_UNREACHED)[1]
         reps += 1
         if time.perf_counter() - t0 > budget_s or reps >= 20:
             break
     gteps = er_total / (us_total * 1e-6) / 1e9
     return {"value": round(gteps, 5), "unit": "GTEPS", "cores": 1, "kind": "reference",
+            "cpu_model": cpu_model(),
             "sample": f"{reps} x (BFS + SSSP from vertex 0) on the full 1024x1024 road graph "
                       f"with refgas.h (GASEngineRef) single-threaded; {us_total / reps / 1e3:.1f} "
                       f"ms per step in the reference's Took window"}
+
+
+def cpu_all_cores(graphs, budget_s=15.0, seed=5):
+    """Per (graph, kind) mean service time of the reference engine with every
+    core of the CPU share busy at once: one worker process per core, worker
+    i answering queries of combination i mod 6 (BFS, SSSP, PPR on each of the
+    two graphs) from uniform sources with out-edges, for budget_s seconds
+    after binding its engine to the graph.  graphs: [(name, Edges)]."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    nw = cpu_share()
+    tmp = tempfile.mkdtemp(prefix="gas_cpu_")
+    combos, files = [], []
+    try:
+        for gi, (name, el) in enumerate(graphs):
+            s, d, w = el.arrays()
+            pre = os.path.join(tmp, name)
+            np.save(pre + "_s.npy", s)
+            np.save(pre + "_d.npy", d)
+            np.save(pre + "_w.npy", w)
+            files += [pre + "_s.npy", pre + "_d.npy", pre + "_w.npy"]
+            deg = np.bincount(s, minlength=el.n_vertices)
+            cand = np.nonzero(deg > 0)[0]
+            for kind in (0, 1, 2):
+                combos.append((name, kind, pre, el.n_vertices, cand))
+        rng = np.random.default_rng(seed)
+        procs = []
+        env = dict(os.environ, PYTHONPATH=ROOT, OMP_NUM_THREADS="1")
+        for i in range(nw):
+            name, kind, pre, n, cand = combos[i % len(combos)]
+            spec = {"prefix": pre, "n_vertices": int(n), "kind": kind, "budget_s": budget_s,
+                    "iterations": 10,
+                    "sources": rng.choice(cand, 256).astype(int).tolist()}
+            path = os.path.join(tmp, f"spec{i}.json")
+            with open(path, "w") as f:
+                json.dump(spec, f)
+            files.append(path)
+            procs.append((i % len(combos), subprocess.Popen(
+                [sys.executable, "-m", "oracle.cpu_worker", path], cwd=ROOT, env=env,
+                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        agg = {}
+        for ci, p in procs:
+            out, err = p.communicate(timeout=900)
+            if p.returncode != 0:
+                raise RuntimeError(f"cpu worker failed: {err[-2000:]}")
+            r = json.loads(out.strip().splitlines()[-1])
+            a = agg.setdefault(ci, {"queries": 0, "took_us": 0, "instances": 0})
+            a["queries"] += r["queries"]
+            a["took_us"] += r["took_us"]
+            a["instances"] += 1
+    finally:
+        for f in files:
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(tmp)
+    out = {}
+    for ci, a in agg.items():
+        name, kind = combos[ci][0], combos[ci][1]
+        out[(name, kind)] = {"mean_ms": a["took_us"] / a["queries"] / 1e3,
+                             "queries": a["queries"], "instances": a["instances"]}
+    return {"workers": nw, "cpu_model": cpu_model(), "combos": out, "budget_s": budget_s}
 
 
 def timed_runs(fn, reps):
@@ -254,15 +410,17 @@ def app_lines(steps):
     gas.set_kernel_timing(True)
     st = g.bfs_device(0, buf.ptr)
     gas.set_kernel_timing(False)
-    er = reached_edges(buf.to_numpy(), g.host_csr()[0], -1)
+    vr, er = reached(buf.to_numpy(), g.host_csr()[0], -1)
     dom, table = roofline_table([st], [kernel_models("bfs", st, V, False)])
     out["grid64_bfs"] = {"gteps": round(er / t / 1e9, 5), "ms": round(t * 1e3, 4),
                          "device_ms": round(st.ms, 4), "supersteps": st.supersteps,
-                         "roofline": roofline_entry(dom, table, "grid64_bfs")}
+                         "roofline": roofline_entry(dom, table, "grid64_bfs"),
+                         "roofline_8d": frac_entry(bytes_8d("bfs", vr, er), t)}
     del g, buf
     log("grid64 done")
     # R-MAT-21 PageRank, 20 sweeps
-    g = gas.Graph(workloads.rmat(21))
+    el = workloads.rmat(21)
+    g = gas.Graph(el)
     V, E = g.n_vertices, g.n_edges
     tiers = g.autotune(gas.PROGRAM_PAGERANK)
     _, res, _ = g.pagerank(20)  # host-result run: the per-sweep residuals
@@ -272,17 +430,17 @@ def app_lines(steps):
     gas.set_kernel_timing(True)
     st = g.pagerank_device(rbuf.ptr, 20)
     gas.set_kernel_timing(False)
-    coff = g.host_csc()[0]
-    indeg = np.diff(coff)
-    hub = indeg > tiers[1]
-    split = (int(hub.sum()), int(indeg[hub].sum()))
-    dom, table = roofline_table([st], [kernel_models("pagerank", st, V, True, split)])
+    rank = rbuf.to_numpy(np.float32)
+    dom, table = roofline_table([st], [kernel_models("pagerank", st, V, True)])
+    s, d, _ = el.arrays()
+    err = fp_errors(rank, fp64_pagerank(V, s, d, 20))
     out["rmat21_pagerank20"] = {
         "gteps": round(20 * E / t / 1e9, 4), "ms": round(t * 1e3, 3),
         "device_ms": round(st0.ms, 3), "V": V, "E": E, "tiers": tiers,
-        "final_residual_l1": float(res[-1]),
-        "roofline": roofline_entry(dom, table, "rmat21_pagerank20"), "kernels": table}
-    del g
+        "final_residual_l1": float(res[-1]), "l1_vs_fp64": err,
+        "roofline": roofline_entry(dom, table, "rmat21_pagerank20"),
+        "roofline_8d": frac_entry(20 * (8 * E + 16 * V), t), "kernels": table}
+    del g, el, s, d
     log("pagerank done")
     # R-MAT-21 symmetrised, CC
     g = gas.Graph(workloads.rmat(21, symmetrize=True))
@@ -293,10 +451,18 @@ def app_lines(steps):
     g.cc_device(lbuf.ptr)
     t, st0 = timed_runs(lambda: g.cc_device(lbuf.ptr), reps)
     assert np.array_equal(lbuf.to_numpy(), lab)
+    gas.set_kernel_timing(True)
+    st = g.cc_device(lbuf.ptr)
+    gas.set_kernel_timing(False)
+    dom, table = roofline_table([st], [kernel_models("cc", st, V, True)])
+    b8 = (16 * st0.active_total + 8 * st0.gathered_edges + 4 * st0.scattered_edges +
+          8 * st0.returned_total)
     out["rmat21s_cc"] = {"gteps": round(st0.gathered_edges / t / 1e9, 4),
                          "ms": round(t * 1e3, 3), "device_ms": round(st0.ms, 3),
                          "supersteps": st0.supersteps, "E": g.n_edges, "tiers": tiers,
-                         "components": int(len(np.unique(lab)))}
+                         "components": int(len(np.unique(lab))),
+                         "roofline": roofline_entry(dom, table, "rmat21s_cc"),
+                         "roofline_8d": frac_entry(b8, t), "kernels": table}
     del g
     log("cc done")
     return out
@@ -329,8 +495,23 @@ def batched_lines(g22, n_plain=8):
                 b.release((1 << 64) - 1)
         t_b = float(np.median(ts))
         er = 0
+        any_lane = np.zeros(V, bool)
         for lane, res in b.results(range(64)).items():
-            er += reached_edges(res, csr_off, unreached)
+            m = res != unreached
+            any_lane |= m
+            er += int(deg[m].astype(np.int64).sum())
+        # compulsory bytes of the batch, each datum once: per vertex some
+        # lane reaches, its offsets and its lane row (BFS: 16-bit depths of
+        # 64 lanes written + visited / frontier masks; SSSP: 64 int32
+        # distances read and written); per out-edge of those, its
+        # destination id (+ weight) and the destination's masks (BFS) or
+        # row (SSSP) once
+        va = int(any_lane.sum())
+        ea = int(deg[any_lane].astype(np.int64).sum())
+        if kind == gas.BATCH_BFS:
+            b8 = va * (8 + 16 + 2 * 64) + ea * (4 + 16)
+        else:
+            b8 = va * (8 + 2 * 4 * 64) + ea * (8 + 4 * 64)
         b.release((1 << 64) - 1)
         del b
         fn = g22.bfs_device if kind == gas.BATCH_BFS else g22.sssp_device
@@ -341,47 +522,69 @@ def batched_lines(g22, n_plain=8):
             fn(int(s), buf.ptr)
             tp.append(time.perf_counter() - t)
         t_p = float(np.mean(tp))
-        out[f"rmat22s_{name}64"] = {
+        line = {
             "batch_ms": round(t_b * 1e3, 3), "batch_qps": round(64 / t_b, 1),
             "batch_gteps": round(er / t_b / 1e9, 4), "rounds": st.supersteps,
             "plain_ms_per_query": round(t_p * 1e3, 3), "plain_qps": round(1 / t_p, 1),
-            "qps_vs_plain": round(t_p * 64 / t_b, 2)}
+            "qps_vs_plain": round(t_p * 64 / t_b, 2),
+            "roofline_8d": frac_entry(b8, t_b)}
+        ks = _profile("kstats_rmat22_batched.json")
+        pm = _profile("pmc_rmat22_batched.json")
+        if ks is not None and pm is not None:
+            kern = ks.get("dominant", {}).get(name)
+            k = ks["kernels"].get(kern) if kern else None
+            tr = pm.get("kernels", {}).get(kern) if kern else None
+            if k and tr:
+                gbs = tr["hbm_bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
+                line["roofline"] = {
+                    "kernel": kern, "bound": "hbm", "basis": "measured HBM traffic "
+                    "(rocprofv3 FETCH_SIZE + WRITE_SIZE) per launch over the kernel's "
+                    "rocprofv3 average duration", "achieved": round(gbs, 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 5),
+                    "traffic": tr["hbm_bytes_per_launch"], "avg_launch_us": k["avg_us"]}
+        out[f"rmat22s_{name}64"] = line
     return out
 
 
-def served_lines(graphs, rates=(100, 1000, 5000, 20000), mix=(0.5, 0.3, 0.2)):
+def served_lines(graphs, edge_lists=None, rates=SERVED_RATES, mix=(0.5, 0.3, 0.2),
+                 err_rate=1000):
     srv = gas.Server(graphs, memory_budget=64 << 30)
     serve.serve(srv, serve.make_stream(1000, 64, graphs, mix=mix, seed=1))  # warm engines
-    ref = serve.make_stream(1000, 120, graphs, mix=mix, seed=7)
-    svc = serve.plain_service_times(graphs, ref)
     out = {"graphs": ["road1024", "rmat22s"],
            "mix": "/".join(str(round(100 * m)) for m in mix) + " bfs/sssp/ppr",
            "ppr_iterations": 10, "memory_budget_gib": 64,
-           "plain_mean_service_ms": round(float(svc.mean() * 1e3), 3),
-           "plain_saturated_qps": round(float(1 / svc.mean()), 1), "rates": {}}
-    rng = np.random.default_rng(11)
+           "plain": "the plain engine serving the same stream one query at a time "
+                    "in arrival order (gas_*_device, results in HBM)", "rates": {}}
     for rate in rates:
-        n = int(min(3000, max(200, rate)))
+        n = int(min(2000, max(200, rate)))
         st = serve.make_stream(rate, n, graphs, mix=mix, seed=rate)
-        res, lat, _ = serve.serve(srv, st)
-        # plain engine, FIFO: service time of each query drawn from the
-        # measured service times of the same kind and graph
-        s_plain = np.zeros(n)
-        for k in (0, 1, 2):
-            for gi in range(len(graphs)):
-                m = (st["kinds"] == k) & (st["graph_ids"] == gi)
-                pool = svc[(ref["kinds"] == k) & (ref["graph_ids"] == gi)]
-                if not len(pool):
-                    pool = svc
-                s_plain[m] = rng.choice(pool, int(m.sum()))
-        lat_p, end_p = serve.fifo_latencies(st["arrivals"], s_plain)
-        out["rates"][str(rate)] = {
+        keep = None
+        if edge_lists is not None and rate == err_rate and mix[2] > 0:
+            ppr = np.nonzero(st["kinds"] == gas.BATCH_PPR)[0]
+            keep = [int(q) for gi in range(len(graphs))
+                    for q in ppr[st["graph_ids"][ppr] == gi][:2 if gi == 0 else 1]]
+        res, lat, results = serve.serve(srv, st, keep=keep)
+        lat_p, end_p = serve.plain_replay(graphs, st)
+        plain_qps = n / end_p
+        entry = {
+            "queries": n,
             "served": {k: res[k] for k in ("qps", "p50_ms", "p99_ms", "max_concurrent",
                                            "rounds", "solo")},
-            "plain_fifo": {"qps": round(n / end_p, 2),
-                           "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
-                           "p99_ms": round(float(np.percentile(lat_p, 99)), 3)},
-            "qps_vs_plain": round(res["qps"] / (n / end_p), 2)}
+            "plain": {"qps": round(plain_qps, 2),
+                      "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat_p, 99)), 3)},
+            "qps_vs_plain": round(res["qps"] / plain_qps, 2)}
+        if keep:
+            errs = []
+            for q in keep:
+                s, d, _ = edge_lists[st["graph_ids"][q]].arrays()
+                nv = graphs[st["graph_ids"][q]].n_vertices
+                e = fp_errors(results[q], fp64_pagerank(nv, s, d, 10, int(st["sources"][q])))
+                e["graph"] = out["graphs"][st["graph_ids"][q]]
+                errs.append(e)
+            entry["ppr_l1_vs_fp64"] = errs
+        out["rates"][str(rate)] = entry
         log(f"served rate {rate} ({out['mix']}) done")
     return out
 
@@ -414,10 +617,12 @@ def main():
     h = headline(g, args.steps, args.warmup)
     log("headline done")
     gteps = h["er"] * args.steps / h["total_s"] / 1e9
+    step_s = h["total_s"] / args.steps
+    host_s = h["total_host_s"] / args.steps
     line = {
         "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(h["total_s"] / args.steps * 1e3, 4),
+        "ms_per_step": round(step_s * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic (seeded generator, graph resident in HBM)",
         "config": {"workload": "road1024: BFS + SSSP from vertex 0, 1024x1024 road-like grid, "
@@ -427,43 +632,91 @@ def main():
             "bfs_device_ms": round(h["bfs"].ms, 4), "sssp_device_ms": round(h["sssp"].ms, 4),
             "bfs_supersteps": h["bfs"].supersteps, "sssp_supersteps": h["sssp"].supersteps,
             "ms_per_step_with_kernel_events": round(h["total_timed_s"] / args.steps * 1e3, 4)},
+        "results_on_host": {"ms_per_step": round(host_s * 1e3, 4),
+                            "gteps": round(h["er"] / host_s / 1e9, 4),
+                            "window": "gas_bfs + gas_sssp: host initial state, upload, run, "
+                                      "download (sssp.cu:77-81)"},
         "roofline": roofline_entry(h["dom"], h["table"], "road1024"),
+        "roofline_8d": frac_entry(h["bytes_8d"], step_s),
         "kernels": h["table"],
     }
     del g
     if args.cpu:
         line["cpu_baseline"] = cpu_baseline(el)
         log("cpu baseline done")
-    del el
     want = set(args.lines.split(","))
     if args.extra:
         if "apps" in want:
             line["apps"] = app_lines(min(args.steps, 5))
         if want & {"batched", "served"}:
-            road = gas.Graph(workloads.road())
-            g22 = gas.Graph(workloads.rmat(22, symmetrize=True, weights=True))
+            road = gas.Graph(el)
+            el22 = workloads.rmat(22, symmetrize=True, weights=True)
+            t = time.perf_counter()
+            g22 = gas.Graph(el22)
+            line["rmat22s_graph_create_s"] = round(time.perf_counter() - t, 3)
             log("rmat22 built")
             # batch autotuner: BFS pull divisor and SSSP distance step per graph
             tuned = {}
-            for name, g in (("road1024", road), ("rmat22s", g22)):
-                tuned[name] = {"bfs_pull_divisor": g.autotune_batch(gas.BATCH_BFS),
-                               "sssp_delta": g.autotune_batch(gas.BATCH_SSSP)}
+            for name, gg in (("road1024", road), ("rmat22s", g22)):
+                tuned[name] = {"bfs_pull_divisor": gg.autotune_batch(gas.BATCH_BFS),
+                               "sssp_delta": gg.autotune_batch(gas.BATCH_SSSP)}
             line["batch_tuning"] = tuned
             log(f"batch tuning {tuned}")
             if "batched" in want:
                 line["batched"] = batched_lines(g22)
                 log("batched done")
             if "served" in want:
-                line["served"] = served_lines([road, g22])
+                line["served"] = served_lines([road, g22], [el, el22])
                 # BFS / SSSP only (the 50/30 part of the mix), at saturation
                 line["served_bfs_sssp"] = served_lines([road, g22], rates=(20000,),
                                                        mix=(0.625, 0.375, 0.0))
+            if args.cpu:
+                allc = cpu_all_cores([("road1024", el), ("rmat22s", el22)])
+                log("cpu all-core done")
+                if allc is not None:
+                    attach_cpu_all_cores(line, allc)
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         # one device-to-device copy of 256 MiB (the runtime's copy kernel)
         a, b = gas.DeviceBuffer(256 << 20), gas.DeviceBuffer(256 << 20)
         gas.device_copy(b, a)
         log("copy calibration: 268435456 bytes read, 268435456 written")
+
+
+def attach_cpu_all_cores(line, allc):
+    """All-core CPU figures beside the batched and served lines."""
+    nw = allc["workers"]
+    combos = allc["combos"]
+    desc = (f"{nw} worker processes at once (the CPU share of one GPU), each with one "
+            f"engine of the reference (refgas.h) bound to its graph, answering uniform "
+            f"sources of one (kind, graph) for {allc['budget_s']:g} s after binding it; "
+            f"all-core queries/s = {nw} / mean "
+            f"service time")
+    per = {f"{g}_{serve.KIND_NAMES[k]}": {"mean_ms": round(v["mean_ms"], 3),
+                                          "queries": v["queries"], "instances": v["instances"]}
+           for (g, k), v in combos.items()}
+    base = {"kind": "reference", "cores": nw, "cpu_model": allc["cpu_model"],
+            "sample": desc, "per_kind": per}
+    for name, kind in (("rmat22s_bfs64", 0), ("rmat22s_sssp64", 1)):
+        b = line.get("batched", {}).get(name)
+        c = combos.get(("rmat22s", kind))
+        if b is not None and c is not None:
+            qps = nw / (c["mean_ms"] / 1e3)
+            b["cpu_all_cores"] = dict(base, qps=round(qps, 3),
+                                      batch_vs_cpu=round(b["batch_qps"] / qps, 1))
+    for key, mix in (("served", (0.5, 0.3, 0.2)), ("served_bfs_sssp", (0.625, 0.375, 0.0))):
+        s = line.get(key)
+        if s is None:
+            continue
+        mean = 0.0
+        for k, frac in enumerate(mix):
+            for g in ("road1024", "rmat22s"):
+                c = combos.get((g, k))
+                if c is not None and frac > 0:
+                    mean += 0.5 * frac * c["mean_ms"]
+        if mean > 0:
+            s["cpu_all_cores"] = dict(base, saturated_qps=round(nw / (mean / 1e3), 3),
+                                      mean_service_ms=round(mean, 3))
 
 
 if __name__ == "__main__":

@@ -20,7 +20,7 @@
  *   gas_bfs/gas_sssp/gas_cc/gas_pagerank  whole runs, the app drivers'
  *                run<Engine>() (bfs.cu:91-112, sssp.cu:67-82,
  *                connected_component.cu:66-81, pagerank.cu:85-101).
- *   gas_ms_*     64-source batched BFS / SSSP / personalised PageRank.
+ *   gas_batch_*  64-source batched BFS / SSSP / personalised PageRank.
  *   gas_server_* continuous-batching query server over resident graphs.
  */
 #ifndef GAS_H_
@@ -261,6 +261,10 @@ typedef struct {
   int32_t scatter_launches;  /* k_apply + k_scatter_vp launches            */
   double  kernel_ms[GAS_K_SLOTS];        /* per kernel, summed over launches */
   int32_t kernel_launches[GAS_K_SLOTS];
+  int64_t returned_total;    /* sum over supersteps of vertices whose apply */
+                             /* returned true (|R_i|)                      */
+  int64_t hub_vertices_total; /* of active_total: gathered by the hub      */
+  int64_t hub_edges_total;    /* kernels, and their in-edges               */
 } gas_run_stats;
 
 /* Set to nonzero to bracket every engine kernel launch with HIP events on

@@ -544,7 +544,7 @@ class GASEngineGPU {
 
   void resetStats() {
     collectTiming();
-    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * 3 * SCAN_BLOCKS,
+    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS,
                        m_stream),
           "memset");
     m_supersteps = 0;
@@ -557,18 +557,21 @@ class GASEngineGPU {
 
   void stats(gas_run_stats *st) {
     collectTiming();
-    std::vector<int64_t> acc((size_t)3 * SCAN_BLOCKS);
+    std::vector<int64_t> acc((size_t)STAT_SLOTS * SCAN_BLOCKS);
     check(gasrt_memcpy_d2h(acc.data(), m_stat.p, sizeof(int64_t) * acc.size(),
                            m_stream),
           "stats");
-    int64_t tot[3] = {0, 0, 0};
-    for (int k = 0; k < 3; ++k)
+    int64_t tot[STAT_SLOTS] = {};
+    for (int k = 0; k < STAT_SLOTS; ++k)
       for (int b = 0; b < SCAN_BLOCKS; ++b) tot[k] += acc[(size_t)k * SCAN_BLOCKS + b];
     st->supersteps = (int32_t)m_supersteps;
     st->reserved = 0;
     st->gathered_edges = tot[0];
     st->scattered_edges = tot[1];
     st->active_total = tot[2];
+    st->returned_total = tot[3];
+    st->hub_vertices_total = tot[4];
+    st->hub_edges_total = tot[5];
     st->gather_ms = m_gather_ms;
     st->scatter_ms = m_scatter_ms;
     st->gather_launches = m_kernel_launches[GAS_K_GATHER];
@@ -792,14 +795,14 @@ class GASEngineGPU {
     m_shoff.alloc(V + 1);
     m_shpart.alloc(tiles);
     m_bsum.alloc(SCAN_BLOCKS);
-    m_stat.alloc(3 * SCAN_BLOCKS);
+    m_stat.alloc(STAT_SLOTS * SCAN_BLOCKS);
     m_bcnt.alloc(2 * SCAN_BLOCKS);
     m_rpart.alloc(SCAN_BLOCKS);
     m_pfirst.alloc(tiles);
     m_plast.alloc(tiles);
     check(gasrt_memset(m_fl0.p, 0, (size_t)nfwords * 4, m_stream), "memset");
     check(gasrt_memset(m_fl1.p, 0, (size_t)nfwords * 4, m_stream), "memset");
-    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * 3 * SCAN_BLOCKS,
+    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS,
                        m_stream),
           "memset");
     check(gasrt_memset(m_bcnt.p, 0, sizeof(int32_t) * 2 * SCAN_BLOCKS,

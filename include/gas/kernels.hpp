@@ -74,6 +74,7 @@ constexpr int T1_MAX = 16;         // largest thread tier (unrolled loads)
 constexpr int MID_UNROLL = 8;      // wave tier: edges per lane per trip
 constexpr int DEFAULT_T1 = 8;      // thread tier: degree <= t1
 constexpr int DEFAULT_T2 = 8;      // wave tier: degree <= t2; above: hubs
+constexpr int STAT_SLOTS = 6;      // run totals kept per workgroup (add_stat)
 
 // ---------------------------------------------------------------------------
 // Superstep number visible to vertex programs.  The engine's apply kernel
@@ -363,7 +364,7 @@ struct Dev {
   int32_t *shub, *shoff, *shpart;
   int32_t *bsum;     // per-chunk sums of the two-pass scans
   int32_t *bcnt;     // [2][SCAN_BLOCKS] active slots per workgroup, bitmap input
-  int64_t *stat_acc; // [3][SCAN_BLOCKS] per-workgroup totals: gathered, scattered, active
+  int64_t *stat_acc; // [STAT_SLOTS][SCAN_BLOCKS] per-workgroup totals (add_stat)
   double *rpart;     // per-chunk FP64 residual sums
   int64_t *status;   // host-mapped status ring of STATUS_SLOTS words (PUB_*)
   // set by the host for every launch
@@ -464,6 +465,32 @@ __device__ inline void block_sum3(int &a, int &b, int &c, int *lds) {
   a = ta;
   b = tb;
   c = tc;
+}
+
+// Workgroup sums of four counters in one LDS exchange (lds: 4 * NT / WAVE).
+__device__ inline void block_sum4(int &a, int &b, int &c, int &e, int *lds) {
+  const int sa = wave_sum(a), sb = wave_sum(b), sc = wave_sum(c), se = wave_sum(e);
+  const int w = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    lds[w] = sa;
+    lds[NT / WAVE + w] = sb;
+    lds[2 * (NT / WAVE) + w] = sc;
+    lds[3 * (NT / WAVE) + w] = se;
+  }
+  __syncthreads();
+  int ta = 0, tb = 0, tc = 0, te = 0;
+#pragma unroll
+  for (int i = 0; i < NT / WAVE; ++i) {
+    ta += lds[i];
+    tb += lds[NT / WAVE + i];
+    tc += lds[2 * (NT / WAVE) + i];
+    te += lds[3 * (NT / WAVE) + i];
+  }
+  __syncthreads();
+  a = ta;
+  b = tb;
+  c = tc;
+  e = te;
 }
 
 __device__ inline double block_sum(double x, double *lds) {
@@ -612,7 +639,8 @@ __device__ inline bool claim_vertex(uint8_t *fl, int u) {
   return (old & bit) == 0;
 }
 
-// Per-workgroup run totals (0 gathered, 1 scattered, 2 active): each
+// Per-workgroup run totals (0 gathered, 1 scattered, 2 active, 3 vertices
+// whose apply returned true, 4 gather hubs, 5 their in-edges): each
 // workgroup adds into its own slot (no contention); a non-returning atomic
 // add, so the workgroup does not wait for a read before it ends.
 template <class P>
@@ -739,7 +767,7 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
   Ctl *c = d.ctl;
   const bool all = d.in_mode == IN_ALL;
   const int lane = lane_id();
-  int gathered = 0;
+  int gathered = 0, hubv = 0, hube = 0;
   for_each_active(d, p, false, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     int o = 0, deg = 0;
     if (valid) {
@@ -750,6 +778,8 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
     const bool small = valid && deg <= d.gt1;
     const bool mid = valid && deg > d.gt1 && deg <= d.gt2;
     const bool hub = valid && deg > d.gt2;
+    hubv += hub ? 1 : 0;
+    hube += hub ? deg : 0;
     if (small) {
       // every load of the vertex issued before the first use
       int src[T1_MAX];
@@ -802,8 +832,13 @@ __global__ __launch_bounds__(NT) void k_gather_vp(Dev<P> d, int p) {
       block_append(hub, i, d.ghub, &c->ring[d.seq & 3].n_ghub, s_buf, &s_cnt,
                    &s_base);
   });
-  gathered = block_sum(gathered, lds);
+  {
+    __shared__ int lds3[3 * (NT / WAVE)];
+    block_sum3(gathered, hubv, hube, lds3);
+  }
   add_stat(d, 0, gathered);
+  add_stat(d, 4, hubv);
+  add_stat(d, 5, hube);
 }
 
 // Hub list of a superstep: all-active supersteps use the fixed list.
@@ -1170,7 +1205,7 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   superstep_prologue(d, p, lds);
   if (threadIdx.x == 0) s_superstep = d.iter;
   __syncthreads();
-  int scattered = 0, active = 0;
+  int scattered = 0, active = 0, returned = 0;
   double res = 0.0;
   for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     bool ret = false;
@@ -1196,6 +1231,7 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
       }
       if constexpr (is_source_only<P>::value)
         d.srcval[v] = P::gatherMap(d.vdata + v, d.vdata + v, nullptr);
+      returned += ret ? 1 : 0;
     }
     if (Fuse)
       scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
@@ -1204,13 +1240,13 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   });
   {
     __shared__ int lds3[3 * (NT / WAVE)];
-    int none = 0;
-    block_sum3(active, scattered, none, lds3);
+    block_sum3(active, scattered, returned, lds3);
   }
   if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
     d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
   add_stat(d, 2, active);
   add_stat(d, 1, scattered);
+  add_stat(d, 3, returned);
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
     if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
@@ -1240,7 +1276,7 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
   __syncthreads();
   uint8_t *chg = d.chg[p];
   const int lane = lane_id();
-  int gathered = 0, scattered = 0, active = 0;
+  int gathered = 0, scattered = 0, active = 0, returned = 0;
   double res = 0.0;
   // vertices changed last superstep and inactive now: copy their state into
   // the buffer this superstep writes (bitmap input; after an all input or a
@@ -1324,18 +1360,20 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
       }
       d.vnext[v] = x;
       if (ret) chg[v] = 1;
+      returned += ret ? 1 : 0;
     }
     scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
   }, fixup);
   {
-    __shared__ int lds3[3 * (NT / WAVE)];
-    block_sum3(active, gathered, scattered, lds3);
+    __shared__ int lds4[4 * (NT / WAVE)];
+    block_sum4(active, gathered, scattered, returned, lds4);
   }
   if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
     d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
   add_stat(d, 2, active);
   add_stat(d, 0, gathered);
   add_stat(d, 1, scattered);
+  add_stat(d, 3, returned);
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
     if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
@@ -1486,7 +1524,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_small(Dev<P> d, int p,
   }
   __syncthreads();
   int cur = p, steps = 0, n = s_n;
-  int64_t gathered = 0, scattered = 0, active = 0;
+  int64_t gathered = 0, scattered = 0, active = 0, returned = 0;
   while (n > 0 && n <= small_max && steps < max_steps) {
     const int32_t *q = d.q[cur];
     int32_t *qn = d.q[cur ^ 1];
@@ -1516,6 +1554,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_small(Dev<P> d, int p,
         d.srcval[v] = P::gatherMap(&x, &x, nullptr);
       if (ret) {
         const int o = d.csr_off[v], o1 = d.csr_off[v + 1];
+        ++returned;
         scattered += o1 - o;
         for (int e = o; e < o1; ++e) {
           const int u = d.csr_dst[e];
@@ -1531,14 +1570,15 @@ __global__ __launch_bounds__(SMALL_NT) void k_small(Dev<P> d, int p,
     ++steps;
   }
   // statistics of this workgroup's threads into slot 0
-  __shared__ unsigned long long s_stat[3];
-  if (threadIdx.x < 3) s_stat[threadIdx.x] = 0;
+  __shared__ unsigned long long s_stat[4];
+  if (threadIdx.x < 4) s_stat[threadIdx.x] = 0;
   __syncthreads();
   if (gathered) atomicAdd(&s_stat[0], (unsigned long long)gathered);
   if (scattered) atomicAdd(&s_stat[1], (unsigned long long)scattered);
   if (active) atomicAdd(&s_stat[2], (unsigned long long)active);
+  if (returned) atomicAdd(&s_stat[3], (unsigned long long)returned);
   __syncthreads();
-  if (threadIdx.x < 3 && s_stat[threadIdx.x])
+  if (threadIdx.x < 4 && s_stat[threadIdx.x])
     atomicAdd(reinterpret_cast<unsigned long long *>(d.stat_acc +
                                                      threadIdx.x * SCAN_BLOCKS),
               s_stat[threadIdx.x]);

@@ -49,7 +49,9 @@ class RunStats(C.Structure):
                 ("active_total", C.c_int64), ("ms", C.c_double),
                 ("gather_ms", C.c_double), ("scatter_ms", C.c_double),
                 ("gather_launches", C.c_int32), ("scatter_launches", C.c_int32),
-                ("kernel_ms", C.c_double * 12), ("kernel_launches", C.c_int32 * 12)]
+                ("kernel_ms", C.c_double * 12), ("kernel_launches", C.c_int32 * 12),
+                ("returned_total", C.c_int64), ("hub_vertices_total", C.c_int64),
+                ("hub_edges_total", C.c_int64)]
 
     def as_dict(self) -> dict:
         out = {f: getattr(self, f) for f, _ in self._fields_

@@ -3,8 +3,7 @@ and the plain engine answering the same stream one query at a time.
 
     stream = make_stream(rate_qps=2000, n=1000, graphs=[road, rmat], seed=5)
     served = serve(server, stream)            # continuous batching
-    svc = plain_service_times(graphs, stream) # one query at a time
-    lat, end = fifo_latencies(stream["arrivals"], svc)
+    lat, end = plain_replay(graphs, stream)    # the plain engine, one at a time
 """
 from __future__ import annotations
 
@@ -77,6 +76,33 @@ def plain_service_times(graphs, stream, ppr_iterations: int = 10):
             g.ppr_device(int(s), buf.ptr, ppr_iterations)
         svc[i] = time.perf_counter() - t
     return svc
+
+
+def plain_replay(graphs, stream, ppr_iterations: int = 10):
+    """The plain engine serving the stream itself: queries answered one at a
+    time in arrival order, each started no earlier than its arrival on the
+    host clock (gas_bfs_device / gas_sssp_device / gas_ppr_device, result
+    left in HBM).  Returns (latency_ms per query, seconds from the first
+    arrival to the last completion)."""
+    from . import DeviceBuffer
+    bufs = [DeviceBuffer(4 * g.n_vertices) for g in graphs]
+    arrivals = np.asarray(stream["arrivals"], dtype=np.float64)
+    n = len(arrivals)
+    finish = np.zeros(n)
+    t0 = time.perf_counter()
+    for i, (k, gi, s) in enumerate(zip(stream["kinds"], stream["graph_ids"],
+                                       stream["sources"])):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        g, buf = graphs[gi], bufs[gi]
+        if k == BATCH_BFS:
+            g.bfs_device(int(s), buf.ptr)
+        elif k == BATCH_SSSP:
+            g.sssp_device(int(s), buf.ptr)
+        else:
+            g.ppr_device(int(s), buf.ptr, ppr_iterations)
+        finish[i] = time.perf_counter() - t0
+    return (finish - arrivals) * 1e3, float(finish[-1]) if n else 0.0
 
 
 def fifo_latencies(arrivals, service_s):
