@@ -894,6 +894,8 @@ class BatchEngine {
           "sources");
     const unsigned grid = gridFor(m_nV);
     for (int i = 0; i < n; ++i) m_hstart[lanes[i]] = m_round;
+    m_alive_hint |= reset;
+    m_added_since_snap |= reset;
     check(gasrt_memcpy_h2d(m_lane_start.p, m_hstart, sizeof(m_hstart),
                            m_stream),
           "lane start");
@@ -924,9 +926,16 @@ class BatchEngine {
   // One round for every used lane.
   void step() {
     if (m_kind == BFS_KIND) {
-      for (int s = 0; s < BW; ++s)
-        if (((m_used >> s) & 1ull) && m_round + 1 - m_hstart[s] > 65534)
-          throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
+      // depths are 16-bit per lane: only lanes still running can outgrow
+      // them (a finished lane kept for its result does not); the host's
+      // view of which lanes run is refreshed when one reaches the limit
+      auto too_deep = [&](uint64_t lanes) {
+        for (int s = 0; s < BW; ++s)
+          if (((lanes >> s) & 1ull) && m_round + 1 - m_hstart[s] > 65534) return true;
+        return false;
+      };
+      if (too_deep(m_used & m_alive_hint) && too_deep(lanesAlive()))
+        throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
       launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       if (m_dev.pull_div > 0)
         launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
@@ -954,6 +963,7 @@ class BatchEngine {
     snapshotAsync();
   }
   void snapshotAsync() {
+    m_added_since_snap = 0;
     if (!m_ev) check(gasrt_event_create(&m_ev), "event");
     check(gasrt_memcpy_d2h_async(&m_hctl[1], m_ctl.p, sizeof(BatchCtl),
                                  m_stream),
@@ -965,18 +975,24 @@ class BatchEngine {
     if (m_ev) check(gasrt_event_query(m_ev, &done), "event query");
     return done != 0;
   }
-  uint64_t aliveSnapshot() const { return m_hctl[1].lanes_alive & m_used; }
+  uint64_t aliveSnapshot() {
+    // lanes added after the snapshot was taken are not in it
+    m_alive_hint &= m_hctl[1].lanes_alive | m_added_since_snap;
+    return m_hctl[1].lanes_alive & m_used;
+  }
 
   // Lanes whose query still has work (reads the device; synchronises).
   uint64_t lanesAlive() {
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(BatchCtl), m_stream),
           "ctl");
+    m_alive_hint = m_hctl[0].lanes_alive;
     return m_hctl[0].lanes_alive & m_used;
   }
 
   BatchCtl readCtl() {
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(BatchCtl), m_stream),
           "ctl");
+    m_alive_hint = m_hctl[0].lanes_alive;
     return m_hctl[0];
   }
 
@@ -1043,6 +1059,8 @@ class BatchEngine {
   int m_round = 0;
   uint64_t m_used = 0;
   int32_t m_hstart[BW] = {};
+  uint64_t m_alive_hint = 0;        // lanes not yet seen finished by the host
+  uint64_t m_added_since_snap = 0;  // lanes added after the last snapshotAsync
   BatchDev m_dev{};
   DevBuf<BatchCtl> m_ctl;
   DevBuf<int4> m_item;

@@ -221,6 +221,25 @@ def test_batch_bfs_deep_levels(gpu):
         assert np.array_equal(res[lane], want), lane
 
 
+def test_batch_bfs_level_limit(gpu):
+    # depths are 16-bit per lane: a lane still running past 65534 levels is
+    # an error, a finished lane held for its result is not
+    n = 70002
+    s = np.concatenate([np.arange(69999), [70000]]).astype(np.int32)
+    d = np.concatenate([np.arange(1, 70000), [70001]]).astype(np.int32)
+    g = gas.Graph(gas.Edges.from_arrays(n, s, d))
+    b = gas.Batch(g, gas.BATCH_BFS)
+    b.add([0], [70000])          # done after two levels
+    b.step(66000)                # held, not released, far past the limit
+    assert b.alive() == 0
+    r = b.results([0])[0]
+    assert r[70000] == 0 and r[70001] == 1 and np.all(r[:70000] == -1)
+    b.add([1], [0])              # a path of 69999 levels
+    with pytest.raises(gas.GasError) as e:
+        b.run()
+    assert e.value.code == 5     # GAS_ERR_STATE
+
+
 def test_batch_and_server_edge_cases(gpu):
     # sources without out-edges, unreachable vertices, a one-vertex graph
     # with no edges, and a server stream of one query per kind
