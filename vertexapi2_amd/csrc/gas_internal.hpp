@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,7 @@ struct gas_graph {
   // with each vertex's id in it (new_of) and the original id of each of
   // its vertices (old_of), on the host and the device
   int32_t vertex_order = 1;
+  std::mutex order_mu;          // guards the lazy build of the ordered copy
   bool order_unskewed = false;  // checked once: too regular to reorder
   gas_graph *ordered = nullptr;
   std::vector<int32_t> new_of, old_of;

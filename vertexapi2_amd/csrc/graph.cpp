@@ -156,6 +156,9 @@ namespace gas {
 // and each in-list stays sorted by (new) source id.  The edge list is laid
 // out by new source, so the stable CSC sort keeps every in-list ascending.
 gas_graph *ordered_graph(gas_graph *g) {
+  // threads of different engine sets may make the first whole-run call on
+  // a graph at once: one builds the copy, the others wait for it
+  std::lock_guard<std::mutex> lock(g->order_mu);
   if (!g->vertex_order || (int64_t)g->nE < kOrderEdges || g->order_unskewed) return g;
   if (g->ordered) return g->ordered;
   const int32_t V = g->nV;
@@ -522,6 +525,7 @@ int gas_graph_ordered_view(gas_graph_t g, gas_graph_t *ordered,
 int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode) {
   if (!g || mode < 0 || mode > 1)
     return set_error(GAS_ERR_ARG, "gas_graph_set_vertex_order: bad arguments");
+  std::lock_guard<std::mutex> lock(g->order_mu);
   if (mode != g->vertex_order && g->ordered) {
     gas_graph_free(g->ordered);
     g->ordered = nullptr;
@@ -606,6 +610,9 @@ static int device_create(gas_edges_t edges, gas_graph *g) {
 int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
   return gas::host_guard("gas_graph_create", [&]() -> int {
     if (!edges || !out) return set_error(GAS_ERR_ARG, "gas_graph_create: null");
+    if (edges->src.size() > (size_t)INT32_MAX)
+      return set_error(GAS_ERR_ARG, "gas_graph_create: %zu edges exceed int32",
+                       edges->src.size());
     auto *g = new gas_graph;
     g->nV = edges->nV;
     g->nE = (int32_t)edges->src.size();

@@ -116,6 +116,11 @@ int read_edge_lines(GzFile &f, const TextRules &r, int64_t line_no,
     if (s == d) {
       if (r.values) continue;
     } else {
+      // edge ids are int32 (gpugas.h:75): a list that would outgrow them
+      // (a symmetric file stores two per line) is refused, not wrapped
+      if ((int64_t)e->src.size() + (r.sym != SYM_NONE ? 2 : 1) > INT32_MAX)
+        return set_error(GAS_ERR_IO, "%s: more than 2^31 - 1 edges at line %ld", r.who,
+                         (long)line_no);
       e->src.push_back((int32_t)s);
       e->dst.push_back((int32_t)d);
       if (r.sym != SYM_NONE) {
