@@ -339,6 +339,20 @@ int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats);
 int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
                         double *residual_dev, gas_run_stats *stats);
 
+/* gas_pagerank_device without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Every buffer the sweeps need comes from 64 MiB of device memory the
+ * library holds on each GPU it runs on until the process ends, enough for
+ * graphs up to about 1M vertices (a larger graph fails with GAS_ERR_ARG
+ * before any work); kernels run on `stream` (NULL: the null stream) and the
+ * call returns when the ranks are in rank_dev.  Runs on g's degree-ordered
+ * copy only if one already exists (gas_pagerank_device builds it), with the
+ * degree tiers of g's whole-run PageRank engine when one exists.  All calls
+ * on one GPU share that memory: calls that overlap on one GPU race. */
+int gas_pagerank_device_noalloc(gas_graph_t g, int32_t iterations, float *rank_dev,
+                                double *residual_dev, gasrt_stream_t stream);
+
 /* Personalised PageRank from one source, `iterations` all-active sweeps in
  * FP32 with compensated summation: r_0 = 0.15 e_s, r_{k+1}(v) = 0.15 [v==s]
  * + 0.85 sum_{u->v} r_k(u)/outdeg(u) (the PageRank program of

@@ -90,25 +90,54 @@ inline int &kernel_timing_flag() {
   return v;
 }
 
+// Device memory an engine may use instead of allocating: its buffers are
+// carved in order from [base, base + bytes) and its kernels run on the
+// caller's stream (the engine then allocates nothing, device or pinned).
+struct ExternalMemory {
+  gasrt_stream_t stream = nullptr;
+  char *base = nullptr;
+  size_t bytes = 0;
+  size_t used = 0;
+  // next 256-byte aligned piece of `bytes`
+  void *take(size_t want) {
+    const size_t b = ((want ? want : 1) + 255) & ~(size_t)255;
+    if (used + b > bytes)
+      throw Error(GAS_ERR_ARG, "engine needs more than the " +
+                                   std::to_string(bytes) + " bytes of external memory");
+    void *p = base + used;
+    used += b;
+    return p;
+  }
+};
+
 template <class T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
-  void alloc(size_t count) {
+  bool owned = false;
+  // from the device allocator, or from ext when given
+  void alloc(size_t count, ExternalMemory *ext = nullptr) {
     release();
+    if (ext) {
+      p = static_cast<T *>(ext->take((count ? count : 1) * sizeof(T)));
+      n = count;
+      return;
+    }
     check(gasrt_malloc((void **)&p, (count ? count : 1) * sizeof(T)),
           "gasrt_malloc");
+    owned = true;
     n = count;
   }
-  void upload(const std::vector<T> &h, gasrt_stream_t s) {
-    alloc(h.size());
+  void upload(const std::vector<T> &h, gasrt_stream_t s, ExternalMemory *ext = nullptr) {
+    alloc(h.size(), ext);
     if (!h.empty())
       check(gasrt_memcpy_h2d(p, h.data(), sizeof(T) * h.size(), s), "upload");
   }
   void release() {
-    if (p) gasrt_free(p);
+    if (p && owned) gasrt_free(p);
     p = nullptr;
     n = 0;
+    owned = false;
   }
   ~DevBuf() { release(); }
 };
@@ -140,6 +169,16 @@ class GASEngineGPU {
   typedef typename Program::GatherResult GatherResult;
   static constexpr bool kEmptyEdge = std::is_empty<EdgeData>::value;
 
+  // An engine that allocates nothing: device buffers from ext (which must
+  // outlive it), kernels on ext.stream, no host status ring (countActive and
+  // whole runs to an empty frontier are not available; fixed supersteps
+  // from setActive are), no kernel timing.
+  explicit GASEngineGPU(const ExternalMemory &ext) : m_ext_mem(ext) {
+    m_ext = &m_ext_mem;
+    m_stream = ext.stream;
+    m_dev.status = static_cast<int64_t *>(m_ext->take(sizeof(int64_t) * STATUS_SLOTS));
+  }
+
   GASEngineGPU() {
     check(gasrt_stream_create(&m_stream), "gasrt_stream_create");
     check(gasrt_host_alloc((void **)&m_hctl, sizeof(Ctl)), "gasrt_host_alloc");
@@ -151,6 +190,7 @@ class GASEngineGPU {
   }
 
   ~GASEngineGPU() {
+    if (m_ext) return;  // nothing of its own: buffers and stream are the caller's
     if (m_stream) gasrt_stream_synchronize(m_stream);
     for (auto &pr : m_timing) {
       gasrt_event_destroy(pr.first);
@@ -262,6 +302,7 @@ class GASEngineGPU {
   }
 
   Int countActive() {
+    if (m_ext) throw Error(GAS_ERR_STATE, "countActive: engine without host memory");
     if (m_in == IN_BITMAP)
       throw Error(GAS_ERR_STATE, "countActive: frontier held as a bitmap");
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(Ctl), m_stream),
@@ -537,6 +578,7 @@ class GASEngineGPU {
   }
 
   Ctl readCtl() {
+    if (m_ext) throw Error(GAS_ERR_STATE, "readCtl: engine without host memory");
     check(gasrt_memcpy_d2h(m_hctl, m_ctl.p, sizeof(Ctl), m_stream),
           "readCtl");
     return *m_hctl;
@@ -544,9 +586,7 @@ class GASEngineGPU {
 
   void resetStats() {
     collectTiming();
-    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS,
-                       m_stream),
-          "memset");
+    zero(m_stat.p, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS);
     m_supersteps = 0;
     m_gather_ms = m_scatter_ms = 0;
     for (int i = 0; i < GAS_K_SLOTS; ++i) {
@@ -707,9 +747,9 @@ class GASEngineGPU {
       }
     }
     std::vector<int32_t> part = host_tile_partition(hoff);
-    m_ahub.upload(hubs, m_stream);
-    m_ahoff.upload(hoff, m_stream);
-    m_ahpart.upload(part, m_stream);
+    m_ahub.upload(hubs, m_stream, m_ext);
+    m_ahoff.upload(hoff, m_stream, m_ext);
+    m_ahpart.upload(part, m_stream, m_ext);
     m_dev.gt1 = m_gt1;
     m_dev.gt2 = m_gt2;
     m_dev.st1 = m_st1;
@@ -729,8 +769,8 @@ class GASEngineGPU {
     m_nV = gv.n_vertices;
     m_nE = gv.n_edges;
     m_vd_host = vertexData;
-    m_vdata.alloc((size_t)m_nV);
-    m_vdata2.alloc((size_t)m_nV);
+    m_vdata.alloc((size_t)m_nV, m_ext);
+    m_vdata2.alloc((size_t)m_nV, m_ext);
     if (vertexData)
       check(gasrt_memcpy_h2d(m_vdata.p, vertexData,
                              sizeof(VertexData) * (size_t)m_nV, m_stream),
@@ -749,7 +789,7 @@ class GASEngineGPU {
       std::vector<int32_t> pos((size_t)m_nE), csr_csc((size_t)m_nE);
       for (Int k = 0; k < m_nE; ++k) pos[(size_t)m_csc_eid[(size_t)k]] = k;
       for (Int k = 0; k < m_nE; ++k) csr_csc[(size_t)k] = pos[(size_t)csr_eid[(size_t)k]];
-      m_csr_csc.upload(csr_csc, m_stream);
+      m_csr_csc.upload(csr_csc, m_stream, m_ext);
     }
     if (!kEmptyEdge) {
       if (edgeData) {
@@ -757,7 +797,7 @@ class GASEngineGPU {
         check(gas_graph_permute_to_csc(g, edgeData, sorted.data(),
                                        sizeof(EdgeData)),
               "permute edge data");
-        m_edata.upload(sorted, m_stream);
+        m_edata.upload(sorted, m_stream, m_ext);
         ed_dev = m_edata.p;
       } else if (sizeof(EdgeData) == 4 && gv.csc_weights) {
         ed_dev = reinterpret_cast<const EdgeData *>(gv.csc_weights);
@@ -779,40 +819,34 @@ class GASEngineGPU {
     const size_t V = (size_t)m_nV;
     const int nfwords = (int)((V + 3) / 4);
     const size_t tiles = ((size_t)m_nE + V) / MP_NV + 2;
-    m_ctl.alloc(1);
-    m_q0.alloc(V);
-    m_q1.alloc(V);
-    m_fl0.alloc((size_t)nfwords * 4);
-    m_fl1.alloc((size_t)nfwords * 4);
-    m_chg0.alloc((size_t)nfwords * 4);
-    m_chg1.alloc((size_t)nfwords * 4);
-    m_rflag.alloc(V);
-    m_gtmp.alloc(V);
-    m_ghub.alloc(V);
-    m_ghoff.alloc(V + 1);
-    m_ghpart.alloc(tiles);
-    m_shub.alloc(V);
-    m_shoff.alloc(V + 1);
-    m_shpart.alloc(tiles);
-    m_bsum.alloc(SCAN_BLOCKS);
-    m_stat.alloc(STAT_SLOTS * SCAN_BLOCKS);
-    m_bcnt.alloc(2 * SCAN_BLOCKS);
-    m_rpart.alloc(SCAN_BLOCKS);
-    m_pfirst.alloc(tiles);
-    m_plast.alloc(tiles);
-    check(gasrt_memset(m_fl0.p, 0, (size_t)nfwords * 4, m_stream), "memset");
-    check(gasrt_memset(m_fl1.p, 0, (size_t)nfwords * 4, m_stream), "memset");
-    check(gasrt_memset(m_stat.p, 0, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS,
-                       m_stream),
-          "memset");
-    check(gasrt_memset(m_bcnt.p, 0, sizeof(int32_t) * 2 * SCAN_BLOCKS,
-                       m_stream),
-          "memset");
-    check(gasrt_memset(m_rpart.p, 0, sizeof(double) * SCAN_BLOCKS, m_stream),
-          "memset");
-    Ctl c;
-    memset(&c, 0, sizeof(c));
-    check(gasrt_memcpy_h2d(m_ctl.p, &c, sizeof(Ctl), m_stream), "ctl");
+    ExternalMemory *x = m_ext;
+    m_ctl.alloc(1, x);
+    m_q0.alloc(V, x);
+    m_q1.alloc(V, x);
+    m_fl0.alloc((size_t)nfwords * 4, x);
+    m_fl1.alloc((size_t)nfwords * 4, x);
+    m_chg0.alloc((size_t)nfwords * 4, x);
+    m_chg1.alloc((size_t)nfwords * 4, x);
+    m_rflag.alloc(V, x);
+    m_gtmp.alloc(V, x);
+    m_ghub.alloc(V, x);
+    m_ghoff.alloc(V + 1, x);
+    m_ghpart.alloc(tiles, x);
+    m_shub.alloc(V, x);
+    m_shoff.alloc(V + 1, x);
+    m_shpart.alloc(tiles, x);
+    m_bsum.alloc(SCAN_BLOCKS, x);
+    m_stat.alloc(STAT_SLOTS * SCAN_BLOCKS, x);
+    m_bcnt.alloc(2 * SCAN_BLOCKS, x);
+    m_rpart.alloc(SCAN_BLOCKS, x);
+    m_pfirst.alloc(tiles, x);
+    m_plast.alloc(tiles, x);
+    zero(m_fl0.p, (size_t)nfwords * 4);
+    zero(m_fl1.p, (size_t)nfwords * 4);
+    zero(m_stat.p, sizeof(int64_t) * STAT_SLOTS * SCAN_BLOCKS);
+    zero(m_bcnt.p, sizeof(int32_t) * 2 * SCAN_BLOCKS);
+    zero(m_rpart.p, sizeof(double) * SCAN_BLOCKS);
+    zero(m_ctl.p, sizeof(Ctl));
 
     Dev<Program> &d = m_dev;
     d.nV = m_nV;
@@ -826,7 +860,7 @@ class GASEngineGPU {
     d.csr_csc = m_csr_csc.p;
     d.vdata = m_vdata.p;
     d.vnext = m_vdata2.p;
-    m_srcval.alloc(is_source_only<Program>::value ? (size_t)m_nV : 1);
+    m_srcval.alloc(is_source_only<Program>::value ? (size_t)m_nV : 1, x);
     d.srcval = m_srcval.p;
     m_srcval_stale = true;
     m_flags_dirty = true;
@@ -873,7 +907,7 @@ class GASEngineGPU {
   // (0 gather, 1 apply / scatter, 2 other).
   template <class K>
   void klaunch(int id, int phase, K k, unsigned grid) {
-    const bool timed = kernel_timing_flag() != 0;
+    const bool timed = kernel_timing_flag() != 0 && !m_ext;
     if (timed) {
       if (m_tpos == m_timing.size()) {
         gasrt_event_t a, b;
@@ -917,6 +951,19 @@ class GASEngineGPU {
       if (m_tpos >= 4096) collectTiming();
     }
   }
+  // Clears device memory: a memset, or on an engine without its own
+  // memory a fill kernel (nothing the caller's stream could not replay)
+  void zero(void *p, size_t bytes) {
+    if (!m_ext) {
+      check(gasrt_memset(p, 0, bytes, m_stream), "memset");
+      return;
+    }
+    const int64_t words = (int64_t)(bytes / 4);
+    const unsigned grid =
+        (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (words + NT - 1) / NT));
+    launch(k_zero_words<uint32_t>, grid, m_stream, static_cast<uint32_t *>(p), words);
+  }
+
   void collectTiming() {
     if (m_tpos == 0) return;
     check(gasrt_stream_synchronize(m_stream), "sync");
@@ -934,6 +981,8 @@ class GASEngineGPU {
     m_tpos = 0;
   }
 
+  ExternalMemory m_ext_mem;          // the external memory, when constructed with it
+  ExternalMemory *m_ext = nullptr;   // &m_ext_mem on an engine without its own memory
   gasrt_stream_t m_stream = nullptr;
   Ctl *m_hctl = nullptr;            // pinned copy of the control block
   int64_t *m_status = nullptr;      // mapped status ring (STATUS_SLOTS)

@@ -1649,6 +1649,15 @@ __global__ void k_srcval_all(Dev<P> d) {
   }
 }
 
+// Zeroes `words` 32-bit words (the engine's clears when it must not call
+// the runtime's memset).
+template <class W>
+__global__ void k_zero_words(W *p, int64_t words) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * NT)
+    p[i] = W(0);
+}
+
 // Vertex-data initialisation on the device: `value` everywhere, `one` at
 // vertex `at` (none if at < 0).  Replaces an upload of the whole array when
 // the initial state is uniform, as it is for BFS and SSSP.

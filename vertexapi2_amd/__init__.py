@@ -420,6 +420,15 @@ class Graph:
                                          C.byref(st)))
         return st
 
+    def pagerank_device_noalloc(self, out_ptr: int, iterations: int = 20,
+                                residual_ptr=None, stream=None) -> None:
+        """gas_pagerank_device_noalloc: the fixed-sweep run with every buffer
+        in the library's fixed device scratch (no allocation)."""
+        _check(lib().gas_pagerank_device_noalloc(
+            self._h, C.c_int32(iterations), C.c_void_p(out_ptr),
+            C.c_void_p(residual_ptr) if residual_ptr else None,
+            C.c_void_p(stream) if stream else None))
+
     def ppr_device(self, source: int, out_ptr: int, iterations: int = 10):
         """Personalised PageRank with the ranks written to device memory."""
         st = RunStats()

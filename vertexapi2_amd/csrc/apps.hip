@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "gas.h"
@@ -200,6 +202,47 @@ __global__ void k_ppr_rank_out(Dev<PersonalizedPageRank> d, const int32_t *new_o
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
        v += (int64_t)gridDim.x * NT)
     out[v] = d.vdata[new_of ? new_of[v] : v].rank;
+}
+
+// Device memory of the allocation-free entry point
+// (gas_pagerank_device_noalloc): every buffer its engine needs is carved
+// from this array, which each GPU the library runs on holds from module
+// load until the process ends.  In host code `scratch` only names the
+// symbol; its device address comes from scratch_address().
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` on the current device, looked up with
+// hipGetSymbolAddress once per device (each GPU has its own copy).
+static int scratch_address(void **out) {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  int dev = 0;
+  GAS_TRY(hip_status(hipGetDevice(&dev), "hipGetDevice"));
+  if (dev < 0 || dev >= 64) return set_error(GAS_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    GAS_TRY(hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                       "hipGetSymbolAddress(scratch)"));
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return GAS_OK;
+}
+
+// Upper bound of the scratch the PageRank engine carves for a graph of nV
+// vertices and nE edges (every buffer of GASEngineGPU::bindGraph, each
+// rounded up to 256 bytes, and the hub lists at their largest).
+static size_t noalloc_pagerank_bytes(int64_t nV, int64_t nE) {
+  const int64_t tiles = (nE + nV) / MP_NV + 2;
+  const int64_t per_vertex = 2 * 8 /* states */ + 4 /* srcval */ + 2 * 4 /* lists */ +
+                             4 * 1 /* activation, changed */ + 1 /* apply flags */ +
+                             4 /* gather results */ + 4 * 4 /* hub lists, offsets */ +
+                             2 * 4 /* all-active hubs, offsets */;
+  const int64_t per_tile = 3 * 4 /* partitions */ + 2 * (int64_t)sizeof(Acc<PageRank>);
+  return (size_t)(per_vertex * (nV + 4) + per_tile * tiles +
+                  SCAN_BLOCKS * (4 + 8 * STAT_SLOTS + 8 + 8) + 64 * 256);
 }
 
 }  // namespace gas
@@ -539,6 +582,54 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
       stats->gathered_edges = (int64_t)iterations * g->nE;
       stats->active_total = (int64_t)iterations * g->nV;
     }
+  });
+}
+
+int gas_pagerank_device_noalloc(gas_graph_t g, int32_t iterations, float *rank_dev,
+                                double *residual_dev, gasrt_stream_t stream) {
+  if (!g || !rank_dev || iterations < 1)
+    return set_error(GAS_ERR_ARG, "gas_pagerank_device_noalloc: bad arguments");
+  return guarded([&] {
+    void *base = nullptr;
+    check(scratch_address(&base), "scratch");
+    // the degree-ordered copy is used only when it already exists (building
+    // it allocates)
+    gas_graph_t pg = g;
+    {
+      std::lock_guard<std::mutex> lock(g->order_mu);
+      if (g->vertex_order && g->ordered) pg = g->ordered;
+    }
+    const size_t need = noalloc_pagerank_bytes(pg->nV, pg->nE);
+    if (need > kNoAllocScratchBytes)
+      throw Error(GAS_ERR_ARG,
+                  "gas_pagerank_device_noalloc: a graph of " + std::to_string(pg->nV) +
+                      " vertices and " + std::to_string(pg->nE) + " edges needs " +
+                      std::to_string(need) + " bytes of scratch; the maximum is " +
+                      std::to_string(kNoAllocScratchBytes));
+    ExternalMemory ext;
+    ext.stream = stream;
+    ext.base = static_cast<char *>(base);
+    ext.bytes = kNoAllocScratchBytes;
+    GASEngineGPU<PageRank> e(ext);
+    // the degree tiers of the whole-run engine on this graph, when there is one
+    if (void *cached = pg->engine_cache[engine_set()][GAS_PROGRAM_PAGERANK]) {
+      int tiers[4];
+      static_cast<GASEngineGPU<PageRank> *>(cached)->thresholds(tiers);
+      e.setThresholds(tiers[0], tiers[1], tiers[2], tiers[3]);
+    }
+    e.setGraph(pg, nullptr, nullptr);
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
+    launch(k_pr_init, grid, stream, e.dev());
+    e.stateChanged();
+    e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
+    for (int it = 0; it < iterations; ++it) {
+      e.gatherApply(true);
+      if (residual_dev) launch(k_residual_sum, 1, stream, e.dev().rpart, residual_dev, it);
+    }
+    launch(k_pr_rank_out, grid, stream, e.dev(),
+           pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, rank_dev);
+    check(gasrt_stream_synchronize(stream), "sync");
   });
 }
 
