@@ -135,6 +135,11 @@ def cpu_share():
 #   k_compact1    V (activation bytes); k_compact2: V + 4 per listed id
 #   k_fused       per active vertex: 8 (offsets) + state read + state write;
 #                 per in-edge as k_gather_vp; per scattered edge 5
+#   k_push        (min-programs pushed by their changed vertices) per
+#                 frontier vertex: 1 (activation byte) + state read + state
+#                 write (its own entry of the other buffer); per pushed
+#                 edge: 4 (destination id) + edge value + destination state
+#                 read + 1 (mark)
 #   k_small       (runs that never left the one-workgroup mode) per active
 #                 vertex: 4 (list id) + 8 (CSR offsets) + state read + write,
 #                 with a gather + 8 (CSC offsets); per in-edge as
@@ -149,6 +154,11 @@ def kernel_models(app, st, n_vertices, have_gather):
     if set(ks) <= {"k_small"}:
         out["k_small"] = (A * (12 + 2 * vd + (8 if have_gather else 0)) +
                           Eg * (4 + ed + sv) + Es * 5)
+        return out
+    if have_gather and "k_push" in ks:
+        # every superstep pushed: the frontier is the changed vertices
+        # (returned_total counts them, scattered_edges the pushes)
+        out["k_push"] = st.returned_total * (1 + 2 * vd) + Es * (4 + ed + vd + 1)
         return out
     if have_gather and "k_fused" in ks and "k_gather_vp" not in ks:
         # every superstep ran as one launch: gather + apply + scatter

@@ -218,6 +218,14 @@ int gas_graph_set_vertex_order(gas_graph_t g, int32_t mode);
 int gas_graph_ordered_view(gas_graph_t g, gas_graph_t *ordered,
                            const int32_t **new_of_dev, const int32_t **new_of_host);
 int gas_graph_view_get(gas_graph_t g, gas_graph_view *view);
+/* Out-neighbour table of a graph whose out-degrees are all at most 8
+ * (device arrays of 8 * n_vertices int32: row v holds v's out-neighbours in
+ * CSR order, then -1; and their weights, then 0, or NULL without weights),
+ * built at the first call and kept with the graph; NULLs for graphs with
+ * larger out-degrees.  The whole-run engines read a vertex's out-edges from
+ * it without first loading its CSR offsets.  No counterpart in the
+ * reference. */
+int gas_graph_ell(gas_graph_t g, const int32_t **ell_dst, const int32_t **ell_w);
 /* Host copies for checking: arrays of V+1 / E int32 (any may be NULL). */
 int gas_graph_host_csc(gas_graph_t g, int32_t *offsets, int32_t *srcs,
                        int32_t *edge_index);
@@ -244,7 +252,8 @@ enum {
   GAS_K_COMPACT2 = 9,  /* k_compact2: ordered frontier write               */
   GAS_K_FUSED = 10,    /* k_fused: gather + apply + scatter in one launch  */
   GAS_K_SMALL = 11,    /* k_small: supersteps of a small frontier, one WG  */
-  GAS_K_SLOTS = 12
+  GAS_K_PUSH = 12,     /* k_push: min-program superstep pushed by changes  */
+  GAS_K_SLOTS = 13
 };
 
 typedef struct {
@@ -298,6 +307,15 @@ int gas_graph_set_frontier_switch(gas_graph_t g, int program, int32_t divisor);
  * (gpugas.h:764-772). */
 int gas_graph_set_small_frontier(gas_graph_t g, int program,
                                  int32_t max_vertices);
+/* Push supersteps of whole SSSP / CC runs (enable 1, the default, or 0):
+ * the vertices whose value changed push it along their out-edges with
+ * atomic minima into the other state buffer, instead of every
+ * out-neighbour of theirs pulling over all its in-edges.  The values after
+ * every superstep, hence the results and the superstep counts, are the
+ * same; used when every out-degree is at most 64, the frontier switch is 0
+ * and the small-frontier mode is off for the graph.  No counterpart in the
+ * reference, whose engine only pulls (gpugas.h:421-513). */
+int gas_graph_set_push(gas_graph_t g, int program, int32_t enable);
 /* Autotuner: times candidate tiers on this graph for the program (gather
  * and scatter phases separately, vertex data untouched) and keeps the
  * fastest; tiers_out (may be NULL) receives them. */

@@ -354,6 +354,19 @@ class GASEngineGPU {
   // otherwise as activation bytes only (bitmap).
   void superstep(bool haveGather, bool outList) {
     m_out_list = outList ? 1 : 0;
+    if (m_push_run && haveGather && m_in != IN_LIST && !outList) {
+      // min-program: the changed vertices push along their out-edges into
+      // the other state buffer (k_push; same values as the pull)
+      if (m_other_stale)
+        check(gasrt_memcpy_d2d(m_dev.vnext, m_dev.vdata,
+                               sizeof(VertexData) * (size_t)m_nV, m_stream),
+              "state copy");
+      m_other_stale = false;
+      klaunch(GAS_K_PUSH, 0, k_push<Program>, SCAN_BLOCKS);
+      std::swap(m_dev.vdata, m_dev.vnext);
+      endSuperstep(false, false);
+      return;
+    }
     if (haveGather && !inHubs() && m_in != IN_LIST &&
         !is_source_only<Program>::value) {
       // gather, apply and scatter in one launch, state double-buffered
@@ -390,6 +403,14 @@ class GASEngineGPU {
   void runUntilEmpty(bool haveGather, bool /*haveScatter*/) {
     int64_t known = countActive();  // exact size of a list / all input
     if (known == 0) return;
+    // push supersteps for the whole run, or none: the push frontier is the
+    // set of vertices that changed, so the run must start from every vertex
+    // (setActive(0, V)), not from a list the pull's semantics defined
+    m_push_run = haveGather && m_in == IN_ALL && pushUsable();
+    struct PushReset {
+      bool &f;
+      ~PushReset() { f = false; }
+    } push_reset{m_push_run};
     const bool small_ok = m_small_max > 0 && m_max_in <= SMALL_DEG_MAX &&
                           m_max_out <= SMALL_DEG_MAX;
     const int64_t s0 = m_seq;
@@ -471,6 +492,24 @@ class GASEngineGPU {
     m_supersteps -= (m_seq - s0) - real;
     m_iter = it0 + (int)real;
     setActive(0, 0);
+  }
+
+  // Push supersteps for pushMin programs in whole runs (default on): used
+  // when every out-degree is at most WAVE, the frontier stays a bitmap
+  // (switch divisor 0) and the small-frontier mode cannot engage; the
+  // pull runs otherwise.  Values and superstep counts are the same either
+  // way; the run statistics count pushes (scattered), pushing vertices
+  // (active) and changed vertices (returned) instead.
+  void setPush(bool on) { m_push = on; }
+  bool pushUsable() const {
+    if constexpr (!is_push_min<Program>::value) {
+      return false;
+    } else {
+      const bool small_ok = m_small_max > 0 && m_max_in <= SMALL_DEG_MAX &&
+                            m_max_out <= SMALL_DEG_MAX;
+      return m_push && m_max_out <= WAVE && m_switch == 0 && !small_ok &&
+             (kEmptyEdge || m_dev.edata_csr != nullptr);
+    }
   }
 
   // Largest list frontier handed to k_small (0: never).
@@ -706,8 +745,8 @@ class GASEngineGPU {
 
   // Hub scatter, optional ordered compaction, and the move to the next
   // superstep: parity, counter-ring index, input mode of the next frontier.
-  void endSuperstep(bool compact) {
-    if (outHubs()) {
+  void endSuperstep(bool compact, bool hubScatter = true) {
+    if (hubScatter && outHubs()) {
       klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
       klaunch(GAS_K_HSCAN2, 1, k_hscan2<Program, 1>, SCAN_BLOCKS);
       klaunch(GAS_K_HSCATTER, 1, k_hscatter<Program>, MP_BLOCKS);
@@ -856,7 +895,30 @@ class GASEngineGPU {
     d.csr_off = gv.csr_offsets;
     d.csr_dst = gv.csr_dsts;
     d.edata = ed_dev;
+    // CSR-order copy of the edge values for push supersteps: the graph's
+    // own weights (caller edge data is kept in CSC order only)
+    d.edata_csr = nullptr;
+    if constexpr (!kEmptyEdge && sizeof(EdgeData) == 4)
+      if (!edgeData && gv.csr_weights)
+        d.edata_csr = reinterpret_cast<const EdgeData *>(gv.csr_weights);
     d.edata_rw = scatters_edges<Program>::value ? m_edata.p : nullptr;
+    // out-neighbour table for the fused activation and the push (graphs of
+    // out-degree <= ELL_W; its values only where the engine uses the
+    // graph's own weights); not on an engine that must not allocate
+    d.ell_dst = nullptr;
+    d.ell_w = nullptr;
+    if (!m_ext && m_max_out <= ELL_W && m_nV > 0) {
+      const int32_t *ed = nullptr, *ew = nullptr;
+      check(gas_graph_ell(g, &ed, &ew), "gas_graph_ell");
+      if constexpr (kEmptyEdge) {
+        d.ell_dst = ed;
+      } else if constexpr (sizeof(EdgeData) == 4) {
+        if (d.edata_csr && ew) {
+          d.ell_dst = ed;
+          d.ell_w = reinterpret_cast<const EdgeData *>(ew);
+        }
+      }
+    }
     d.csr_csc = m_csr_csc.p;
     d.vdata = m_vdata.p;
     d.vnext = m_vdata2.p;
@@ -994,6 +1056,8 @@ class GASEngineGPU {
   bool m_first = true;              // next launch is the superstep's first
   int m_switch = 0;                 // density switch divisor (0: bitmap only)
   int m_small_max = 0;              // k_small frontier limit (0: off)
+  bool m_push = true;               // push supersteps when usable (pushMin)
+  bool m_push_run = false;          // the current whole run pushes
   static constexpr int kSmallGraph = 8192;  // default k_small graphs: V <= this
   bool m_other_stale = true;        // vnext does not mirror vdata
   bool m_srcval_stale = true;       // srcval does not match the state

@@ -75,6 +75,7 @@ constexpr int MID_UNROLL = 8;      // wave tier: edges per lane per trip
 constexpr int DEFAULT_T1 = 8;      // thread tier: degree <= t1
 constexpr int DEFAULT_T2 = 8;      // wave tier: degree <= t2; above: hubs
 constexpr int STAT_SLOTS = 6;      // run totals kept per workgroup (add_stat)
+constexpr int ELL_W = 8;           // width of the out-neighbour table
 
 // ---------------------------------------------------------------------------
 // Superstep number visible to vertex programs.  The engine's apply kernel
@@ -163,6 +164,29 @@ struct is_source_only : std::false_type {};
 template <class P>
 struct is_source_only<P, std::void_t<decltype(P::gatherSourceOnly)>>
     : std::integral_constant<bool, P::gatherSourceOnly> {};
+
+// Optional program trait: `static constexpr bool pushMin = true` declares
+// a min-program whose superstep can run as a push: VertexData and
+// GatherResult are the same int, gatherMap(dst, src, edge) does not read
+// dst, gatherReduce is min, and apply(v, g) lowers v to g when g < v and
+// returns whether it did (SSSP, CC).  Then the superstep's new values are
+// min(old, min over changed in-neighbours u of gatherMap(u, edge)), which
+// the changed vertices can push along their out-edges with atomicMin
+// (k_push) -- the same values, superstep by superstep, as the pull over
+// the out-neighbours of the changed vertices.  An optional
+// `static bool pushSkip(const VertexData &)` names values whose pushes
+// cannot lower anything (SSSP's unreached distance).
+template <class P, class = void>
+struct is_push_min : std::false_type {};
+template <class P>
+struct is_push_min<P, std::void_t<decltype(P::pushMin)>>
+    : std::integral_constant<bool, P::pushMin> {};
+template <class P, class = void>
+struct has_push_skip : std::false_type {};
+template <class P>
+struct has_push_skip<P, std::void_t<decltype(P::pushSkip(
+                            std::declval<const typename P::VertexData &>()))>>
+    : std::true_type {};
 
 template <class T>
 __device__ inline T shfl_xor_4b(T x, int m) {
@@ -341,6 +365,11 @@ struct Dev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst;
   const ED *edata;  // CSC order
+  const ED *edata_csr;  // the same values in CSR order (push supersteps), or null
+  // out-neighbour table (gas_graph_ell): ELL_W ids per vertex, -1 after the
+  // last, and their edge values; null unless every out-degree <= ELL_W
+  const int32_t *ell_dst;
+  const ED *ell_w;
   ED *edata_rw;     // the same array, written by scatterEdges programs
   const int32_t *csr_csc;  // CSR slot -> CSC slot (scatterEdges programs)
   VD *vdata;        // current vertex state (read by gathers)
@@ -574,6 +603,15 @@ __device__ inline void block_append(bool take, int slot, int32_t *list,
   if (threadIdx.x == 0 && n) *s_base = atomicAdd(count, n);
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += NT) list[*s_base + i] = s_buf[i];
+}
+
+// Row v of the out-neighbour table: ELL_W ids (-1 after the last) in two
+// 16-byte loads.
+__device__ __forceinline__ void load_ell_row(const int32_t *ell, int v, int (&e)[ELL_W]) {
+  const int4 *row = reinterpret_cast<const int4 *>(ell + (int64_t)v * ELL_W);
+  const int4 r0 = row[0], r1 = row[1];
+  e[0] = r0.x; e[1] = r0.y; e[2] = r0.z; e[3] = r0.w;
+  e[4] = r1.x; e[5] = r1.y; e[6] = r1.z; e[7] = r1.w;
 }
 
 // ---------------------------------------------------------------------------
@@ -1207,12 +1245,20 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
   __syncthreads();
   int scattered = 0, active = 0, returned = 0;
   double res = 0.0;
+  // fused activation from the out-neighbour table: the ids load with the
+  // state, no CSR offsets first (bitmap output only)
+  const bool ell = Fuse && d.ell_dst && !d.out_list;
   for_each_active(d, p, Fuse, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
     bool ret = false;
     int so = 0, sdeg = 0;
+    int e[ELL_W];
     if (valid && Fuse) {
-      so = d.csr_off[v];
-      sdeg = d.csr_off[v + 1] - so;
+      if (ell) {
+        load_ell_row(d.ell_dst, v, e);
+      } else {
+        so = d.csr_off[v];
+        sdeg = d.csr_off[v + 1] - so;
+      }
     }
     if (valid) {
       ++active;
@@ -1233,10 +1279,21 @@ __global__ __launch_bounds__(NT) void k_apply(Dev<P> d, int p) {
         d.srcval[v] = P::gatherMap(d.vdata + v, d.vdata + v, nullptr);
       returned += ret ? 1 : 0;
     }
-    if (Fuse)
+    if (Fuse && ell) {
+      if (ret) {
+        uint8_t *fl = d.fl[p ^ 1];
+#pragma unroll
+        for (int k = 0; k < ELL_W; ++k)
+          if (e[k] >= 0) {
+            fl[e[k]] = 1;
+            ++scattered;
+          }
+      }
+    } else if (Fuse) {
       scattered += scatter_tiers(d, p, ret, i, so, sdeg, s_buf, &s_cnt, &s_base, lds);
-    else if (valid)
+    } else if (valid) {
       d.rflag[i] = ret ? 1 : 0;
+    }
   });
   {
     __shared__ int lds3[3 * (NT / WAVE)];
@@ -1377,6 +1434,163 @@ __global__ __launch_bounds__(NT) void k_fused(Dev<P> d, int p) {
   if constexpr (has_residual<P>::value) {
     res = block_sum(res, ldsd);
     if (threadIdx.x == 0) d.rpart[blockIdx.x] = res;
+  }
+}
+
+// A whole superstep of a pushMin program (see is_push_min) on an all or
+// bitmap frontier of the vertices whose value changed in the previous
+// superstep.  State is double-buffered like k_fused: vdata holds this
+// superstep's values (read only), vnext the previous superstep's values
+// everywhere except where they changed, which each frontier vertex first
+// lowers to its own value with atomicMin (a push may already have lowered
+// it further).  Each frontier vertex then pushes gatherMap(own value,
+// edge) along its out-edges: where the pushed value is below the
+// destination's value in vdata, an atomicMin into vnext and a mark (plain
+// byte store) that the destination changes in this superstep -- exactly
+// the destinations whose final vnext value ends below vdata.  After the
+// launch vnext holds min(old, pushes) everywhere: the pull's values.  The
+// frontier size reported for the next superstep counts the changed
+// vertices that push (they have out-edges; a changed value is never one
+// pushSkip names): the pull's next frontier is their out-neighbours, so
+// the two stop after the same superstep.  Out-degrees are at most WAVE (one
+// thread up to st1, one wave above), or at most ELL_W with the
+// out-neighbour table.
+template <class P>
+__global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
+  typedef typename P::VertexData VD;
+  typedef typename P::EdgeData ED;
+  __shared__ int32_t s_ids[BM_WORDS * 4];
+  __shared__ int lds[NT / WAVE];
+  if constexpr (is_push_min<P>::value) {
+    superstep_prologue(d, p, lds);
+    uint8_t *fn = d.fl[p ^ 1];
+    const int lane = lane_id();
+    const bool all = d.in_mode == IN_ALL;
+    int active = 0, pushed = 0, frontier = 0;
+    // a push can only matter below the destination's value in vdata (vnext
+    // ends at most there: its own frontier entry lowers it to that), and a
+    // push below it makes the destination change whatever else lands, so
+    // the test on vdata alone decides both the atomic and the mark; the
+    // atomic's old value is not needed (no returning round trip)
+    auto push_one = [&](VD x, int u) __attribute__((always_inline)) {
+      if (x < d.vdata[u]) {
+        atomicMin(d.vnext + u, x);
+        fn[u] = 1;
+      }
+    };
+    if (d.ell_dst) {
+      // out-edges from the out-neighbour table: ids and values load with
+      // the vertex's own value (after it in the first, all-vertex
+      // superstep, whose vertices mostly skip)
+      for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
+        VD a{};
+        int e[ELL_W];
+        ED w[ELL_W];
+        bool push = false;
+        if (valid) {
+          a = d.vdata[v];
+          bool skip = false;
+          if (all) {
+            if constexpr (has_push_skip<P>::value) skip = P::pushSkip(a);
+          } else {
+            atomicMin(d.vnext + v, a);
+          }
+          if (!skip) {
+            load_ell_row(d.ell_dst, v, e);
+            if constexpr (!std::is_empty<ED>::value) {
+              static_assert(sizeof(ED) == 4, "table values are 4 bytes");
+              int wi[ELL_W];
+              load_ell_row(reinterpret_cast<const int32_t *>(d.ell_w), v, wi);
+#pragma unroll
+              for (int k = 0; k < ELL_W; ++k) w[k] = __builtin_bit_cast(ED, wi[k]);
+            }
+            push = e[0] >= 0;
+          }
+          ++frontier;
+          if (push) ++active;
+        }
+        if (push) {
+          // every destination's value loaded before the first store (a byte
+          // store may alias them, which would serialise the loads)
+          VD cur[ELL_W];
+#pragma unroll
+          for (int k = 0; k < ELL_W; ++k)
+            if (e[k] >= 0) cur[k] = d.vdata[e[k]];
+#pragma unroll
+          for (int k = 0; k < ELL_W; ++k)
+            if (e[k] >= 0) {
+              const VD x = P::gatherMap(nullptr, &a, w + k);
+              if (x < cur[k]) {
+                atomicMin(d.vnext + e[k], x);
+                fn[e[k]] = 1;
+              }
+              ++pushed;
+            }
+        }
+      });
+    } else
+    for_each_active(d, p, true, s_ids, lds, [&](bool valid, int i, int v) __attribute__((always_inline)) {
+      VD a{};
+      int so = 0, deg = 0;
+      if (valid) {
+        a = d.vdata[v];
+        so = d.csr_off[v];
+        deg = d.csr_off[v + 1] - so;
+        ++frontier;
+        if (!all) atomicMin(d.vnext + v, a);  // all: the buffers are equal
+      }
+      bool push = valid && deg > 0;
+      if constexpr (has_push_skip<P>::value)
+        push = push && !P::pushSkip(a);
+      if (push) ++active;
+      if (push && deg <= d.st1) {
+        int dst[T1_MAX];
+        ED w[T1_MAX];
+#pragma unroll
+        for (int k = 0; k < T1_MAX; ++k) {
+          dst[k] = k < deg ? d.csr_dst[so + k] : 0;
+          if constexpr (!std::is_empty<ED>::value)
+            w[k] = k < deg ? d.edata_csr[so + k] : ED();
+        }
+        VD cur[T1_MAX];
+#pragma unroll
+        for (int k = 0; k < T1_MAX; ++k)
+          if (k < deg) cur[k] = d.vdata[dst[k]];
+#pragma unroll
+        for (int k = 0; k < T1_MAX; ++k)
+          if (k < deg) {
+            const VD x = P::gatherMap(nullptr, &a, w + k);
+            if (x < cur[k]) {
+              atomicMin(d.vnext + dst[k], x);
+              fn[dst[k]] = 1;
+            }
+          }
+        pushed += deg;
+      }
+      uint64_t m = __ballot(push && deg > d.st1);
+      while (m) {
+        const int j = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        const VD aj = __shfl(a, j, WAVE);
+        const int oj = __shfl(so, j, WAVE);
+        const int dj = __shfl(deg, j, WAVE);
+        if (lane < dj) {
+          ED w{};
+          if constexpr (!std::is_empty<ED>::value) w = d.edata_csr[oj + lane];
+          push_one(P::gatherMap(nullptr, &aj, &w), d.csr_dst[oj + lane]);
+        }
+        if (lane == j) pushed += dj;
+      }
+    });
+    {
+      __shared__ int lds3[3 * (NT / WAVE)];
+      block_sum3(active, pushed, frontier, lds3);
+    }
+    if (d.in_mode == IN_BITMAP && threadIdx.x == 0)
+      d.bcnt[p * SCAN_BLOCKS + blockIdx.x] = active;
+    add_stat(d, 2, active);
+    add_stat(d, 1, pushed);
+    add_stat(d, 3, frontier);
   }
 }
 

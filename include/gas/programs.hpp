@@ -46,13 +46,20 @@ struct BFS {
                                           const VertexData *, EdgeData *) {}
 };
 
-// sssp.cu:25-64
+// sssp.cu:25-64, plus the engine's pushMin trait (whole runs may push the
+// changed distances instead of pulling; same values every superstep)
 struct SSSP {
   typedef int VertexData;
   typedef int EdgeData;
   typedef int GatherResult;
   static constexpr int maxLength = 100000;
   static constexpr int gatherZero = INT_MAX - maxLength;
+  static constexpr bool pushMin = true;
+
+  // an unreached vertex's pushes (gatherZero + w) lower nothing
+  __host__ __device__ static bool pushSkip(const VertexData &d) {
+    return d >= gatherZero;
+  }
 
   __host__ __device__ static int gatherReduce(const int &left,
                                               const int &right) {
@@ -149,12 +156,13 @@ struct PersonalizedPageRank {
                                           const VertexData *, EdgeData *) {}
 };
 
-// connected_component.cu:25-63
+// connected_component.cu:25-63, plus the engine's pushMin trait
 struct CC {
   typedef int VertexData;
   struct EdgeData {};
   typedef int GatherResult;
   static constexpr int gatherZero = INT_MAX;
+  static constexpr bool pushMin = true;
 
   __host__ __device__ static int gatherReduce(const int &left,
                                               const int &right) {

@@ -330,6 +330,7 @@ def test_small_frontier_mode(gpu, small_max):
                                     lambda: oracle.bfs(n, s, d, src)),
                                    (gas.PROGRAM_SSSP, lambda: g.sssp(src),
                                     lambda: oracle.sssp(n, s, d, w, src))):
+                g.set_push(prog, False)  # the statistics below are the pull's
                 g.set_small_frontier(prog, 0)
                 _, st0 = run()
                 g.set_small_frontier(prog, small_max)
@@ -344,6 +345,55 @@ def test_small_frontier_mode(gpu, small_max):
         ref, ost = oracle.cc(n, s, d)
         assert np.array_equal(lab, ref)
         assert st.supersteps == ost.supersteps
+
+
+def test_push_equals_pull(gpu):
+    # SSSP and CC whole runs with push supersteps (the changed vertices push
+    # with atomic minima) against the pull and the oracle: identical values
+    # and superstep counts, on road graphs (long links, dropped links), on
+    # random graphs with vertices of no out-edges (changed vertices that
+    # push nothing) and of in/out-degree up to 64 (wave tier), unweighted
+    # CC included; the device-result entry points too
+    rng = np.random.default_rng(31)
+    graphs = [workloads.road(256, 256, seed=5), workloads.road(100, 300, seed=8)]
+    for trial in range(3):
+        n = int(rng.integers(9000, 30000))
+        m = int(rng.integers(n, 4 * n))
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        s[: m // 50] = 17  # one vertex of out-degree ~m/50 <= 64 ...
+        s = s[(s != 17) | (np.arange(m) < 60)]
+        d = d[: len(s)]
+        graphs.append(_edges(n, s, d, rng.integers(1, 64, len(s))))
+    for el in graphs:
+        s, d, w = el.arrays()
+        n = el.n_vertices
+        g = gas.Graph(el)
+        for src in (0, n // 3, n - 1):
+            ref, ost = oracle.sssp(n, s, d, w, src)
+            for push in (True, False):
+                g.set_push(gas.PROGRAM_SSSP, push)
+                dist, st = g.sssp(src)
+                assert np.array_equal(dist, ref), (n, src, push)
+                assert st.supersteps == ost.supersteps, (n, src, push)
+                buf = gas.DeviceBuffer(4 * n)
+                g.sssp_device(src, buf.ptr)
+                assert np.array_equal(buf.to_numpy(), ref), (n, src, push)
+        ref, ost = oracle.cc(n, s, d)
+        for push in (True, False):
+            g.set_push(gas.PROGRAM_CC, push)
+            lab, st = g.cc()
+            assert np.array_equal(lab, ref), (n, push)
+            assert st.supersteps == ost.supersteps, (n, push)
+    # the push kernel ran (kernel timing books its launches)
+    g = gas.Graph(workloads.road(128, 128, seed=1))
+    g.set_push(gas.PROGRAM_SSSP, True)
+    gas.set_kernel_timing(True)
+    try:
+        _, st = g.sssp(0)
+    finally:
+        gas.set_kernel_timing(False)
+    assert "k_push" in st.kernels() and "k_fused" not in st.kernels()
 
 
 def test_device_result_entry_points(gpu):

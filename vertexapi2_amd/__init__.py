@@ -49,7 +49,7 @@ class RunStats(C.Structure):
                 ("active_total", C.c_int64), ("ms", C.c_double),
                 ("gather_ms", C.c_double), ("scatter_ms", C.c_double),
                 ("gather_launches", C.c_int32), ("scatter_launches", C.c_int32),
-                ("kernel_ms", C.c_double * 12), ("kernel_launches", C.c_int32 * 12),
+                ("kernel_ms", C.c_double * 13), ("kernel_launches", C.c_int32 * 13),
                 ("returned_total", C.c_int64), ("hub_vertices_total", C.c_int64),
                 ("hub_edges_total", C.c_int64)]
 
@@ -67,7 +67,7 @@ class RunStats(C.Structure):
 
 KERNEL_NAMES = ["k_gather_vp", "k_apply", "k_scatter_vp", "k_hscan1", "k_hscan2",
                 "k_hgather", "k_hfix", "k_hscatter", "k_compact1", "k_compact2",
-                "k_fused", "k_small"]
+                "k_fused", "k_small", "k_push"]
 
 
 _lib = None
@@ -357,6 +357,11 @@ class Graph:
         p = C.c_int32(0)
         _check(lib().gas_graph_batch_param(self._h, C.c_int(kind), C.byref(p)))
         return p.value
+
+    def set_push(self, program: int, enable: bool) -> None:
+        """Push supersteps of whole SSSP / CC runs (same results and
+        superstep counts as the pull)."""
+        _check(lib().gas_graph_set_push(self._h, C.c_int(program), C.c_int32(1 if enable else 0)))
 
     def set_small_frontier(self, program: int, max_vertices: int) -> None:
         """Runs supersteps of list frontiers of at most `max_vertices` inside

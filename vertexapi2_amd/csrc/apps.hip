@@ -334,6 +334,12 @@ int gas_graph_set_small_frontier(gas_graph_t g, int program,
                      [&](auto &e) { e.setSmallFrontier(max_vertices); });
 }
 
+int gas_graph_set_push(gas_graph_t g, int program, int32_t enable) {
+  if (enable < 0 || enable > 1)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_push: enable must be 0 or 1");
+  return with_engine(g, program, [&](auto &e) { e.setPush(enable != 0); });
+}
+
 int gas_graph_get_tiers(gas_graph_t g, int program, int32_t *tiers) {
   if (!tiers) return set_error(GAS_ERR_ARG, "gas_graph_get_tiers: null");
   return with_engine(g, program, [&](auto &e) {

@@ -61,6 +61,10 @@ struct gas_graph {
   gas_graph *ordered = nullptr;
   std::vector<int32_t> new_of, old_of;
   int32_t *d_new_of = nullptr, *d_old_of = nullptr;
+  // out-neighbour table of graphs whose out-degrees are all <= kEllWidth
+  // (gas_graph_ell): built at first use, guarded by order_mu
+  int32_t ell_state = 0;  // 0 not looked at, 1 built, -1 degrees too large
+  int32_t *ell_dst = nullptr, *ell_w = nullptr;
 };
 
 namespace gas {
@@ -87,6 +91,16 @@ int device_build_layout(int32_t nV, int64_t nE, const int32_t *d_keys,
                         int32_t *d_out_w, uint32_t *d_sorted_keys, void *d_temp,
                         size_t temp_bytes, gasrt_stream_t stream);
 size_t device_build_temp_bytes(int64_t nE);
+
+// Width of the out-neighbour table (gas_graph_ell).
+constexpr int kEllWidth = 8;
+
+// Fills the out-neighbour table from the CSR on the device (graph_build.hip):
+// row v holds v's out-neighbours (and weights, if d_w) in CSR order, -1
+// (and 0) after the last; every out-degree must be <= kEllWidth.
+int device_build_ell(int32_t nV, const int32_t *d_off, const int32_t *d_dst,
+                     const int32_t *d_w, int32_t *ell_dst, int32_t *ell_w,
+                     gasrt_stream_t stream);
 
 // Stable counting sort of edges by key: offsets (V+1), values permuted into
 // key order, and the original edge index of each slot.  Equal keys keep

@@ -492,6 +492,9 @@ static void graph_release(gas_graph *g) {
   gasrt_free(g->d_new_of);
   gasrt_free(g->d_old_of);
   g->d_new_of = g->d_old_of = nullptr;
+  gasrt_free(g->ell_dst);
+  gasrt_free(g->ell_w);
+  g->ell_dst = g->ell_w = nullptr;
   gasrt_free(g->csc_off);
   gasrt_free(g->csc_src);
   gasrt_free(g->csr_off);
@@ -656,6 +659,47 @@ int gas_graph_create(gas_edges_t edges, gas_graph_t *out) {
       return rc;
     }
     *out = g;
+    return GAS_OK;
+  });
+}
+
+int gas_graph_ell(gas_graph_t g, const int32_t **ell_dst, const int32_t **ell_w) {
+  if (!g || !ell_dst || !ell_w) return set_error(GAS_ERR_ARG, "gas_graph_ell: null");
+  return gas::host_guard("gas_graph_ell", [&]() -> int {
+    std::lock_guard<std::mutex> lock(g->order_mu);
+    if (g->ell_state == 0) {
+      int32_t max_out = 0;
+      for (int32_t v = 0; v < g->nV; ++v)
+        max_out = std::max(max_out, g->h_csr_off[(size_t)v + 1] - g->h_csr_off[(size_t)v]);
+      g->ell_state = -1;
+      if (max_out <= gas::kEllWidth && g->nV > 0) {
+        const size_t b = sizeof(int32_t) * gas::kEllWidth * (size_t)g->nV;
+        int32_t *dst = nullptr, *w = nullptr;
+        gasrt_stream_t s = nullptr;
+        int rc = gasrt_malloc((void **)&dst, b);
+        if (!rc && g->csr_w) rc = gasrt_malloc((void **)&w, b);
+        if (!rc) rc = gasrt_stream_create(&s);
+        if (!rc)
+          rc = gas::device_build_ell(g->nV, g->csr_off, g->csr_dst, g->csr_w, dst, w, s);
+        if (s) {
+          const int rs = gasrt_stream_synchronize(s);
+          if (!rc) rc = rs;
+          gasrt_stream_destroy(s);
+        }
+        if (rc) {
+          gasrt_free(dst);
+          gasrt_free(w);
+          g->ell_state = 0;
+          return rc;
+        }
+        g->ell_dst = dst;
+        g->ell_w = w;
+        g->device_bytes += b * (w ? 2 : 1);
+        g->ell_state = 1;
+      }
+    }
+    *ell_dst = g->ell_dst;
+    *ell_w = g->ell_w;
     return GAS_OK;
   });
 }

@@ -199,6 +199,23 @@ __global__ __launch_bounds__(SORT_NT) void k_digit_scatter(
   }
 }
 
+// Row v of the out-neighbour table: its CSR out-neighbours (weights) then
+// -1 (0) up to kEllWidth.
+__global__ __launch_bounds__(kThreads) void k_build_ell(
+    int32_t nV, const int32_t *__restrict__ off, const int32_t *__restrict__ dst,
+    const int32_t *__restrict__ w, int32_t *__restrict__ ell_dst,
+    int32_t *__restrict__ ell_w) {
+  for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < nV;
+       v += (int64_t)gridDim.x * kThreads) {
+    const int32_t o = off[v], deg = off[v + 1] - o;
+#pragma unroll
+    for (int k = 0; k < kEllWidth; ++k) {
+      ell_dst[v * kEllWidth + k] = k < deg ? dst[o + k] : -1;
+      if (w) ell_w[v * kEllWidth + k] = k < deg ? w[o + k] : 0;
+    }
+  }
+}
+
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (n + kThreads - 1) / kThreads));
 }
@@ -231,6 +248,13 @@ SortScratch carve(void *temp, int64_t nE) {
 }
 
 }  // namespace
+
+int device_build_ell(int32_t nV, const int32_t *d_off, const int32_t *d_dst,
+                     const int32_t *d_w, int32_t *ell_dst, int32_t *ell_w,
+                     gasrt_stream_t stream) {
+  return launch(k_build_ell, grid_for(nV), kThreads, stream, nV, d_off, d_dst, d_w,
+                ell_dst, ell_w);
+}
 
 size_t device_build_temp_bytes(int64_t nE) {
   const size_t eb = align256((size_t)std::max<int64_t>(nE, 1) * 4);
