@@ -464,13 +464,32 @@ typedef struct {
   int32_t max_concurrent;       /* most queries running at once           */
   int32_t solo;                 /* BFS/SSSP queries answered by the plain  */
                                 /* engine alone (own engine idle)         */
+  int64_t evictions;            /* idle batch engines freed to make room  */
+  int64_t device_bytes;         /* device memory the server holds at end  */
 } gas_serve_stats;
 
-/* Per-query state bytes charged against the budget (lane state + result). */
+/* Device bytes of one query's lane state and result (an estimate of the
+ * per-query share; admission charges the actual allocations, see below). */
 size_t gas_server_query_bytes(int kind, int64_t n_vertices);
+/* memory_budget_bytes bounds the device memory the server itself holds:
+ * each batch engine's state for its 64 lanes (charged when the engine is
+ * built), each lane's result buffer (charged when first used) and the solo
+ * threads' result buffers.  A query is admitted only while what it needs
+ * fits, evicting batch engines with no running query (least recently used
+ * first) to make room; the graphs and their whole-run engines are outside
+ * the budget.  0: 80% of the device memory free at creation. */
 int gas_server_create(const gas_graph_t *graphs, int32_t n_graphs,
                       size_t memory_budget_bytes, int32_t ppr_iterations,
                       gas_server_t *out);
+/* The budget and the device bytes the server holds now. */
+int gas_server_budget(gas_server_t s, size_t *budget_bytes, size_t *held_bytes);
+/* A new budget (not while serving online); frees the batch engines, which
+ * are rebuilt as queries need them. */
+int gas_server_set_budget(gas_server_t s, size_t budget_bytes);
+/* State bytes of the batch engine of (kind, graph), measured when it was
+ * first built (0 before the first replay or start). */
+int gas_server_engine_bytes(gas_server_t s, int32_t kind, int32_t graph_id,
+                            size_t *bytes);
 /* Serves n queries arriving at arrival_s[i] seconds (sorted, relative to
  * the call) in real time.  latency_ms[i] receives arrival-to-result time;
  * host_results[i], if non-NULL, receives the V int32 (BFS, SSSP) or float
@@ -496,6 +515,28 @@ int gas_server_set_gang(gas_server_t s, int32_t backlog, int32_t rounds);
  * otherwise).  0, 0 = always batch.
  * Default 2 ms and 50 ms. */
 int gas_server_set_solo(gas_server_t s, double max_ms, double idle_max_ms);
+/* Online serving: gas_server_start runs the serving loop on a thread of the
+ * server's own (after measuring the engines on first use, as the first
+ * replay does); gas_server_submit hands it one query, arriving now, and
+ * returns a ticket; the result (V int32 for BFS / SSSP, V float for PPR)
+ * goes to host_result, if non-NULL, before the ticket completes.
+ * gas_server_poll reports a ticket's state (0 waiting or running, 1 done,
+ * -1 failed, with the error returned) and latency (arrival to result in
+ * HBM); gas_server_wait blocks until it is done or timeout_s passes
+ * (GAS_ERR_STATE then).  gas_server_stop stops taking queries, finishes
+ * those submitted, joins the thread and returns the session's statistics.
+ * Submission fails with GAS_ERR_FULL when the query can never fit the
+ * budget.  Replays are refused while the server runs online; the settings
+ * (gas_server_set_*) are read by the loop and are set before the start.
+ * Thread-safe: any host thread may submit, poll and wait. */
+int gas_server_start(gas_server_t s);
+int gas_server_submit(gas_server_t s, int32_t kind, int32_t graph_id,
+                      int32_t source, void *host_result, int64_t *ticket);
+int gas_server_poll(gas_server_t s, int64_t ticket, int32_t *state,
+                    double *latency_ms);
+int gas_server_wait(gas_server_t s, int64_t ticket, double timeout_s,
+                    double *latency_ms);
+int gas_server_stop(gas_server_t s, gas_serve_stats *stats);
 int gas_server_free(gas_server_t s);
 
 /* ---- the engine API, step by step --------------------------------------- */

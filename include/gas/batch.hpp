@@ -822,6 +822,7 @@ class BatchEngine {
       d.chg_next = m_chg_next.p;
     }
     m_lane_start.alloc(BW);
+    m_tmp.alloc(V);  // single-lane extraction (held from the start)
     check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
     d.lane_start = m_lane_start.p;
     d.sssp_delta = 0;
@@ -860,6 +861,14 @@ class BatchEngine {
   Kind kind() const { return m_kind; }
   gasrt_stream_t stream() const { return m_stream; }
   uint64_t lanesUsed() const { return m_used; }
+  // Device memory the engine holds (lane state for all 64 lanes, lists).
+  size_t deviceBytes() const {
+    return m_ctl.bytes() + m_item.bytes() + m_list.bytes() + m_bsum.bytes() +
+           m_lanes.bytes() + m_sources.bytes() + m_lane_start.bytes() + m_dist.bytes() +
+           m_tmp.bytes() + m_esum.bytes() + m_lor.bytes() + m_vn.bytes() +
+           m_front0.bytes() + m_front1.bytes() + m_chg.bytes() + m_chg_next.bytes() +
+           m_hub_mask.bytes() + m_hub.bytes() + m_depth.bytes();
+  }
   int round() const { return m_round; }
   // BFS rounds pull instead of push when the frontier's out-edges reach
   // E / div (0: never, the default).  With 64 lanes a pull cannot stop
@@ -1279,6 +1288,7 @@ class PprBatch {
     m_src.alloc(BW);
     m_lanes.alloc(BW);
     m_sources.alloc(BW);
+    m_tmp.alloc(V);  // single-lane extraction (held from the start)
     check(gasrt_memset(m_rank0.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_rank1.p, 0, V * BW * 4, m_stream), "memset");
     check(gasrt_memset(m_contrib0.p, 0, V * BW * 4, m_stream), "memset");
@@ -1320,6 +1330,16 @@ class PprBatch {
   }
 
   gasrt_stream_t stream() const { return m_stream; }
+  // Device memory the engine holds (lane state for all 64 lanes, items).
+  size_t deviceBytes() const {
+    size_t b = m_outdeg.bytes() + m_itv.bytes() + m_ite0.bytes() + m_pslot.bytes() +
+               m_vitem0.bytes() + m_mv.bytes() + m_src.bytes() + m_lanes.bytes() +
+               m_sources.bytes() + m_rank0.bytes() + m_rank1.bytes() +
+               m_contrib0.bytes() + m_contrib1.bytes() + m_ps.bytes() + m_pc.bytes() +
+               m_tmp.bytes();
+    for (const auto &z : m_nz) b += z.bytes();
+    return b;
+  }
   uint64_t lanesUsed() const { return m_used; }
   int iterations() const { return m_iters; }
 

@@ -139,6 +139,7 @@ struct DevBuf {
     n = 0;
     owned = false;
   }
+  size_t bytes() const { return p ? (n ? n : 1) * sizeof(T) : 0; }
   ~DevBuf() { release(); }
 };
 

@@ -614,7 +614,7 @@ class Batch:
 class ServeStats(C.Structure):
     _fields_ = [("elapsed_s", C.c_double), ("rounds", C.c_int64),
                 ("admissions_deferred", C.c_int64), ("max_concurrent", C.c_int32),
-                ("solo", C.c_int32)]
+                ("solo", C.c_int32), ("evictions", C.c_int64), ("device_bytes", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -651,6 +651,60 @@ class Server:
         """While `backlog` queries wait for a BFS/SSSP engine, they join its
         batch only within `rounds` rounds of its start (0: always join)."""
         _check(lib().gas_server_set_gang(self._h, C.c_int32(backlog), C.c_int32(rounds)))
+
+    def budget(self):
+        """(budget, device bytes the server holds now)."""
+        b, h = C.c_size_t(), C.c_size_t()
+        _check(lib().gas_server_budget(self._h, C.byref(b), C.byref(h)))
+        return b.value, h.value
+
+    def set_budget(self, nbytes: int) -> None:
+        _check(lib().gas_server_set_budget(self._h, C.c_size_t(nbytes)))
+
+    def engine_bytes(self, kind: int, graph_id: int) -> int:
+        b = C.c_size_t()
+        _check(lib().gas_server_engine_bytes(self._h, C.c_int32(kind), C.c_int32(graph_id),
+                                             C.byref(b)))
+        return b.value
+
+    # ---- online serving ---------------------------------------------------
+    def start(self) -> None:
+        """Runs the serving loop on the server's own thread (submit / poll /
+        wait / stop)."""
+        _check(lib().gas_server_start(self._h))
+
+    def submit(self, kind: int, graph_id: int, source: int, out: np.ndarray = None) -> int:
+        """Hands one query to the running server; returns its ticket.  `out`
+        (V int32 for BFS / SSSP, V float32 for PPR, kept alive by the
+        caller until the ticket is done) receives the result."""
+        if out is not None:
+            want = np.float32 if kind == BATCH_PPR else np.int32
+            n = self.graphs[graph_id].n_vertices
+            if out.dtype != want or out.size != n or not out.flags["C_CONTIGUOUS"]:
+                raise ValueError("result array must be contiguous, %d x %s" % (n, want))
+        t = C.c_int64()
+        _check(lib().gas_server_submit(
+            self._h, C.c_int32(kind), C.c_int32(graph_id), C.c_int32(source),
+            C.c_void_p(out.ctypes.data) if out is not None else None, C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int):
+        """(state, latency_ms): state 0 waiting / running, 1 done."""
+        st, lat = C.c_int32(), C.c_double()
+        _check(lib().gas_server_poll(self._h, C.c_int64(ticket), C.byref(st), C.byref(lat)))
+        return st.value, lat.value
+
+    def wait(self, ticket: int, timeout_s: float = 60.0) -> float:
+        """Blocks until the ticket is done; returns its latency in ms."""
+        lat = C.c_double()
+        _check(lib().gas_server_wait(self._h, C.c_int64(ticket), C.c_double(timeout_s),
+                                     C.byref(lat)))
+        return lat.value
+
+    def stop(self) -> "ServeStats":
+        st = ServeStats()
+        _check(lib().gas_server_stop(self._h, C.byref(st)))
+        return st
 
     def replay(self, kinds, graph_ids, sources, arrivals_s, keep=None):
         """Serves the stream in real time; returns (latency_ms, results, stats)
