@@ -478,9 +478,9 @@ def app_lines(steps):
     return out
 
 
-def batched_lines(g22, n_plain=8):
-    """64-source batches on R-MAT-22 (symmetrised, weighted) against the
-    plain engine on the same sources one at a time."""
+def batched_lines(g22, n_plain=8, name22="rmat22s"):
+    """64-source batches on a weighted graph (R-MAT-22 symmetrised, or the
+    road grid) against the plain engine on the same sources one at a time."""
     V = g22.n_vertices
     csr_off = g22.host_csr()[0]
     deg = np.diff(csr_off)
@@ -538,8 +538,8 @@ def batched_lines(g22, n_plain=8):
             "plain_ms_per_query": round(t_p * 1e3, 3), "plain_qps": round(1 / t_p, 1),
             "qps_vs_plain": round(t_p * 64 / t_b, 2),
             "roofline_8d": frac_entry(b8, t_b)}
-        ks = _profile("kstats_rmat22_batched.json")
-        pm = _profile("pmc_rmat22_batched.json")
+        ks = _profile(f"kstats_{name22}_batched.json")
+        pm = _profile(f"pmc_{name22}_batched.json")
         if ks is not None and pm is not None:
             kern = ks.get("dominant", {}).get(name)
             k = ks["kernels"].get(kern) if kern else None
@@ -553,7 +553,7 @@ def batched_lines(g22, n_plain=8):
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 5),
                     "traffic": tr["hbm_bytes_per_launch"], "avg_launch_us": k["avg_us"]}
-        out[f"rmat22s_{name}64"] = line
+        out[f"{name22}_{name}64"] = line
     return out
 
 
@@ -674,6 +674,8 @@ def main():
             log(f"batch tuning {tuned}")
             if "batched" in want:
                 line["batched"] = batched_lines(g22)
+                # high-diameter graph: 50-100 sparse rounds per batch
+                line["batched"].update(batched_lines(road, name22="road1024"))
                 log("batched done")
             if "served" in want:
                 line["served"] = served_lines([road, g22], [el, el22])

@@ -93,6 +93,12 @@ struct BatchDev {
                                         // (0: never, SSSP)
   int32_t sssp_delta;                   // SSSP distance bound step (0: none)
   int32_t round;                        // round of this launch
+  // Activity of 64-vertex slabs (SLAB): a slab whose flag is 0 holds no
+  // nonzero mask of that array, so the O(V) passes skip it.  sl_next: some
+  // vertex of the slab got bits in next (BFS) / chg_next (SSSP) this round;
+  // sl_cur[q]: front[q] (BFS) / chg (SSSP, q = 0) may be nonzero there.
+  uint8_t *sl_next;
+  uint8_t *sl_cur[2];
 };
 
 // SSSP: a lane relaxes from a vertex in a round only while its distance
@@ -197,7 +203,8 @@ __device__ inline int slab_vertex(int r) {
 
 // Pass 1 over contiguous chunks of vertices with mask array m: per chunk the
 // number of nonzero masks, their push items and out-edges, and the OR.
-static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m) {
+static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m,
+                                                       const uint8_t *slm) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
@@ -207,7 +214,7 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
   const int nr = slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
-    if (v >= d.nV) continue;
+    if (v >= d.nV || !slm[v / SLAB]) continue;
     const uint64_t x = m[v];
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
@@ -235,7 +242,8 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
 
 // Pass 2: ordered list of vertices with nonzero masks, item offsets and the
 // owner of every item; workgroup 0 publishes totals and the direction.
-static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m) {
+static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m,
+                                                      const uint8_t *slm) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
@@ -256,7 +264,10 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
   const int nr = pull ? 0 : slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
-    const uint64_t mv = v < d.nV ? m[v] : 0ull;
+    const bool act = v < d.nV && slm[v / SLAB];
+    // a round of the workgroup's slabs with none active adds nothing
+    if (!__syncthreads_or(act)) continue;
+    const uint64_t mv = act ? m[v] : 0ull;
     const bool on = mv != 0;
     int nit = 0, o = 0, o1 = 0;
     if (on) {
@@ -376,6 +387,7 @@ static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
   const uint64_t bit = 1ull << s;
   atomicOr((unsigned long long *)&d.vn[2 * (size_t)v], (unsigned long long)bit);
   atomicOr((unsigned long long *)&d.front[p][v], (unsigned long long)bit);
+  d.sl_cur[p][v / SLAB] = 1;
   d.depth16[(size_t)s * d.rowV + v] = 0;
 }
 
@@ -419,9 +431,11 @@ static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
       // bits already in next (possibly a stale read: then the atomic is
       // merely redundant) need no atomic
       const uint64_t nb = f & used & ~vn[g].x & ~vn[g].y;
-      if (nb)
+      if (nb) {
         atomicOr((unsigned long long *)&d.vn[2 * (size_t)w[g] + 1],
                  (unsigned long long)nb);
+        d.sl_next[w[g] / SLAB] = 1;
+      }
     }
   }
 }
@@ -461,7 +475,10 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
         for (int x = 0; x < 8; ++x) acc |= f[x];
         if ((acc & need) == need) break;
       }
-      if (acc & need) d.vn[2 * (size_t)v + 1] = acc & need;
+      if (acc & need) {
+        d.vn[2 * (size_t)v + 1] = acc & need;
+        d.sl_next[v / SLAB] = 1;
+      }
     }
     uint64_t m = __ballot(wide);
     while (m) {
@@ -479,7 +496,10 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
         acc |= wave_or(x);
         if ((acc & nj) == nj) break;
       }
-      if (lane == j && (acc & nj)) d.vn[2 * (size_t)v + 1] = acc & nj;
+      if (lane == j && (acc & nj)) {
+        d.vn[2 * (size_t)v + 1] = acc & nj;
+        d.sl_next[v / SLAB] = 1;
+      }
     }
   }
 }
@@ -497,13 +517,19 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
   if (threadIdx.x < BW) s_start[threadIdx.x] = d.lane_start[threadIdx.x];
   __syncthreads();
   uint64_t *nf = d.front[p ^ 1];
+  uint8_t *slf = d.sl_cur[p ^ 1];
   int cnt = 0, items = 0, hubs = 0;
   int64_t edges = 0;
   uint64_t any = 0;
   const int nr = slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
-    const bool valid = v < d.nV;
+    // a slab (one wave) with no next bits and no stale frontier is left as
+    // it is: both arrays are zero there
+    const int sl = v / SLAB;
+    const bool work = v < d.nV && (d.sl_next[sl] | slf[sl]);
+    if (!__any(work)) continue;
+    const bool valid = work;
     uint64_t nv = 0;
     if (valid) {
       const ulonglong2 vnv = reinterpret_cast<const ulonglong2 *>(d.vn)[v];
@@ -517,10 +543,15 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
     // depths of the newly visited lanes: a wave holds 64 consecutive
     // vertices of a slab and writes lane by lane, one contiguous run of
     // the lane's depth row per store
-    for (uint64_t wx = wave_or(nv); wx; wx &= wx - 1) {
+    const uint64_t wnv = wave_or(nv);
+    for (uint64_t wx = wnv; wx; wx &= wx - 1) {
       const int s = __ffsll((unsigned long long)wx) - 1;
       if ((nv >> s) & 1ull)
         d.depth16[(size_t)s * d.rowV + v] = (uint16_t)(round + 1 - s_start[s]);
+    }
+    if (lane_id() == 0 && v < d.nV) {
+      slf[sl] = wnv != 0;
+      d.sl_next[sl] = 0;
     }
     if (nv) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
@@ -576,6 +607,7 @@ static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
   const int s = lanes[i], v = sources[i];
   d.dist[(size_t)v * BW + s] = 0;
   atomicOr((unsigned long long *)&d.chg[v], (unsigned long long)(1ull << s));
+  d.sl_cur[0][v / SLAB] = 1;
 }
 
 // Push from the changed list: one wave per item, lane = query.  The item's
@@ -611,9 +643,11 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
       act[g] = changed && du[g] < bound;
       // changed lanes above the bound wait for a later round
       const uint64_t wait = __ballot(changed && !act[g]);
-      if (wait && lane == 0)
+      if (wait && lane == 0) {
         atomicOr((unsigned long long *)&d.chg_next[u[g]],
                  (unsigned long long)wait);
+        d.sl_next[u[g] / SLAB] = 1;
+      }
       my_w[g] = 0;
       my_wt[g] = 0;
       if (lane < ne[g]) {
@@ -647,9 +681,11 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
           const bool imp = act[g] && j0 + x < ne[g] && cand[g][x] < cur[g][x];
           if (imp) atomicMin(&d.dist[(size_t)w[g][x] * BW + lane], cand[g][x]);
           const uint64_t bm = __ballot(imp);
-          if (bm && lane == 0)
+          if (bm && lane == 0) {
             atomicOr((unsigned long long *)&d.chg_next[w[g][x]],
                      (unsigned long long)bm);
+            d.sl_next[w[g][x] / SLAB] = 1;
+          }
         }
     }
   }
@@ -666,10 +702,21 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
   const int nr = slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
-    if (v >= d.nV) continue;
-    const uint64_t x = d.chg_next[v];
-    if (x) d.chg_next[v] = 0;
-    d.chg[v] = x;
+    // slabs with no new changes and no stale ones are zero in both arrays
+    const int sl = v / SLAB;
+    const bool work = v < d.nV && (d.sl_next[sl] | d.sl_cur[0][sl]);
+    if (!__any(work)) continue;
+    uint64_t x = 0;
+    if (work) {
+      x = d.chg_next[v];
+      if (x) d.chg_next[v] = 0;
+      d.chg[v] = x;
+    }
+    const uint64_t wx = wave_or(x);
+    if (lane_id() == 0 && v < d.nV) {
+      d.sl_cur[0][sl] = wx != 0;
+      d.sl_next[sl] = 0;
+    }
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
@@ -824,6 +871,16 @@ class BatchEngine {
     m_lane_start.alloc(BW);
     m_tmp.alloc(V);  // single-lane extraction (held from the start)
     check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
+    {
+      const size_t nsl = (V + SLAB - 1) / SLAB;
+      for (auto *b : {&m_sl_next, &m_sl0, &m_sl1}) {
+        b->alloc(nsl);
+        check(gasrt_memset(b->p, 0, std::max<size_t>(nsl, 1), m_stream), "memset");
+      }
+      d.sl_next = m_sl_next.p;
+      d.sl_cur[0] = m_sl0.p;
+      d.sl_cur[1] = m_sl1.p;
+    }
     d.lane_start = m_lane_start.p;
     d.sssp_delta = 0;
     d.ctl = m_ctl.p;
@@ -867,7 +924,8 @@ class BatchEngine {
            m_lanes.bytes() + m_sources.bytes() + m_lane_start.bytes() + m_dist.bytes() +
            m_tmp.bytes() + m_esum.bytes() + m_lor.bytes() + m_vn.bytes() +
            m_front0.bytes() + m_front1.bytes() + m_chg.bytes() + m_chg_next.bytes() +
-           m_hub_mask.bytes() + m_hub.bytes() + m_depth.bytes();
+           m_hub_mask.bytes() + m_hub.bytes() + m_depth.bytes() + m_sl_next.bytes() +
+           m_sl0.bytes() + m_sl1.bytes();
   }
   int round() const { return m_round; }
   // BFS rounds pull instead of push when the frontier's out-edges reach
@@ -913,18 +971,18 @@ class BatchEngine {
       launch(kb_bfs_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.front[m_par]);
+             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.front[m_par]);
+             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       launch(kb_sssp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, reset);
       launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n);
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.chg);
+             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.chg);
+             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     }
   }
@@ -945,6 +1003,8 @@ class BatchEngine {
       };
       if (too_deep(m_used & m_alive_hint) && too_deep(lanesAlive()))
         throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
+    }
+    if (m_kind == BFS_KIND) {
       launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       if (m_dev.pull_div > 0)
         launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
@@ -952,14 +1012,14 @@ class BatchEngine {
              m_round);
       m_par ^= 1;
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.front[m_par]);
+             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
       m_dev.round = m_round;
       launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.chg);
+             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     }
     ++m_round;
@@ -1080,6 +1140,7 @@ class BatchEngine {
       m_hub_mask;
   DevBuf<int4> m_hub;
   DevBuf<uint16_t> m_depth;
+  DevBuf<uint8_t> m_sl_next, m_sl0, m_sl1;  // slab activity flags
   bool m_may_hub = true;  // some vertex has more than HUB_ITEMS items
 };
 
