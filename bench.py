@@ -201,6 +201,17 @@ def roofline_table(stats_list, models_list):
     return dom, table
 
 
+# Timed regions, cut out of rocprofv3 kernel traces by k_trace_mark
+# dispatches of grid 64 * (2 * id - 1) (start) and 64 * (2 * id) (end).
+REGIONS = {"road1024": 1, "grid64_bfs": 2, "rmat21_pagerank20": 3, "rmat21s_cc": 4,
+           "rmat22s_batched_bfs": 5, "rmat22s_batched_sssp": 6,
+           "road1024_batched_bfs": 7, "road1024_batched_sssp": 8, "copy_calibration": 9}
+
+
+def mark(region, end=False):
+    gas.trace_mark(2 * REGIONS[region] - (0 if end else 1))
+
+
 def _profile(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -269,11 +280,13 @@ def headline(g, steps, warmup):
         g.bfs_device(0, dbuf.ptr)
         g.sssp_device(0, sbuf.ptr)
     # timed region
+    mark("road1024")
     t0 = time.perf_counter()
     for _ in range(steps):
         sb0 = g.bfs_device(0, dbuf.ptr)
         ss0 = g.sssp_device(0, sbuf.ptr)
     total = time.perf_counter() - t0
+    mark("road1024", end=True)
     # results on the host (the reference's Took window)
     g.bfs(0)
     g.sssp(0)
@@ -416,7 +429,9 @@ def app_lines(steps):
     buf = gas.DeviceBuffer(4 * V)
     g.autotune(gas.PROGRAM_BFS)
     g.bfs_device(0, buf.ptr)
+    mark("grid64_bfs")
     t, _ = timed_runs(lambda: g.bfs_device(0, buf.ptr), reps)
+    mark("grid64_bfs", end=True)
     gas.set_kernel_timing(True)
     st = g.bfs_device(0, buf.ptr)
     gas.set_kernel_timing(False)
@@ -436,7 +451,9 @@ def app_lines(steps):
     _, res, _ = g.pagerank(20)  # host-result run: the per-sweep residuals
     rbuf = gas.DeviceBuffer(4 * V)
     g.pagerank_device(rbuf.ptr, 20)
+    mark("rmat21_pagerank20")
     t, st0 = timed_runs(lambda: g.pagerank_device(rbuf.ptr, 20), reps)
+    mark("rmat21_pagerank20", end=True)
     gas.set_kernel_timing(True)
     st = g.pagerank_device(rbuf.ptr, 20)
     gas.set_kernel_timing(False)
@@ -459,7 +476,9 @@ def app_lines(steps):
     lab, _ = g.cc()
     lbuf = gas.DeviceBuffer(4 * V)
     g.cc_device(lbuf.ptr)
+    mark("rmat21s_cc")
     t, st0 = timed_runs(lambda: g.cc_device(lbuf.ptr), reps)
+    mark("rmat21s_cc", end=True)
     assert np.array_equal(lbuf.to_numpy(), lab)
     gas.set_kernel_timing(True)
     st = g.cc_device(lbuf.ptr)
@@ -496,6 +515,7 @@ def batched_lines(g22, n_plain=8, name22="rmat22s"):
         b.run()
         b.release((1 << 64) - 1)
         ts = []
+        mark(f"{name22}_batched_{name}")
         for _ in range(3):
             b.add(lanes, srcs)
             t = time.perf_counter()
@@ -503,6 +523,7 @@ def batched_lines(g22, n_plain=8, name22="rmat22s"):
             ts.append(time.perf_counter() - t)
             if _ < 2:
                 b.release((1 << 64) - 1)
+        mark(f"{name22}_batched_{name}", end=True)
         t_b = float(np.median(ts))
         er = 0
         any_lane = np.zeros(V, bool)
@@ -538,10 +559,10 @@ def batched_lines(g22, n_plain=8, name22="rmat22s"):
             "plain_ms_per_query": round(t_p * 1e3, 3), "plain_qps": round(1 / t_p, 1),
             "qps_vs_plain": round(t_p * 64 / t_b, 2),
             "roofline_8d": frac_entry(b8, t_b)}
-        ks = _profile(f"kstats_{name22}_batched.json")
-        pm = _profile(f"pmc_{name22}_batched.json")
+        ks = _profile(f"kstats_{name22}_batched_{name}.json")
+        pm = _profile(f"pmc_{name22}_batched_{name}.json")
         if ks is not None and pm is not None:
-            kern = ks.get("dominant", {}).get(name)
+            kern = ks.get("dominant")
             k = ks["kernels"].get(kern) if kern else None
             tr = pm.get("kernels", {}).get(kern) if kern else None
             if k and tr:
@@ -691,7 +712,9 @@ def main():
     if args.copy_calibration:
         # one device-to-device copy of 256 MiB (the runtime's copy kernel)
         a, b = gas.DeviceBuffer(256 << 20), gas.DeviceBuffer(256 << 20)
+        mark("copy_calibration")
         gas.device_copy(b, a)
+        mark("copy_calibration", end=True)
         log("copy calibration: 268435456 bytes read, 268435456 written")
 
 

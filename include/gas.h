@@ -281,6 +281,12 @@ typedef struct {
  * per-kernel roofline figures). */
 int gas_set_kernel_timing(int enable);
 
+/* Launches an empty kernel (k_trace_mark) of `tag` workgroups of 64 lanes
+ * on the default stream and waits for it, so that a kernel trace taken by a
+ * profiler can be cut at this point: the mark's grid size is 64 * tag.
+ * tag in 1..65535.  No counterpart in the reference. */
+int gas_trace_mark(uint32_t tag);
+
 /* Degree tiers of the engine the whole-run entry points keep for (graph,
  * program): tiers[0..1] for the gather, tiers[2..3] for the scatter.  A
  * vertex of degree d is walked by one thread if d <= t1, by one wave64 if

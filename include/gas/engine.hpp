@@ -185,9 +185,10 @@ class GASEngineGPU {
     check(gasrt_host_alloc((void **)&m_hctl, sizeof(Ctl)), "gasrt_host_alloc");
     void *dev_status = nullptr;
     check(gasrt_host_alloc_mapped((void **)&m_status, &dev_status,
-                                  sizeof(int64_t) * STATUS_SLOTS),
+                                  sizeof(int64_t) * (STATUS_SLOTS + STAT_SLOTS)),
           "gasrt_host_alloc_mapped");
     m_dev.status = static_cast<int64_t *>(dev_status);
+    m_dev_totals = m_dev.status + STATUS_SLOTS;
   }
 
   ~GASEngineGPU() {
@@ -197,6 +198,8 @@ class GASEngineGPU {
       gasrt_event_destroy(pr.first);
       gasrt_event_destroy(pr.second);
     }
+    for (gasrt_event_t ev : m_run_ev)
+      if (ev) gasrt_event_destroy(ev);
     gasrt_host_free(m_hctl);
     gasrt_host_free((void *)m_status);
     if (m_own_graph) gas_graph_free(m_own_graph);
@@ -272,9 +275,12 @@ class GASEngineGPU {
     m_vd_host = nullptr;
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)m_nV + NT - 1) / NT));
+    // a min-program's run may push from its first superstep: both state
+    // buffers start equal
+    VertexData *also = is_push_min<Program>::value ? m_dev.vnext : nullptr;
     launch(k_fill_vdata<Program>, grid, m_stream, m_dev, value, (int32_t)at,
-           one);
-    m_other_stale = true;
+           one, also);
+    m_other_stale = also == nullptr;
     m_srcval_stale = true;
   }
 
@@ -300,6 +306,7 @@ class GASEngineGPU {
     m_in = (vertexStart == 0 && vertexEnd == m_nV) ? IN_ALL : IN_LIST;
     m_prev_in = IN_LIST;  // nothing of an earlier superstep left to report
     m_first = true;
+    m_host_active = (int64_t)vertexEnd - vertexStart;
   }
 
   Int countActive() {
@@ -402,7 +409,9 @@ class GASEngineGPU {
   // m_small_max vertices is handed to k_small, which runs supersteps in one
   // workgroup until the frontier outgrows it or empties.
   void runUntilEmpty(bool haveGather, bool /*haveScatter*/) {
-    int64_t known = countActive();  // exact size of a list / all input
+    // exact size of a list / all input: known on the host right after
+    // setActive, read back otherwise
+    int64_t known = m_host_active >= 0 ? m_host_active : countActive();
     if (known == 0) return;
     // push supersteps for the whole run, or none: the push frontier is the
     // set of vertices that changed, so the run must start from every vertex
@@ -488,7 +497,8 @@ class GASEngineGPU {
         __builtin_ia32_pause();
       }
     }
-    check(gasrt_stream_synchronize(m_stream), "sync");
+    // the supersteps enqueued past the empty one find empty frontiers and
+    // change nothing; later work on the stream runs after them
     const int64_t real = empty_at - s0;  // supersteps with a non-empty frontier
     m_supersteps -= (m_seq - s0) - real;
     m_iter = it0 + (int)real;
@@ -635,15 +645,32 @@ class GASEngineGPU {
     }
   }
 
+  // Enqueues the sum of the run counters into host-mapped memory, read by
+  // the next stats() (enqueue it before the caller's own wait, and stats()
+  // needs no copy of its own).
+  void enqueueStatTotals() {
+    if (m_ext) return;
+    launch(k_stat_totals<Program>, 1, m_stream, m_dev, m_dev_totals);
+    m_totals_enqueued = true;
+  }
+
   void stats(gas_run_stats *st) {
     collectTiming();
-    std::vector<int64_t> acc((size_t)STAT_SLOTS * SCAN_BLOCKS);
-    check(gasrt_memcpy_d2h(acc.data(), m_stat.p, sizeof(int64_t) * acc.size(),
-                           m_stream),
-          "stats");
     int64_t tot[STAT_SLOTS] = {};
-    for (int k = 0; k < STAT_SLOTS; ++k)
-      for (int b = 0; b < SCAN_BLOCKS; ++b) tot[k] += acc[(size_t)k * SCAN_BLOCKS + b];
+    if (!m_ext) {
+      if (!m_totals_enqueued) enqueueStatTotals();
+      check(gasrt_stream_synchronize(m_stream), "sync");
+      m_totals_enqueued = false;
+      for (int k = 0; k < STAT_SLOTS; ++k)
+        tot[k] = __atomic_load_n(m_status + STATUS_SLOTS + k, __ATOMIC_ACQUIRE);
+    } else {
+      std::vector<int64_t> acc((size_t)STAT_SLOTS * SCAN_BLOCKS);
+      check(gasrt_memcpy_d2h(acc.data(), m_stat.p, sizeof(int64_t) * acc.size(),
+                             m_stream),
+            "stats");
+      for (int k = 0; k < STAT_SLOTS; ++k)
+        for (int b = 0; b < SCAN_BLOCKS; ++b) tot[k] += acc[(size_t)k * SCAN_BLOCKS + b];
+    }
     st->supersteps = (int32_t)m_supersteps;
     st->reserved = 0;
     st->gathered_edges = tot[0];
@@ -664,6 +691,14 @@ class GASEngineGPU {
   }
 
   gasrt_stream_t stream() const { return m_stream; }
+
+  // Two events kept for timing whole runs on the engine's stream.
+  gasrt_event_t *runEvents() {
+    if (m_ext) throw Error(GAS_ERR_STATE, "runEvents: engine without host resources");
+    for (gasrt_event_t &ev : m_run_ev)
+      if (!ev) check(gasrt_event_create(&ev), "event");
+    return m_run_ev;
+  }
   Int numVertices() const { return m_nV; }
   Int numEdges() const { return m_nE; }
   int maxInDegree() const { return m_max_in; }
@@ -705,6 +740,7 @@ class GASEngineGPU {
   // Supersteps of k_small from the current list frontier, then the host's
   // bookkeeping of the supersteps it ran (synchronises the stream).
   void smallSteps(bool haveGather) {
+    m_host_active = -1;
     refreshSourceValues();
     m_dev.seq = (int32_t)m_seq;
     m_dev.iter = m_iter;
@@ -764,6 +800,7 @@ class GASEngineGPU {
     ++m_iter;
     ++m_supersteps;
     m_first = true;
+    m_host_active = -1;
   }
 
   // What the first kernel of the current superstep reports.
@@ -1048,7 +1085,12 @@ class GASEngineGPU {
   ExternalMemory *m_ext = nullptr;   // &m_ext_mem on an engine without its own memory
   gasrt_stream_t m_stream = nullptr;
   Ctl *m_hctl = nullptr;            // pinned copy of the control block
-  int64_t *m_status = nullptr;      // mapped status ring (STATUS_SLOTS)
+  int64_t *m_status = nullptr;      // mapped status ring (STATUS_SLOTS), then
+                                    // the run totals (STAT_SLOTS)
+  int64_t *m_dev_totals = nullptr;  // device address of the run totals
+  bool m_totals_enqueued = false;   // k_stat_totals enqueued since stats()
+  gasrt_event_t m_run_ev[2] = {nullptr, nullptr};  // whole-run timer
+  int64_t m_host_active = -1;       // frontier size set by setActive (-1: unknown)
   int64_t m_seq = 0;                // superstep number
   int m_iter = 0;                   // program-visible superstep counter
   int64_t m_supersteps = 0;         // supersteps with work since resetStats

@@ -1875,12 +1875,40 @@ __global__ void k_zero_words(W *p, int64_t words) {
 // Vertex-data initialisation on the device: `value` everywhere, `one` at
 // vertex `at` (none if at < 0).  Replaces an upload of the whole array when
 // the initial state is uniform, as it is for BFS and SSSP.
+// `also` (if not null) gets the same values: the other state buffer of a
+// double-buffered run, which then needs no copy before its first superstep.
 template <class P>
 __global__ void k_fill_vdata(Dev<P> d, typename P::VertexData value,
-                             int32_t at, typename P::VertexData one) {
+                             int32_t at, typename P::VertexData one,
+                             typename P::VertexData *also) {
   for (int64_t v = blockIdx.x * (int64_t)NT + threadIdx.x; v < d.nV;
-       v += (int64_t)gridDim.x * NT)
-    d.vdata[v] = v == at ? one : value;
+       v += (int64_t)gridDim.x * NT) {
+    const typename P::VertexData x = v == at ? one : value;
+    d.vdata[v] = x;
+    if (also) also[v] = x;
+  }
+}
+
+// Run totals: slot k of the per-workgroup counters summed over the
+// workgroups into out[k] (host-mapped memory, system-scope stores), one
+// workgroup in a fixed order.
+template <class P>
+__global__ __launch_bounds__(NT) void k_stat_totals(Dev<P> d, int64_t *out) {
+  __shared__ int64_t lds[NT / WAVE];
+  for (int k = 0; k < STAT_SLOTS; ++k) {
+    int64_t s = 0;
+    for (int b = threadIdx.x; b < SCAN_BLOCKS; b += NT)
+      s += d.stat_acc[k * SCAN_BLOCKS + b];
+    for (int x = WAVE / 2; x > 0; x >>= 1) s += __shfl_xor(s, x, WAVE);
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int w = 0; w < NT / WAVE; ++w) t += lds[w];
+      __hip_atomic_store(out + k, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace gas

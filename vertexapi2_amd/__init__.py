@@ -118,6 +118,12 @@ def set_kernel_timing(enable: bool) -> None:
     _check(lib().gas_set_kernel_timing(1 if enable else 0))
 
 
+def trace_mark(tag: int) -> None:
+    """Empty kernel of `tag` wave64 workgroups on the default stream, waited
+    for: a cut point in profiler kernel traces (grid size 64 * tag)."""
+    _check(lib().gas_trace_mark(C.c_uint32(tag)))
+
+
 def set_device_build(min_edges: int) -> None:
     """Graphs of at least `min_edges` edges get their CSC / CSR laid out on
     the GPU (default 2**20; 0: every graph)."""

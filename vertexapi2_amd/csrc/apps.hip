@@ -37,6 +37,10 @@ __global__ __launch_bounds__(NT) void k_residual_sum(const double *rpart,
   if (threadIdx.x == 0) hist[slot] = s;
 }
 
+// Empty kernel whose grid size carries a tag into profiler traces
+// (gas_trace_mark).
+__global__ void k_trace_mark() {}
+
 // Type-erased engine for the C ABI.
 struct AnyEngine {
   virtual ~AnyEngine() {}
@@ -106,12 +110,16 @@ static GASEngineGPU<P> &cached_engine(gas_graph_t g, int slot,
 }
 
 // Times a whole run on the engine's stream with HIP events.
+// Device time of a whole run on the engine's stream, with the engine's two
+// timing events; stop() waits for the run.
 struct RunTimer {
   gasrt_event_t a = nullptr, b = nullptr;
   gasrt_stream_t s;
-  explicit RunTimer(gasrt_stream_t st) : s(st) {
-    check(gasrt_event_create(&a), "event");
-    check(gasrt_event_create(&b), "event");
+  template <class E>
+  explicit RunTimer(E &e) : s(e.stream()) {
+    gasrt_event_t *ev = e.runEvents();
+    a = ev[0];
+    b = ev[1];
     check(gasrt_event_record(a, s), "event");
   }
   double stop() {
@@ -120,10 +128,6 @@ struct RunTimer {
     float ms = 0;
     check(gasrt_event_elapsed_ms(a, b, &ms), "elapsed");
     return ms;
-  }
-  ~RunTimer() {
-    gasrt_event_destroy(a);
-    gasrt_event_destroy(b);
   }
 };
 
@@ -256,6 +260,15 @@ int gas_set_kernel_timing(int enable) {
   return GAS_OK;
 }
 
+int gas_trace_mark(uint32_t tag) {
+  if (tag == 0 || tag > 65535) return set_error(GAS_ERR_ARG, "trace tag %u outside 1..65535", tag);
+  return guarded([&] {
+    launch_wg(k_trace_mark, tag, WAVE, nullptr);
+    check(gasrt_stream_synchronize(nullptr), "sync");
+    return GAS_OK;
+  });
+}
+
 }  // extern "C"
 
 // Runs f on the cached engine of (graph, program), creating it with
@@ -371,10 +384,11 @@ int gas_bfs(gas_graph_t g, int32_t source, int32_t *depth_out,
     for (auto &x : vd) x.depth = -1;
     auto &e = cached_engine<BFS>(og, GAS_PROGRAM_BFS, vd.data());
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.setActive(src, src + 1);
     e.setIteration(0);
     e.runUntilEmpty(false, false);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v)
@@ -395,13 +409,14 @@ int gas_bfs_device(gas_graph_t g, int32_t source, int32_t *depth_dev,
     const int32_t src = og == g ? source : g->new_of[source];
     auto &e = cached_engine<BFS>(og, GAS_PROGRAM_BFS, nullptr);
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.fillVertexData(BFS::VertexData{-1}, -1, BFS::VertexData{-1});
     e.setActive(src, src + 1);
     e.setIteration(0);
     e.runUntilEmpty(false, false);
     result_to_caller(g, og, reinterpret_cast<const int32_t *>(e.deviceVertexData()),
                      depth_dev, e.stream());
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -422,10 +437,11 @@ int gas_sssp(gas_graph_t g, int32_t source, int32_t *dist_out,
     vd[og == g ? source : g->new_of[source]] = 0;
     auto &e = cached_engine<SSSP>(og, GAS_PROGRAM_SSSP, vd.data());
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.setActive(0, g->nV);
     e.setIteration(0);
     e.runUntilEmpty(true, true);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v) dist_out[v] = vd[og == g ? v : g->new_of[v]];
@@ -446,12 +462,13 @@ int gas_sssp_device(gas_graph_t g, int32_t source, int32_t *dist_dev,
     gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
     auto &e = cached_engine<SSSP>(og, GAS_PROGRAM_SSSP, nullptr);
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.fillVertexData(SSSP::gatherZero, og == g ? source : g->new_of[source], 0);
     e.setActive(0, g->nV);
     e.setIteration(0);
     e.runUntilEmpty(true, true);
     result_to_caller(g, og, e.deviceVertexData(), dist_dev, e.stream());
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -469,10 +486,11 @@ int gas_cc(gas_graph_t g, int32_t *label_out, gas_run_stats *stats) {
     for (int32_t v = 0; v < cg->nV; ++v) vd[v] = cg == g ? v : g->old_of[v];
     auto &e = cached_engine<CC>(cg, GAS_PROGRAM_CC, vd.data());
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.setActive(0, g->nV);
     e.setIteration(0);
     e.runUntilEmpty(true, true);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v) label_out[v] = vd[cg == g ? v : g->new_of[v]];
@@ -498,7 +516,7 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
     e.resetStats();
     DevBuf<double> hist;
     hist.alloc(iterations > 0 ? (size_t)iterations : 1);
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     if (iterations > 0) {
       // every vertex stays active: one setActive serves all sweeps (a
       // gatherApply without a scatter leaves the input as it was)
@@ -512,6 +530,7 @@ int gas_pagerank(gas_graph_t g, int32_t iterations, float *rank_out,
       e.setIteration(0);
       e.runUntilEmpty(true, true);
     }
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v)
@@ -539,7 +558,7 @@ int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats) {
     const bool same = cg == g;
     auto &e = cached_engine<CC>(cg, GAS_PROGRAM_CC, nullptr);
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_cc_init, grid, e.stream(), e.dev(),
@@ -550,6 +569,7 @@ int gas_cc_device(gas_graph_t g, int32_t *label_dev, gas_run_stats *stats) {
     e.runUntilEmpty(true, true);
     launch(k_cc_label_out, grid, e.stream(), e.dev(),
            same ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, label_dev);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -567,7 +587,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
     auto &e = cached_engine<PageRank>(pg, GAS_PROGRAM_PAGERANK, nullptr);
     e.resetStats();
     double *res = residual_dev;  // residual sums only when asked for
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_pr_init, grid, e.stream(), e.dev());
@@ -580,6 +600,7 @@ int gas_pagerank_device(gas_graph_t g, int32_t iterations, float *rank_dev,
     launch(k_pr_rank_out, grid, e.stream(), e.dev(),
            pg == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of,
            rank_dev);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
@@ -655,9 +676,10 @@ int gas_ppr(gas_graph_t g, int32_t source, int32_t iterations,
     }
     auto &e = cached_engine<PPR>(og, 4, vd.data());
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     e.setActive(0, g->nV);  // all sweeps: see gas_pagerank
     for (int it = 0; it < iterations; ++it) e.gatherApply(true);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     e.getResults();
     for (int32_t v = 0; v < g->nV; ++v) rank_out[v] = vd[og == g ? v : g->new_of[v]].rank;
@@ -679,7 +701,7 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
     gas_graph_t og = ordered_graph(g);  // g or its degree-ordered copy
     auto &e = cached_engine<PPR>(og, 4, nullptr);
     e.resetStats();
-    RunTimer tm(e.stream());
+    RunTimer tm(e);
     const unsigned grid = (unsigned)std::min<int64_t>(
         2048, std::max<int64_t>(1, ((int64_t)g->nV + NT - 1) / NT));
     launch(k_ppr_init, grid, e.stream(), e.dev(), og == g ? source : g->new_of[source]);
@@ -688,6 +710,7 @@ int gas_ppr_device(gas_graph_t g, int32_t source, int32_t iterations,
     for (int it = 0; it < iterations; ++it) e.gatherApply(true);
     launch(k_ppr_rank_out, grid, e.stream(), e.dev(),
            og == g ? (const int32_t *)nullptr : (const int32_t *)g->d_new_of, rank_dev);
+    if (stats) e.enqueueStatTotals();
     double ms = tm.stop();
     if (stats) {
       e.stats(stats);
