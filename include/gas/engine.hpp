@@ -720,6 +720,13 @@ class GASEngineGPU {
         klaunch(GAS_K_HSCAN1, 0, k_hscan1<Program, 0>, SCAN_BLOCKS);
         klaunch(GAS_K_HSCAN2, 0, k_hscan2<Program, 0>, SCAN_BLOCKS);
       }
+      if constexpr (is_source_only<Program>::value && is_compensated<Program>::value) {
+        if (m_in == IN_ALL && m_dev.span_first) {
+          klaunch(GAS_K_HGATHER, 0, k_hgather<Program, true>, MP_BLOCKS);
+          klaunch(GAS_K_HFIX, 0, k_hfix<Program, true>, SCAN_BLOCKS);
+          return;
+        }
+      }
       klaunch(GAS_K_HGATHER, 0, k_hgather<Program>, MP_BLOCKS);
       klaunch(GAS_K_HFIX, 0, k_hfix<Program>, SCAN_BLOCKS);
     }
@@ -836,6 +843,34 @@ class GASEngineGPU {
     m_dev.ahpart = m_ahpart.p;
     m_dev.n_ahub = (int32_t)hubs.size();
     m_dev.ahub_items = hoff.back();
+    buildSpan(hubs);
+  }
+
+  // The all-active hubs as one span of vertices (see hgather_span), when
+  // they are consecutive ids and the program can use it: the vertex holding
+  // the first edge of each SPAN_TE-edge tile of their in-edges.
+  void buildSpan(const std::vector<int32_t> &hubs) {
+    m_dev.span_first = nullptr;
+    if constexpr (is_source_only<Program>::value && is_compensated<Program>::value) {
+      if (hubs.empty() || hubs.back() - hubs.front() + 1 != (int32_t)hubs.size()) return;
+      const int32_t lo = hubs.front(), hi = hubs.back() + 1;
+      const int32_t e0 = m_h_csc_off[lo], e1 = m_h_csc_off[hi];
+      if ((int64_t)e1 + SPAN_TE >= INT32_MAX) return;
+      const int32_t ntiles = (e1 - e0 + SPAN_TE - 1) / SPAN_TE;
+      std::vector<int32_t> first((size_t)ntiles);
+      int32_t v = lo;
+      for (int32_t t = 0; t < ntiles; ++t) {
+        const int32_t e = e0 + t * SPAN_TE;
+        while (m_h_csc_off[v + 1] <= e) ++v;
+        first[(size_t)t] = v;
+      }
+      m_span.upload(first, m_stream, m_ext);
+      m_dev.span_first = m_span.p;
+      m_dev.span_lo = lo;
+      m_dev.span_hi = hi;
+      m_dev.span_e0 = e0;
+      m_dev.span_e1 = e1;
+    }
   }
 
   void bindGraph(gas_graph_t g, VertexData *vertexData,
@@ -1121,7 +1156,7 @@ class GASEngineGPU {
   DevBuf<EdgeData> m_edata;
   DevBuf<Ctl> m_ctl;
   DevBuf<int32_t> m_q0, m_q1, m_ghub, m_ghoff, m_ghpart, m_shub, m_shoff,
-      m_shpart, m_bsum, m_bcnt, m_ahub, m_ahoff, m_ahpart;
+      m_shpart, m_bsum, m_bcnt, m_ahub, m_ahoff, m_ahpart, m_span;
   DevBuf<int64_t> m_stat;
   DevBuf<uint8_t> m_fl0, m_fl1, m_chg0, m_chg1, m_rflag;
   DevBuf<double> m_rpart;

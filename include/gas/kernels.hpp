@@ -70,6 +70,11 @@ constexpr int MP_BLOCKS = 4096;    // grid of the merge-path kernels
 constexpr int HUB_MIN_T2 = 4;      // hubs have degree > t2 >= 4 ...
 // ... so a segment spans >= 6 merge-path items and a tile touches at most
 constexpr int HUB_SEG_MAX = MP_NV / (HUB_MIN_T2 + 2) + 2;  // segments
+constexpr int SPAN_VT = 8;                // span hub gather: edges per thread
+constexpr int SPAN_TE = NT * SPAN_VT;     // ... per tile
+// offsets a span tile needs: the segments it overlaps (one holds its first
+// edge, at most (SPAN_TE - 1) / (HUB_MIN_T2 + 1) more start in it) + 1
+constexpr int SPAN_SEG = (SPAN_TE - 1) / (HUB_MIN_T2 + 1) + 3;
 constexpr int T1_MAX = 16;         // largest thread tier (unrolled loads)
 constexpr int MID_UNROLL = 8;      // wave tier: edges per lane per trip
 constexpr int DEFAULT_T1 = 8;      // thread tier: degree <= t1
@@ -242,18 +247,20 @@ struct Acc {
 };
 
 // Compensated float sum: (s, c) with TwoSum error terms (Knuth, TAOCP 2,
-// 4.2.2; Neumaier's variant for merging two partial sums).
+// 4.2.2; Neumaier's variant for merging two partial sums).  No "holds a
+// value" flag: the empty sum is (0, 0), and TwoSum(s, 0) = (s, 0) and
+// TwoSum(0, x) = (x, 0) exactly, so adding or merging an empty partial
+// changes nothing, and finishing it gives gatherZero (0).
 template <class P>
 struct Acc<P, true> {
   typedef typename P::GatherResult T;
   static_assert(std::is_same<T, float>::value, "compensated sum needs float");
+  static_assert(P::gatherZero == 0.0f, "compensated sum starts from 0");
   float s, c;
-  int ok;
   __device__ static Acc empty() {
     Acc a;
     a.s = 0.f;
     a.c = 0.f;
-    a.ok = 0;
     return a;
   }
   __device__ static float two_sum(float a, float b, float &err) {
@@ -263,22 +270,11 @@ struct Acc<P, true> {
     return t;
   }
   __device__ void add(float x) {
-    if (!ok) {
-      s = x;
-      c = 0.f;
-      ok = 1;
-      return;
-    }
     float err;
     s = two_sum(s, x, err);
     c += err;
   }
   __device__ void merge(const Acc &b) {
-    if (!b.ok) return;
-    if (!ok) {
-      *this = b;
-      return;
-    }
     float err;
     s = two_sum(s, b.s, err);
     c = c + b.c + err;
@@ -287,20 +283,15 @@ struct Acc<P, true> {
     Acc r;
     r.s = __shfl_xor(s, m, WAVE);
     r.c = __shfl_xor(c, m, WAVE);
-    r.ok = __shfl_xor(ok, m, WAVE);
     return r;
   }
   __device__ Acc up_peer(int o) const {
     Acc r;
     r.s = __shfl_up(s, o, WAVE);
     r.c = __shfl_up(c, o, WAVE);
-    r.ok = __shfl_up(ok, o, WAVE);
     return r;
   }
-  __device__ T finish() const {
-    T z = P::gatherZero;
-    return ok ? P::gatherReduce(z, s + c) : z;
-  }
+  __device__ T finish() const { return s + c; }
 };
 
 template <class P>
@@ -389,6 +380,11 @@ struct Dev {
   // gather hubs when every vertex is active (fixed per graph): vertex ids
   const int32_t *ahub, *ahoff, *ahpart;
   int32_t n_ahub, ahub_items;
+  // the same hubs as one span of vertices [span_lo, span_hi) with in-edges
+  // [span_e0, span_e1), and the vertex holding each span tile's first edge
+  // (null: the hubs are not one span, or the program cannot use it)
+  const int32_t *span_first;
+  int32_t span_lo, span_hi, span_e0, span_e1;
   // scatter hubs of the current superstep: slots
   int32_t *shub, *shoff, *shpart;
   int32_t *bsum;     // per-chunk sums of the two-pass scans
@@ -975,7 +971,7 @@ __global__ __launch_bounds__(NT) void k_hscan2(Dev<P> d, int p) {
 // leaves the partial before its first end (part_first) and after its last
 // end (part_last), and k_hfix folds them.
 template <class P>
-__global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
+__device__ __forceinline__ void hgather_tiles(const Dev<P> &d, int p) {
   typedef typename P::GatherResult GR;
   __shared__ int32_t s_end[HUB_SEG_MAX];
   __shared__ int32_t s_base[HUB_SEG_MAX];
@@ -1069,9 +1065,9 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
     int nend = 0, first_k = -1;
 #pragma unroll
     for (int it = 0; it < MP_VT; ++it) {
-      if (kind[it] == 1) {
-        acc.add(val[it]);
-      } else if (kind[it] == 2) {
+      // a compensated sum adds the 0 of a non-edge item exactly: no branch
+      if (is_compensated<P>::value || kind[it] == 1) acc.add(val[it]);
+      if (kind[it] == 2) {
         if (nend == 0) {
           head = acc;
           first_k = kk[it];
@@ -1107,7 +1103,7 @@ __global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
 // Folds the tile partials of hubs whose edges cross tiles.
 constexpr int HFIX_LOADS = 16;
 template <class P>
-__global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
+__device__ __forceinline__ void hfix_tiles(const Dev<P> &d, int p) {
   if (frontier_size(d) == 0) return;
   const HubList<P> h = gather_hubs(d);
   if (h.n == 0) return;
@@ -1137,6 +1133,149 @@ __global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
     }
     d.gtmp[h.list[s]] = a.finish();
   }
+}
+
+// Hub gather over one contiguous span of hubs (all-active supersteps of a
+// source-only, compensated-sum program whose hubs are the vertices
+// [span_lo, span_hi), as in a graph numbered by descending in-degree):
+// their in-edges are the CSC range [span_e0, span_e1), cut into tiles of
+// SPAN_TE edges.  A tile's source ids load at once (striped: lanes take
+// consecutive edges) beside the offsets of the segments it overlaps --
+// span_first[t] names the vertex holding its first edge -- so its only
+// dependent load is the gather itself.  Values go through LDS to thread
+// order; each thread sums its SPAN_VT consecutive edges and closes a
+// segment after its last edge; a segmented workgroup scan carries partial
+// sums between threads; segments crossing tiles leave per-tile partials
+// (part_first / part_last) that hfix_span folds in tile order.
+template <class P>
+__device__ __forceinline__ void hgather_span(const Dev<P> &d) {
+  typedef typename P::GatherResult GR;
+  static_assert(is_source_only<P>::value && is_compensated<P>::value,
+                "span gather: source-only programs with a compensated sum");
+  constexpr int PADDED = SPAN_TE + SPAN_TE / 32;
+  __shared__ GR s_val[PADDED];
+  __shared__ int32_t s_off[SPAN_SEG];
+  __shared__ SegItem<P> s_wtot[NT / WAVE];
+  const int hi = d.span_hi, e0 = d.span_e0, e1 = d.span_e1;
+  const int ntiles = (e1 - e0 + SPAN_TE - 1) / SPAN_TE;
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int E0 = e0 + t * SPAN_TE;
+    const int vb = d.span_first[t];
+    const int nsl = min(SPAN_SEG, hi - vb + 1);  // offsets of vb .. vb + nsl - 1
+    int src[SPAN_VT];
+#pragma unroll
+    for (int it = 0; it < SPAN_VT; ++it) {
+      const int e = E0 + it * NT + tid;
+      src[it] = e < e1 ? d.csc_src[e] : 0;
+    }
+    for (int k = tid; k < nsl; k += NT) s_off[k] = d.csc_off[vb + k];
+#pragma unroll
+    for (int it = 0; it < SPAN_VT; ++it) {
+      const int e = E0 + it * NT + tid;
+      const int j = it * NT + tid;
+      s_val[j + (j >> 5)] = e < e1 ? d.srcval[src[it]] : GR();
+    }
+    __syncthreads();
+    // this thread's edges q0 .. q0 + SPAN_VT - 1; k: the segment holding q0
+    // (past the span's end: no edges, nothing closes)
+    const int q0 = E0 + tid * SPAN_VT;
+    int lo_k = 0, hi_k = nsl;  // last k with s_off[k] <= q0
+    while (hi_k - lo_k > 1) {
+      const int mid = (lo_k + hi_k) >> 1;
+      if (s_off[mid] <= q0) lo_k = mid; else hi_k = mid;
+    }
+    int k = lo_k;
+    int next = k + 1 < nsl ? s_off[k + 1] : INT_MAX;
+    Acc<P> acc = Acc<P>::empty(), head = Acc<P>::empty();
+    int nend = 0, first_k = 0;
+#pragma unroll
+    for (int i = 0; i < SPAN_VT; ++i) {
+      const int j = tid * SPAN_VT + i;
+      acc.add(s_val[j + (j >> 5)]);  // 0 past the end: exact
+      if (q0 + i + 1 == next) {      // the last edge of segment vb + k
+        if (nend == 0) {
+          head = acc;
+          first_k = k;
+        } else {
+          d.gtmp[vb + k] = acc.finish();
+        }
+        ++nend;
+        acc = Acc<P>::empty();
+        ++k;
+        next = k + 1 < nsl ? s_off[k + 1] : INT_MAX;
+      }
+    }
+    SegItem<P> me;
+    me.a = acc;
+    me.flag = nend > 0;
+    SegItem<P> total;
+    const SegItem<P> carry = block_seg_scan(me, total, s_wtot);
+    if (nend > 0) {
+      Acc<P> first = carry.a;
+      first.merge(head);
+      if (s_off[first_k] >= E0)
+        d.gtmp[vb + first_k] = first.finish();  // started in this tile
+      else
+        d.part_first[t] = first;  // the tile's first segment, from earlier tiles
+    }
+    if (tid == NT - 1) {
+      if (total.flag)
+        d.part_last[t] = total.a;
+      else
+        d.part_first[t] = total.a;
+    }
+    __syncthreads();
+  }
+}
+
+// Folds the partials of span segments that cross tiles: one thread per
+// tile boundary t; the segment holding the tile's first edge crosses it
+// when it starts before, and is folded at its first crossing (it starts in
+// tile t - 1): its head in tile t - 1, then the first partials of the tiles
+// up to the one holding its last edge, in tile order.
+template <class P>
+__device__ __forceinline__ void hfix_span(const Dev<P> &d) {
+  const int e0 = d.span_e0, e1 = d.span_e1;
+  const int ntiles = (e1 - e0 + SPAN_TE - 1) / SPAN_TE;
+  for (int t = 1 + blockIdx.x * NT + threadIdx.x; t < ntiles;
+       t += gridDim.x * NT) {
+    const int s = d.span_first[t];
+    const int st = d.csc_off[s];
+    const int E0 = e0 + t * SPAN_TE;
+    if (st >= E0 || st < E0 - SPAN_TE) continue;
+    const int t1 = (d.csc_off[s + 1] - 1 - e0) / SPAN_TE;
+    Acc<P> a = st > E0 - SPAN_TE ? d.part_last[t - 1] : d.part_first[t - 1];
+    for (int u = t; u <= t1; u += HFIX_LOADS) {
+      Acc<P> b[HFIX_LOADS];
+#pragma unroll
+      for (int j = 0; j < HFIX_LOADS; ++j)
+        b[j] = u + j <= t1 ? d.part_first[u + j] : Acc<P>::empty();
+#pragma unroll
+      for (int j = 0; j < HFIX_LOADS; ++j)
+        if (u + j <= t1) a.merge(b[j]);
+    }
+    d.gtmp[s] = a.finish();
+  }
+}
+
+// Hub gather of a superstep: merge-path tiles over the hub list, or (Span)
+// the contiguous-span form above.
+template <class P, bool Span = false>
+__global__ __launch_bounds__(NT) void k_hgather(Dev<P> d, int p) {
+  if constexpr (Span)
+    hgather_span(d);
+  else
+    hgather_tiles(d, p);
+}
+
+// Tile-crossing partials of the hub gather (Span: of the span form).
+template <class P, bool Span = false>
+__global__ __launch_bounds__(NT) void k_hfix(Dev<P> d, int p) {
+  if constexpr (Span)
+    hfix_span(d);
+  else
+    hfix_tiles(d, p);
 }
 
 // Optional program hook: `static double residual(const VertexData &before,

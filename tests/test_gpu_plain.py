@@ -240,6 +240,30 @@ def test_hub_segments_cross_tiles(gpu):
     assert rel_l1(r, oracle.check_pagerank_fp64(n, s, d, 10)) <= PR_REL_L1
 
 
+def test_pagerank_hub_span(gpu):
+    # hubs are vertices 0..H-1 (in-degree > 4), every other vertex has
+    # in-degree <= 4: with gather t2 = 4 the all-active hub list is one span
+    # of ids, gathered in 2048-edge tiles.  Hub degrees put segment ends on
+    # tile boundaries (2048, 2048 + 2048), inside one tile, and across many
+    # (70000); the PageRank values are checked vertex by vertex against FP64.
+    rng = np.random.default_rng(11)
+    degs = [2048, 100, 1948, 6000, 5, 5, 2038, 70000, 9, 4096, 5]
+    degs += rng.integers(5, 300, 300).tolist()
+    H, n = len(degs), 50000
+    hd = np.repeat(np.arange(H), degs)
+    ld = np.repeat(np.arange(H, n), rng.integers(0, 5, n - H))
+    d = np.concatenate([hd, ld])
+    s = rng.integers(0, n, len(d))
+    s = np.where(s == d, (s + 1) % n, s)
+    g = gas.Graph(_edges(n, s, d))
+    for tiers in ([4, 4, 16, 32], [4, 64, 16, 32]):
+        g.set_tiers(gas.PROGRAM_PAGERANK, tiers)
+        r, _, _ = g.pagerank(10)
+        r64 = oracle.check_pagerank_fp64(n, s, d, 10)
+        assert rel_l1(r, r64) <= PR_REL_L1
+        assert np.max(np.abs(r - r64) / r64) <= 1e-5
+
+
 def test_stepwise_engine_api_bfs_loop(gpu):
     # the reference BFS driver loop (bfs.cu:91-112) through gas_engine_*
     el = workloads.road(128, 128, seed=3)

@@ -244,7 +244,8 @@ static size_t noalloc_pagerank_bytes(int64_t nV, int64_t nE) {
                              4 * 1 /* activation, changed */ + 1 /* apply flags */ +
                              4 /* gather results */ + 4 * 4 /* hub lists, offsets */ +
                              2 * 4 /* all-active hubs, offsets */;
-  const int64_t per_tile = 3 * 4 /* partitions */ + 2 * (int64_t)sizeof(Acc<PageRank>);
+  const int64_t per_tile = 3 * 4 /* partitions */ + 4 /* span tiles */ +
+                           2 * (int64_t)sizeof(Acc<PageRank>);
   return (size_t)(per_vertex * (nV + 4) + per_tile * tiles +
                   SCAN_BLOCKS * (4 + 8 * STAT_SLOTS + 8 + 8) + 64 * 256);
 }
