@@ -2030,23 +2030,30 @@ __global__ void k_fill_vdata(Dev<P> d, typename P::VertexData value,
 
 // Run totals: slot k of the per-workgroup counters summed over the
 // workgroups into out[k] (host-mapped memory, system-scope stores), one
-// workgroup in a fixed order.
+// workgroup in a fixed order; every load is issued before the first sum.
 template <class P>
 __global__ __launch_bounds__(NT) void k_stat_totals(Dev<P> d, int64_t *out) {
-  __shared__ int64_t lds[NT / WAVE];
+  static_assert(SCAN_BLOCKS % NT == 0, "whole rounds of the workgroup");
+  constexpr int PER = SCAN_BLOCKS / NT;
+  __shared__ int64_t lds[STAT_SLOTS][NT / WAVE];
+  int64_t x[STAT_SLOTS][PER];
+#pragma unroll
+  for (int k = 0; k < STAT_SLOTS; ++k)
+#pragma unroll
+    for (int i = 0; i < PER; ++i) x[k][i] = d.stat_acc[k * SCAN_BLOCKS + i * NT + threadIdx.x];
+#pragma unroll
   for (int k = 0; k < STAT_SLOTS; ++k) {
     int64_t s = 0;
-    for (int b = threadIdx.x; b < SCAN_BLOCKS; b += NT)
-      s += d.stat_acc[k * SCAN_BLOCKS + b];
-    for (int x = WAVE / 2; x > 0; x >>= 1) s += __shfl_xor(s, x, WAVE);
-    if (lane_id() == 0) lds[threadIdx.x / WAVE] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t t = 0;
-      for (int w = 0; w < NT / WAVE; ++w) t += lds[w];
-      __hip_atomic_store(out + k, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) s += x[k][i];
+    for (int o = WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
+    if (lane_id() == 0) lds[k][threadIdx.x / WAVE] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < STAT_SLOTS) {
+    int64_t t = 0;
+    for (int w = 0; w < NT / WAVE; ++w) t += lds[threadIdx.x][w];
+    __hip_atomic_store(out + threadIdx.x, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
