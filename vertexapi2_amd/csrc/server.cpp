@@ -72,6 +72,11 @@ constexpr int kRoundsPerPoll = 4;
 constexpr double kSoloMaxMs = 2.0;
 constexpr double kSoloIdleMaxMs = 50.0;
 constexpr int kSoloBacklog = 4;
+// While this many queries wait or run in batch lanes, none goes solo: under
+// load the batches answer each for a fraction of the plain engine's GPU
+// time (road1024 + R-MAT-22 mix at 5,000 q/s offered: 2,884 q/s with solo
+// queries at any load, 3,097 q/s with none).
+constexpr int kSoloMaxInflight = 48;
 constexpr double kSoloQueueMs = 8.0;
 // Solo threads: the plain engine's supersteps are latency-bound, so two of
 // them (each with its own plain engines and stream) overlap on the GPU.
@@ -509,7 +514,8 @@ struct Session {
     // 3. waiting queries go to the solo thread with the shortest queue, in
     // arrival order, when their plain-engine time is at most solo_max_ms, or
     // at most solo_idle_max_ms while fewer than kSoloBacklog queries of their
-    // kind and graph wait (more would share a batch), and that queue would
+    // kind and graph wait (more would share a batch), fewer than
+    // kSoloMaxInflight queries wait or run in batches, and that queue would
     // finish them no later than a batch of one (an idle solo thread takes
     // one regardless).  Solo threads after the first take queries only while
     // at most kSoloBacklog queries wait in all: under load the batches answer
@@ -520,6 +526,7 @@ struct Session {
       std::vector<int> backlog(s->engines.size(), 0);
       for (int i : waiting) ++backlog[s->key(q[i].kind, q[i].graph)];
       const int n_solo = waiting.size() <= (size_t)kSoloBacklog ? kSoloThreads : 1;
+      const bool loaded = (int)waiting.size() + runningTotal() >= kSoloMaxInflight;
       for (auto it = waiting.begin(); it != waiting.end();) {
         const int i = *it;
         const size_t key = s->key(q[i].kind, q[i].graph);
@@ -532,7 +539,7 @@ struct Session {
           if (workers[w].queued_ms < wk->queued_ms ||
               (workers[w].pending == 0 && wk->pending > 0))
             wk = &workers[w];
-        if (ms > cap || (wk->pending > 0 && wk->queued_ms + ms > budget)) {
+        if (ms > cap || loaded || (wk->pending > 0 && wk->queued_ms + ms > budget)) {
           ++it;
           continue;
         }
