@@ -99,7 +99,12 @@ struct BatchDev {
   // sl_cur[q]: front[q] (BFS) / chg (SSSP, q = 0) may be nonzero there.
   uint8_t *sl_next;
   uint8_t *sl_cur[2];
+  // host-mapped ring of BSTATUS (tag, lanes alive) pairs: the list pass of
+  // round r stores the lanes still alive after it, then tag r + 1
+  // (system-scope stores; null: no report)
+  int64_t *status;
 };
+constexpr int BSTATUS = 16;
 
 // SSSP: a lane relaxes from a vertex in a round only while its distance
 // there is below (rounds since the lane began) * delta; changed lanes above
@@ -337,6 +342,12 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
       c->lanes_alive = any;
       c->pull = pull ? 1 : 0;
       c->n_hubs = pull ? 0 : htotal;
+      if (d.status) {
+        int64_t *e = d.status + 2 * (d.round & (BSTATUS - 1));
+        __hip_atomic_store(e + 1, (int64_t)any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(e, (int64_t)d.round + 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
 }
@@ -805,6 +816,12 @@ class BatchEngine {
   BatchEngine(gas_graph_t caller, Kind kind) : m_kind(kind) {
     check(gasrt_stream_create(&m_stream), "stream");
     check(gasrt_host_alloc((void **)&m_hctl, sizeof(BatchCtl) * 2), "pinned");
+    void *dev_status = nullptr;
+    check(gasrt_host_alloc_mapped((void **)&m_status, &dev_status,
+                                  sizeof(int64_t) * 2 * BSTATUS),
+          "mapped status");
+    for (int i = 0; i < 2 * BSTATUS; ++i) m_status[i] = 0;
+    m_dev.status = static_cast<int64_t *>(dev_status);
     gas_graph_t g = nullptr;
     const int32_t *dnew = nullptr;
     check(gas_graph_ordered_view(caller, &g, &dnew, &m_hnew), "vertex order");
@@ -912,6 +929,7 @@ class BatchEngine {
     if (m_stream) gasrt_stream_synchronize(m_stream);
     if (m_ev) gasrt_event_destroy(m_ev);
     gasrt_host_free(m_hctl);
+    gasrt_host_free((void *)m_status);
     gasrt_stream_destroy(m_stream);
   }
 
@@ -972,7 +990,7 @@ class BatchEngine {
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
-      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+      launch(kb_list, SCAN_BLOCKS, m_stream, quiet(),
              (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
@@ -981,10 +999,18 @@ class BatchEngine {
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
-      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
+      launch(kb_list, SCAN_BLOCKS, m_stream, quiet(),
              (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     }
+  }
+
+  // The engine's device view without the round report (list passes
+  // outside a round).
+  BatchDev quiet() const {
+    BatchDev d = m_dev;
+    d.status = nullptr;
+    return d;
   }
 
   // Frees lanes (their results stay readable until the lane is reused).
@@ -1004,6 +1030,7 @@ class BatchEngine {
       if (too_deep(m_used & m_alive_hint) && too_deep(lanesAlive()))
         throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
     }
+    m_dev.round = m_round;
     if (m_kind == BFS_KIND) {
       launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
       if (m_dev.pull_div > 0)
@@ -1015,7 +1042,6 @@ class BatchEngine {
              (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
       if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
     } else {
-      m_dev.round = m_round;
       launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
@@ -1067,14 +1093,39 @@ class BatchEngine {
 
   // Rounds until every used lane is finished.  BFS levels of a batch that
   // started together are at most ~V, so the loop is bounded by the graph.
+  // Rounds are enqueued up to kAhead ahead of the host, which reads each
+  // round's report from the mapped ring in order and stops at the first
+  // with no used lane alive; the rounds enqueued past it find empty lists
+  // and change nothing.
   int runToCompletion() {
-    int rounds = 0;
-    while (lanesAlive()) {
-      step();
-      ++rounds;
+    if (!lanesAlive()) return 0;
+    const int r0 = m_round;
+    int next = r0;  // the round whose report is awaited
+    for (;;) {
+      while (m_round - next < kAhead) step();
+      const int64_t *e = m_status + 2 * (next & (BSTATUS - 1));
+      uint64_t alive = 0;
+      for (int64_t spin = 0;; ++spin) {
+        if (__atomic_load_n(e, __ATOMIC_ACQUIRE) == (int64_t)next + 1) {
+          alive = (uint64_t)__atomic_load_n(e + 1, __ATOMIC_RELAXED);
+          break;
+        }
+        if ((spin & 255) == 255) {
+          int idle = 0;
+          check(gasrt_stream_query(m_stream, &idle), "stream query");
+          if (idle && __atomic_load_n(e, __ATOMIC_ACQUIRE) != (int64_t)next + 1)
+            throw Error(GAS_ERR_STATE, "batch round status was not reported");
+        }
+        __builtin_ia32_pause();
+      }
+      if (!(alive & m_used)) {
+        m_alive_hint = alive;
+        return next - r0 + 1;
+      }
+      ++next;
     }
-    return rounds;
   }
+  static constexpr int kAhead = 3;
 
   // Result of a lane into a device buffer of V int32 (BFS depth, -1
   // unreached; SSSP distance, 2147383647 unreached).
@@ -1126,6 +1177,7 @@ class BatchEngine {
   const int32_t *m_hnew = nullptr;  // caller's vertex -> engine graph's (host)
   int m_par = 0;
   int m_round = 0;
+  int64_t *m_status = nullptr;  // mapped round reports (BSTATUS pairs)
   uint64_t m_used = 0;
   int32_t m_hstart[BW] = {};
   uint64_t m_alive_hint = 0;        // lanes not yet seen finished by the host
