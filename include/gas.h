@@ -447,11 +447,21 @@ int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta);
  * among the default (the first candidate) and the settings no slower than
  * 1.1x the default on any sample -- GAS_BATCH_BFS: pull divisor in {0, 2, 8, 32};
  * GAS_BATCH_SSSP: distance step in {0, w/4, w/2, w} for the largest edge
- * weight w.  Batch engines created on the graph afterwards (gas_batch_create,
- * the server's engines) start with it; param_out (may be NULL) receives it. */
+ * weight w -- then, with that setting, the hub threshold
+ * (gas_graph_batch_hub_items).  Batch engines created on the graph afterwards
+ * (gas_batch_create, the server's engines) start with both; param_out (may
+ * be NULL) receives the first. */
 int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out);
 /* The setting batch engines on this graph start with (0 until tuned). */
 int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param);
+/* Hub threshold of the batch engines on this graph: a list vertex with more
+ * push items (16 out-edges each for BFS, 64 for SSSP) than this has its
+ * items written by a whole workgroup.  The tuned value (the autotuner's
+ * second stage, {32, 16, 64, 128} under the same rule, with the first
+ * stage's setting), 32 until tuned.  No counterpart in the reference. */
+int gas_graph_batch_hub_items(gas_graph_t g, int kind, int32_t *items);
+/* Sets one batch engine's hub threshold (>= 16 items). */
+int gas_batch_set_hub_items(gas_batch_t b, int32_t items);
 int gas_batch_free(gas_batch_t b);
 
 /* ---- continuous-batching query server -------------------------------------

@@ -45,7 +45,9 @@ constexpr int BFS_ITEM_EDGES = 16;   // BFS push: out-edges per item, 4 items pe
 constexpr int SSSP_ITEM_EDGES = 64;  // SSSP push: out-edges per item (one wave)
 constexpr int BFS_ITEMS_PER_TRIP = 8;  // BFS push: items in flight per wave
 constexpr int HUB_ITEMS = 32;     // a vertex with more items is written by
-                                  // a whole workgroup of kb_hub_items
+                                  // a whole workgroup of kb_hub_items (the
+                                  // default; gas_batch_set_hub_items)
+constexpr int HUB_ITEMS_MIN = 16; // smallest hub threshold (sizes the hub list)
 constexpr int SSSP_UNROLL = 8;    // edges per item per wave trip in the SSSP push
 constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks together
 // Workgroups of the SSSP push and the PPR item pass: items differ widely in
@@ -83,7 +85,8 @@ struct BatchDev {
   int4 *item;                           // push items: BFS {e0, e1, mask};
                                         // SSSP {u, e0, e1, 0}
   int32_t item_edges;                   // out-edges per item
-  int4 *hub;                            // vertices with > HUB_ITEMS items:
+  int32_t hub_items;                    // hub threshold in items
+  int4 *hub;                            // vertices with > hub_items items:
                                         // {v, first item, e0, e1}
   uint64_t *hub_mask;                   // their masks
   int32_t *bsum;                        // [3][SCAN_BLOCKS] count | items | hubs
@@ -226,7 +229,7 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
       ++cnt;
       const int nit = (deg + d.item_edges - 1) / d.item_edges;
       items += nit;
-      hubs += nit > HUB_ITEMS;
+      hubs += nit > d.hub_items;
       edges += deg;
       any |= x;
     }
@@ -282,7 +285,7 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
     }
     // one scan for four counters: list slots | hub slots (16 bits each,
     // at most NT per round) and items | non-hub items (32 bits each)
-    const bool hub = nit > HUB_ITEMS;
+    const bool hub = nit > d.hub_items;
     int pk, tk;
     int64_t pi, ti;
     block_excl_scan2((on ? 1 : 0) | (hub ? 1 << 16 : 0),
@@ -304,7 +307,7 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
     s_m[threadIdx.x] = mv;
     if (threadIdx.x == 0) s_jpos[NT] = jtot;
     const int h = hcarry + (pk >> 16);
-    if (nit > HUB_ITEMS) {
+    if (nit > d.hub_items) {
       d.hub[h] = make_int4(v, icarry + iex, o, o1);
       d.hub_mask[h] = mv;
     }
@@ -569,7 +572,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
       ++cnt;
       const int nit = (deg + d.item_edges - 1) / d.item_edges;
       items += nit;
-      hubs += nit > HUB_ITEMS;
+      hubs += nit > d.hub_items;
       edges += deg;
       any |= nv;
     }
@@ -733,7 +736,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
       ++cnt;
       const int nit = (deg + d.item_edges - 1) / d.item_edges;
       items += nit;
-      hubs += nit > HUB_ITEMS;
+      hubs += nit > d.hub_items;
       edges += deg;
       any |= x;
     }
@@ -844,14 +847,13 @@ class BatchEngine {
     d.pull_div = 0;  // push only: see setPullDivisor
     d.item_edges = kind == BFS_KIND ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
     {
-      // kb_hub_items runs only where some vertex can have > HUB_ITEMS items
+      // kb_hub_items runs only where some vertex can have > hub_items items
       std::vector<int32_t> off((size_t)gv.n_vertices + 1);
       check(gas_graph_host_csr(g, off.data(), nullptr, nullptr), "csr");
-      int64_t max_out = 0;
       for (int32_t v = 0; v < gv.n_vertices; ++v)
-        max_out = std::max<int64_t>(max_out, off[v + 1] - off[v]);
-      m_may_hub = max_out > (int64_t)HUB_ITEMS * d.item_edges;
+        m_max_out = std::max<int64_t>(m_max_out, off[v + 1] - off[v]);
     }
+    setHubItems(HUB_ITEMS);
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
     m_item.alloc((size_t)gv.n_edges / d.item_edges + V + 1);
@@ -903,9 +905,10 @@ class BatchEngine {
     d.ctl = m_ctl.p;
     d.list = m_list.p;
     d.item = m_item.p;
-    // a hub has > HUB_ITEMS items, so at most E / (HUB_ITEMS * item_edges)
-    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS * d.item_edges) + 1);
-    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS * d.item_edges) + 1);
+    // a hub has > HUB_ITEMS_MIN items, so at most
+    // E / (HUB_ITEMS_MIN * item_edges) of them
+    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * d.item_edges) + 1);
+    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * d.item_edges) + 1);
     d.hub = m_hub.p;
     d.hub_mask = m_hub_mask.p;
     d.bsum = m_bsum.p;
@@ -923,6 +926,11 @@ class BatchEngine {
       setPullDivisor(param);
     else
       setSsspDelta(param);
+    int32_t hub = 0;
+    check(gas_graph_batch_hub_items(caller, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP,
+                                    &hub),
+          "batch hub items");
+    setHubItems(hub);
   }
 
   ~BatchEngine() {
@@ -957,6 +965,14 @@ class BatchEngine {
   // SSSP distance bound step (0: Bellman-Ford rounds).
   void setSsspDelta(int delta) {
     if (m_kind == SSSP_KIND) m_dev.sssp_delta = delta;
+  }
+  // Items above which a list vertex's push items are written by a whole
+  // workgroup (kb_hub_items) instead of its list pass thread.
+  void setHubItems(int items) {
+    if (items < HUB_ITEMS_MIN || items > (1 << 24))
+      throw Error(GAS_ERR_ARG, "hub threshold outside [16, 2^24] items");
+    m_dev.hub_items = items;
+    m_may_hub = m_max_out > (int64_t)items * m_dev.item_edges;
   }
 
   // Starts queries: lane[i] runs from source[i].  The lanes must be free.
@@ -1193,7 +1209,8 @@ class BatchEngine {
   DevBuf<int4> m_hub;
   DevBuf<uint16_t> m_depth;
   DevBuf<uint8_t> m_sl_next, m_sl0, m_sl1;  // slab activity flags
-  bool m_may_hub = true;  // some vertex has more than HUB_ITEMS items
+  bool m_may_hub = true;  // some vertex has more than hub_items items
+  int64_t m_max_out = 0;  // largest out-degree
 };
 
 // ---- personalised PageRank, 64 FP32 lanes ----------------------------------

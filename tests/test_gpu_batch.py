@@ -293,6 +293,8 @@ def test_batch_autotune(gpu):
     assert g.batch_param(gas.BATCH_SSSP) == delta
     pull = g.autotune_batch(gas.BATCH_BFS)
     assert pull in (0, 2, 8, 32)
+    for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
+        assert g.batch_hub_items(kind) in (16, 32, 64, 128)
     # batches created now start with the tuned settings; results stay exact
     src = _sources(np.random.default_rng(4), el, 10)
     for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
@@ -307,6 +309,31 @@ def test_batch_autotune(gpu):
     nw = gas.Graph(gas.Edges.from_arrays(3, [0, 1], [1, 2]))
     with pytest.raises(gas.GasError):
         nw.autotune_batch(gas.BATCH_SSSP)  # no weights
+
+
+@pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP])
+def test_batch_hub_threshold(gpu, kind):
+    # every hub threshold gives the same, exact per-lane results (R-MAT-14:
+    # list vertices of hundreds of push items)
+    el = workloads.rmat(14, symmetrize=True, weights=True)
+    s, d, w = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    assert g.batch_hub_items(kind) == 32
+    src = _sources(np.random.default_rng(6), el, 16)
+    ref = [oracle.check_bfs(n, s, d, int(x)) if kind == gas.BATCH_BFS
+           else oracle.check_dijkstra(n, s, d, w, int(x)) for x in src]
+    for items in (16, 32, 64, 128, 1 << 20):
+        b = gas.Batch(g, kind)
+        b.set_hub_items(items)
+        b.add(np.arange(16), src)
+        b.run()
+        res = b.results(range(16))
+        for lane in range(16):
+            assert np.array_equal(res[lane], ref[lane]), (items, lane)
+    b = gas.Batch(g, kind)
+    with pytest.raises(gas.GasError):
+        b.set_hub_items(15)
 
 
 @pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP, gas.BATCH_PPR])

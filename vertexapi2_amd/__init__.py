@@ -364,6 +364,13 @@ class Graph:
         _check(lib().gas_graph_batch_param(self._h, C.c_int(kind), C.byref(p)))
         return p.value
 
+    def batch_hub_items(self, kind: int) -> int:
+        """Hub threshold (push items) the batch engines on this graph start
+        with: the tuned value, 32 until tuned."""
+        p = C.c_int32(0)
+        _check(lib().gas_graph_batch_hub_items(self._h, C.c_int(kind), C.byref(p)))
+        return p.value
+
     def set_push(self, program: int, enable: bool) -> None:
         """Push supersteps of whole SSSP / CC runs (same results and
         superstep counts as the pull)."""
@@ -610,6 +617,11 @@ class Batch:
 
     def set_pull_divisor(self, divisor: int) -> None:
         _check(lib().gas_batch_set_pull_divisor(self._h, C.c_int32(divisor)))
+
+    def set_hub_items(self, items: int) -> None:
+        """List vertices with more push items than this have their items
+        written by a whole workgroup (>= 16)."""
+        _check(lib().gas_batch_set_hub_items(self._h, C.c_int32(items)))
 
     def set_sssp_delta(self, delta: int) -> None:
         """SSSP lanes relax a vertex only while its distance is below

@@ -208,6 +208,22 @@ int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param) {
   return GAS_OK;
 }
 
+int gas_graph_batch_hub_items(gas_graph_t g, int kind, int32_t *items) {
+  if (!g || !items || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_graph_batch_hub_items: bad arguments");
+  *items = g->batch_hub[kind] ? g->batch_hub[kind] : HUB_ITEMS;
+  return GAS_OK;
+}
+
+int gas_batch_set_hub_items(gas_batch_t b, int32_t items) {
+  BATCH_OR_FAIL(b);
+  if (items < HUB_ITEMS_MIN)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_hub_items: need >= %d", HUB_ITEMS_MIN);
+  return guarded_b([&] {
+    if (b->e) b->e->setHubItems(items);
+  });
+}
+
 int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
   if (!g || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
     return set_error(GAS_ERR_ARG, "gas_graph_autotune_batch: bad arguments");
@@ -256,11 +272,12 @@ int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
     BatchEngine e(g, kind == GAS_BATCH_BFS ? BatchEngine::BFS_KIND
                                            : BatchEngine::SSSP_KIND);
     // per-sample times of one setting
-    auto time_one = [&](int32_t c) {
+    auto time_one = [&](int32_t c, int32_t hub) {
       if (kind == GAS_BATCH_BFS)
         e.setPullDivisor(c);
       else
         e.setSsspDelta(c);
+      e.setHubItems(hub);
       std::vector<double> t;
       for (const auto &src : samples) {
         const uint64_t all = src.size() == 64 ? ~0ull : ((1ull << src.size()) - 1);
@@ -280,20 +297,36 @@ int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
       for (double x : t) s += x;
       return s;
     };
-    time_one(cand[0]);  // warm-up
-    const std::vector<double> base = time_one(cand[0]);
+    time_one(cand[0], HUB_ITEMS);  // warm-up
+    const std::vector<double> base = time_one(cand[0], HUB_ITEMS);
     int32_t best = cand[0];
     double best_s = total(base);
+    std::vector<double> best_t = base;
     for (size_t i = 1; i < cand.size(); ++i) {
-      const std::vector<double> t = time_one(cand[i]);
+      const std::vector<double> t = time_one(cand[i], HUB_ITEMS);
       bool safe = true;
       for (int k = 0; k < kSamples; ++k) safe = safe && t[k] <= 1.1 * base[k];
       if (safe && total(t) < best_s) {
         best_s = total(t);
         best = cand[i];
+        best_t = t;
       }
     }
     g->batch_param[kind] = best;
+    // second stage: the hub threshold with that setting, same rule against
+    // the default threshold's times
+    int32_t best_hub = HUB_ITEMS;
+    const std::vector<double> hub_base = best_t;
+    for (int32_t h : {16, 64, 128}) {
+      const std::vector<double> t = time_one(best, h);
+      bool safe = true;
+      for (int k = 0; k < kSamples; ++k) safe = safe && t[k] <= 1.1 * hub_base[k];
+      if (safe && total(t) < best_s) {
+        best_s = total(t);
+        best_hub = h;
+      }
+    }
+    g->batch_hub[kind] = best_hub;
     if (param_out) *param_out = best;
   });
 }
