@@ -72,7 +72,8 @@ constexpr int kRoundsPerPoll = 4;
 constexpr double kSoloMaxMs = 2.0;
 constexpr double kSoloIdleMaxMs = 50.0;
 constexpr int kSoloBacklog = 4;
-// While this many queries wait or run in batch lanes, none goes solo: under
+// While this many queries wait, run in batch lanes or sit in the solo
+// queues, none goes solo: under
 // load the batches answer each for a fraction of the plain engine's GPU
 // time (road1024 + R-MAT-22 mix at 5,000 q/s offered: 2,884 q/s with solo
 // queries at any load, 3,097 q/s with none).
@@ -517,16 +518,20 @@ struct Session {
     // kind and graph wait (more would share a batch), fewer than
     // kSoloMaxInflight queries wait or run in batches, and that queue would
     // finish them no later than a batch of one (an idle solo thread takes
-    // one regardless).  Solo threads after the first take queries only while
-    // at most kSoloBacklog queries wait in all: under load the batches answer
-    // them for less GPU time each.  BFS / SSSP results are exact on either
+    // one regardless; queries above solo_max_ms queue on the first thread up
+    // to solo_idle_max_ms of work).  Solo threads after the first take
+    // queries only while at most kSoloBacklog queries wait in all: under
+    // load the batches answer them for less GPU time each.  BFS / SSSP results are exact on either
     // engine; a PPR result from the plain engine is within the same 1e-5 of
     // FP64 as a batch lane but not bit-identical to one.
     {
       std::vector<int> backlog(s->engines.size(), 0);
       for (int i : waiting) ++backlog[s->key(q[i].kind, q[i].graph)];
       const int n_solo = waiting.size() <= (size_t)kSoloBacklog ? kSoloThreads : 1;
-      const bool loaded = (int)waiting.size() + runningTotal() >= kSoloMaxInflight;
+      int solo_pending = 0;
+      for (int w = 0; w < kSoloThreads; ++w) solo_pending += workers[w].pending;
+      const bool loaded =
+          (int)waiting.size() + runningTotal() + solo_pending >= kSoloMaxInflight;
       for (auto it = waiting.begin(); it != waiting.end();) {
         const int i = *it;
         const size_t key = s->key(q[i].kind, q[i].graph);
@@ -534,12 +539,17 @@ struct Session {
         const double cap =
             backlog[key] < kSoloBacklog ? s->solo_idle_max_ms : s->solo_max_ms;
         const double budget = std::max(kSoloQueueMs, s->batch1_ms[key]);
+        // queries above solo_max_ms queue on the first solo thread, one
+        // after another, up to solo_idle_max_ms of queued work: two such
+        // runs at once share the GPU's bandwidth and both finish late
+        const bool heavy = ms > s->solo_max_ms;
         SoloWorker *wk = &workers[0];
-        for (int w = 1; w < n_solo; ++w)
+        for (int w = 1; w < n_solo && !heavy; ++w)
           if (workers[w].queued_ms < wk->queued_ms ||
               (workers[w].pending == 0 && wk->pending > 0))
             wk = &workers[w];
-        if (ms > cap || loaded || (wk->pending > 0 && wk->queued_ms + ms > budget)) {
+        const double limit = heavy ? std::max(budget, s->solo_idle_max_ms) : budget;
+        if (ms > cap || loaded || (wk->pending > 0 && wk->queued_ms + ms > limit)) {
           ++it;
           continue;
         }
