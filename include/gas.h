@@ -462,6 +462,19 @@ int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param);
 int gas_graph_batch_hub_items(gas_graph_t g, int kind, int32_t *items);
 /* Sets one batch engine's hub threshold (>= 16 items). */
 int gas_batch_set_hub_items(gas_batch_t b, int32_t items);
+/* Rounds per launch of a BFS / SSSP batch engine's captured round graph:
+ * k in 1..8 records k consecutive rounds' kernel launches once (hipGraph
+ * stream capture) and issues rounds as replays of that graph; 0 launches
+ * every round's kernels directly.  The round, its parity and the lanes in
+ * use live in device memory, so replays need no new arguments; results are
+ * identical either way.  The graph's value is what batch engines created on
+ * it afterwards (gas_batch_create, the server's engines) start with; 0 unless
+ * set (on one MI355X replays were no faster than direct launches on the road
+ * and R-MAT batches).  Replaces the reference's host-driven superstep loop
+ * (gpugas.h:764-772) for batches; the reference has no batched engine. */
+int gas_graph_batch_graph_rounds(gas_graph_t g, int kind, int32_t *rounds);
+int gas_graph_set_batch_graph_rounds(gas_graph_t g, int kind, int32_t rounds);
+int gas_batch_set_graph_rounds(gas_batch_t b, int32_t rounds);
 int gas_batch_free(gas_batch_t b);
 
 /* ---- continuous-batching query server -------------------------------------

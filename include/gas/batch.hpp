@@ -66,6 +66,13 @@ struct BatchCtl {
   uint64_t lanes_alive; // OR of the masks of the current list
   int64_t stat_edges;   // edges examined so far
   int64_t stat_rounds;
+  // Round state read by the kernels (not by-value arguments, so a captured
+  // graph of rounds replays unchanged): the rounds done so far, moved on by
+  // the list pass that ends each round (its workgroup 0, the only reader of
+  // the round in that launch), and the lanes in use.
+  int32_t round;
+  int32_t pad_;
+  uint64_t used;
 };
 
 struct BatchDev {
@@ -75,7 +82,9 @@ struct BatchDev {
   BatchCtl *ctl;
   uint64_t *vn;                         // BFS [V][2]: visited, next bits
                                         // (one 16-byte pair per vertex)
-  uint64_t *front[2];                   // BFS frontier masks by parity
+  uint64_t *front;                      // BFS frontier masks (the finalize
+                                        // pass overwrites them in place:
+                                        // push reads the items instead)
   uint16_t *depth16;                    // BFS [BW][rowV]
   int32_t rowV;                         // depth row stride (V rounded to 4)
   int32_t *lane_start;                  // [BW]: round the lane began
@@ -95,13 +104,12 @@ struct BatchDev {
   int32_t pull_div;                     // pull when list_edges*div >= E
                                         // (0: never, SSSP)
   int32_t sssp_delta;                   // SSSP distance bound step (0: none)
-  int32_t round;                        // round of this launch
   // Activity of 64-vertex slabs (SLAB): a slab whose flag is 0 holds no
   // nonzero mask of that array, so the O(V) passes skip it.  sl_next: some
   // vertex of the slab got bits in next (BFS) / chg_next (SSSP) this round;
-  // sl_cur[q]: front[q] (BFS) / chg (SSSP, q = 0) may be nonzero there.
+  // sl_cur: front (BFS) / chg (SSSP) may be nonzero there.
   uint8_t *sl_next;
-  uint8_t *sl_cur[2];
+  uint8_t *sl_cur;
   // host-mapped ring of BSTATUS (tag, lanes alive) pairs: the list pass of
   // round r stores the lanes still alive after it, then tag r + 1
   // (system-scope stores; null: no report)
@@ -113,10 +121,10 @@ constexpr int BSTATUS = 16;
 // there is below (rounds since the lane began) * delta; changed lanes above
 // the bound stay marked for a later round.  delta = 0: no bound
 // (Bellman-Ford rounds).
-__device__ inline int32_t sssp_bound(const BatchDev &d, int lane) {
+__device__ inline int32_t sssp_bound(const BatchDev &d, int lane, int round) {
   if (d.sssp_delta <= 0) return INT32_MAX;
   const int64_t b =
-      (int64_t)(d.round + 1 - d.lane_start[lane]) * (int64_t)d.sssp_delta;
+      (int64_t)(round + 1 - d.lane_start[lane]) * (int64_t)d.sssp_delta;
   return b > INT32_MAX ? INT32_MAX : (int32_t)b;
 }
 
@@ -248,15 +256,18 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
   }
 }
 
-// Pass 2: ordered list of vertices with nonzero masks, item offsets and the
-// owner of every item; workgroup 0 publishes totals and the direction.
-static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t *m,
-                                                      const uint8_t *slm) {
+// Pass 2: ordered list of vertices with nonzero masks (the BFS frontier or
+// the SSSP changes), item offsets and the owner of every item; workgroup 0
+// publishes totals and the direction and, at the end of a round
+// (advance = 1), reports the round and moves the round on.
+static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
   __shared__ int32_t s_ipos[NT], s_jpos[NT + 1], s_v[NT], s_o[NT], s_o1[NT];
   __shared__ uint64_t s_m[NT];
+  const uint64_t *m = d.dist ? d.chg : d.front;
+  const uint8_t *slm = d.sl_cur;
   int pre, total, ipre, itotal, hpre, htotal;
   chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
   chunk_prefix(d.bsum + 2 * SCAN_BLOCKS, blockIdx.x, hpre, htotal, lds);
@@ -345,11 +356,16 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, const uint64_t 
       c->lanes_alive = any;
       c->pull = pull ? 1 : 0;
       c->n_hubs = pull ? 0 : htotal;
-      if (d.status) {
-        int64_t *e = d.status + 2 * (d.round & (BSTATUS - 1));
-        __hip_atomic_store(e + 1, (int64_t)any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(e, (int64_t)d.round + 1, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+      if (advance) {
+        const int32_t round = c->round;
+        c->round = round + 1;
+        if (d.status) {
+          int64_t *e = d.status + 2 * (round & (BSTATUS - 1));
+          __hip_atomic_store(e + 1, (int64_t)any, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+          __hip_atomic_store(e, (int64_t)round + 1, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+        }
       }
     }
   }
@@ -375,12 +391,12 @@ static __global__ __launch_bounds__(NT) void kb_hub_items(BatchDev d) {
 
 // ---- BFS ---------------------------------------------------------------------
 // Clears lanes `reset` (visited bits, frontier bits of parity p, depth rows).
-static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
+static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d,
                                                    uint64_t reset) {
   const int stride = gridDim.x * NT;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += stride) {
     d.vn[2 * (size_t)v] &= ~reset;
-    d.front[p][v] &= ~reset;
+    d.front[v] &= ~reset;
   }
   // depth rows of the reset lanes, two depths per thread
   uint32_t *rows = reinterpret_cast<uint32_t *>(d.depth16);
@@ -393,15 +409,16 @@ static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d, int p,
 }
 
 // One thread per new lane: seed its source.
-static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
-                            const int32_t *sources, int n) {
+static __global__ void kb_bfs_seed(BatchDev d, const int32_t *lanes,
+                            const int32_t *sources, int n, uint64_t used) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) d.ctl->used = used;
   if (i >= n) return;
   const int s = lanes[i], v = sources[i];
   const uint64_t bit = 1ull << s;
   atomicOr((unsigned long long *)&d.vn[2 * (size_t)v], (unsigned long long)bit);
-  atomicOr((unsigned long long *)&d.front[p][v], (unsigned long long)bit);
-  d.sl_cur[p][v / SLAB] = 1;
+  atomicOr((unsigned long long *)&d.front[v], (unsigned long long)bit);
+  d.sl_cur[v / SLAB] = 1;
   d.depth16[(size_t)s * d.rowV + v] = 0;
 }
 
@@ -409,11 +426,11 @@ static __global__ void kb_bfs_seed(BatchDev d, int p, const int32_t *lanes,
 // and one lane per out-edge; all their loads are issued before any use.
 // New bits (frontier mask minus the destination's visited bits) are OR-ed
 // into the destination's next mask.
-static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
-                                                  uint64_t used) {
+static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d) {
   const BatchCtl *c = d.ctl;
   if (c->pull) return;
   const int n_items = c->n_items;
+  const uint64_t used = c->used;
   const int lane = lane_id();
   const int sub = lane / BFS_ITEM_EDGES, k = lane % BFS_ITEM_EDGES;
   constexpr int G = BFS_ITEMS_PER_TRIP / (WAVE / BFS_ITEM_EDGES);
@@ -453,12 +470,12 @@ static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d, int p,
     }
   }
 }
-static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
-                                                  uint64_t used) {
+static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d) {
   const BatchCtl *c = d.ctl;
   if (!c->pull) return;
-  used &= c->lanes_alive;  // a finished lane cannot reach more vertices
-  const uint64_t *fr = d.front[p];
+  // a finished lane cannot reach more vertices
+  const uint64_t used = c->used & c->lanes_alive;
+  const uint64_t *fr = d.front;
   const int lane = lane_id();
   const int stride = gridDim.x * NT;
   const int rounds = (d.nV + stride - 1) / stride;
@@ -521,17 +538,17 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d, int p,
 // New bits become visited and form the next frontier (parity p^1); depth
 // rows get round + 1 - lane_start.  Contiguous chunks, with the list
 // counters of the new frontier (pass 1 of kb_list).
-static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d, int p,
-                                                      uint64_t used,
-                                                      int round) {
+static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d) {
   __shared__ int lds[NT / WAVE];
   __shared__ int64_t lds64[NT / WAVE];
   __shared__ uint64_t ldsu[NT / WAVE];
   __shared__ int32_t s_start[BW];
   if (threadIdx.x < BW) s_start[threadIdx.x] = d.lane_start[threadIdx.x];
   __syncthreads();
-  uint64_t *nf = d.front[p ^ 1];
-  uint8_t *slf = d.sl_cur[p ^ 1];
+  const int round = d.ctl->round;
+  const uint64_t used = d.ctl->used;
+  uint64_t *nf = d.front;
+  uint8_t *slf = d.sl_cur;
   int cnt = 0, items = 0, hubs = 0;
   int64_t edges = 0;
   uint64_t any = 0;
@@ -615,13 +632,14 @@ static __global__ __launch_bounds__(NT) void kb_sssp_reset(BatchDev d,
 }
 
 static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
-                             const int32_t *sources, int n) {
+                             const int32_t *sources, int n, uint64_t used) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) d.ctl->used = used;
   if (i >= n) return;
   const int s = lanes[i], v = sources[i];
   d.dist[(size_t)v * BW + s] = 0;
   atomicOr((unsigned long long *)&d.chg[v], (unsigned long long)(1ull << s));
-  d.sl_cur[0][v / SLAB] = 1;
+  d.sl_cur[v / SLAB] = 1;
 }
 
 // Push from the changed list: one wave per item, lane = query.  The item's
@@ -635,7 +653,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   const int n_items = c->n_items;
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
-  const int32_t bound = sssp_bound(d, lane);
+  const int32_t bound = sssp_bound(d, lane, c->round);
   constexpr int G = SSSP_ITEMS_PER_TRIP;
   for (int base = (blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE) * G;
        base < n_items; base += waves * G) {
@@ -718,7 +736,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
     const int v = slab_vertex(r);
     // slabs with no new changes and no stale ones are zero in both arrays
     const int sl = v / SLAB;
-    const bool work = v < d.nV && (d.sl_next[sl] | d.sl_cur[0][sl]);
+    const bool work = v < d.nV && (d.sl_next[sl] | d.sl_cur[sl]);
     if (!__any(work)) continue;
     uint64_t x = 0;
     if (work) {
@@ -728,7 +746,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
     }
     const uint64_t wx = wave_or(x);
     if (lane_id() == 0 && v < d.nV) {
-      d.sl_cur[0][sl] = wx != 0;
+      d.sl_cur[sl] = wx != 0;
       d.sl_next[sl] = 0;
     }
     if (x) {
@@ -753,6 +771,11 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
     d.esum[blockIdx.x] = edges;
     d.lor[blockIdx.x] = any;
   }
+}
+
+// Lanes in use after a release (stream-ordered like the rounds).
+static __global__ void kb_set_used(BatchDev d, uint64_t used) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.ctl->used = used;
 }
 
 static __global__ __launch_bounds__(NT) void kb_sssp_extract(BatchDev d, int s,
@@ -865,17 +888,14 @@ class BatchEngine {
     if (kind == BFS_KIND) {
       d.rowV = (int32_t)((V + 3) / 4 * 4);
       m_vn.alloc(2 * V);
-      m_front0.alloc(V);
-      m_front1.alloc(V);
+      m_front.alloc(V);
       m_depth.alloc((size_t)d.rowV * BW);
       check(gasrt_memset(m_vn.p, 0, 2 * V * 8, m_stream), "memset");
-      check(gasrt_memset(m_front0.p, 0, V * 8, m_stream), "memset");
-      check(gasrt_memset(m_front1.p, 0, V * 8, m_stream), "memset");
+      check(gasrt_memset(m_front.p, 0, V * 8, m_stream), "memset");
       check(gasrt_memset(m_depth.p, 0xff, (size_t)d.rowV * BW * 2, m_stream),
             "memset");
       d.vn = m_vn.p;
-      d.front[0] = m_front0.p;
-      d.front[1] = m_front1.p;
+      d.front = m_front.p;
       d.depth16 = m_depth.p;
     } else {
       m_dist.alloc(V * BW);
@@ -892,13 +912,12 @@ class BatchEngine {
     check(gasrt_memset(m_lane_start.p, 0, BW * 4, m_stream), "memset");
     {
       const size_t nsl = (V + SLAB - 1) / SLAB;
-      for (auto *b : {&m_sl_next, &m_sl0, &m_sl1}) {
+      for (auto *b : {&m_sl_next, &m_sl_cur}) {
         b->alloc(nsl);
         check(gasrt_memset(b->p, 0, std::max<size_t>(nsl, 1), m_stream), "memset");
       }
       d.sl_next = m_sl_next.p;
-      d.sl_cur[0] = m_sl0.p;
-      d.sl_cur[1] = m_sl1.p;
+      d.sl_cur = m_sl_cur.p;
     }
     d.lane_start = m_lane_start.p;
     d.sssp_delta = 0;
@@ -931,10 +950,16 @@ class BatchEngine {
                                     &hub),
           "batch hub items");
     setHubItems(hub);
+    int32_t graph_rounds = 0;
+    check(gas_graph_batch_graph_rounds(
+              caller, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP, &graph_rounds),
+          "batch graph rounds");
+    setGraphRounds(graph_rounds);
   }
 
   ~BatchEngine() {
     if (m_stream) gasrt_stream_synchronize(m_stream);
+    if (m_gexec) gasrt_graph_destroy(m_gexec);
     if (m_ev) gasrt_event_destroy(m_ev);
     gasrt_host_free(m_hctl);
     gasrt_host_free((void *)m_status);
@@ -949,9 +974,9 @@ class BatchEngine {
     return m_ctl.bytes() + m_item.bytes() + m_list.bytes() + m_bsum.bytes() +
            m_lanes.bytes() + m_sources.bytes() + m_lane_start.bytes() + m_dist.bytes() +
            m_tmp.bytes() + m_esum.bytes() + m_lor.bytes() + m_vn.bytes() +
-           m_front0.bytes() + m_front1.bytes() + m_chg.bytes() + m_chg_next.bytes() +
+           m_front.bytes() + m_chg.bytes() + m_chg_next.bytes() +
            m_hub_mask.bytes() + m_hub.bytes() + m_depth.bytes() + m_sl_next.bytes() +
-           m_sl0.bytes() + m_sl1.bytes();
+           m_sl_cur.bytes();
   }
   int round() const { return m_round; }
   // BFS rounds pull instead of push when the frontier's out-edges reach
@@ -960,17 +985,24 @@ class BatchEngine {
   // the R-MAT batches measured here every pull round cost more than the
   // push it replaced.
   void setPullDivisor(int div) {
-    if (m_kind == BFS_KIND) m_dev.pull_div = div;
+    if (m_kind == BFS_KIND && m_dev.pull_div != div) {
+      dropGraph();
+      m_dev.pull_div = div;
+    }
   }
   // SSSP distance bound step (0: Bellman-Ford rounds).
   void setSsspDelta(int delta) {
-    if (m_kind == SSSP_KIND) m_dev.sssp_delta = delta;
+    if (m_kind == SSSP_KIND && m_dev.sssp_delta != delta) {
+      dropGraph();
+      m_dev.sssp_delta = delta;
+    }
   }
   // Items above which a list vertex's push items are written by a whole
   // workgroup (kb_hub_items) instead of its list pass thread.
   void setHubItems(int items) {
     if (items < HUB_ITEMS_MIN || items > (1 << 24))
       throw Error(GAS_ERR_ARG, "hub threshold outside [16, 2^24] items");
+    if (m_dev.hub_items != items) dropGraph();
     m_dev.hub_items = items;
     m_may_hub = m_max_out > (int64_t)items * m_dev.item_edges;
   }
@@ -1001,76 +1033,70 @@ class BatchEngine {
                            m_stream),
           "lane start");
     if (m_kind == BFS_KIND) {
-      launch(kb_bfs_reset, grid, m_stream, m_dev, m_par, reset);
-      launch(kb_bfs_seed, 1, m_stream, m_dev, m_par, m_lanes.p, m_sources.p, n);
+      launch(kb_bfs_reset, grid, m_stream, m_dev, reset);
+      launch(kb_bfs_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n,
+             m_used | reset);
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
-      launch(kb_list, SCAN_BLOCKS, m_stream, quiet(),
-             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
-      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
+             (const uint64_t *)m_dev.front, (const uint8_t *)m_dev.sl_cur);
     } else {
       launch(kb_sssp_reset, gridFor((int64_t)m_nV * BW), m_stream, m_dev, reset);
-      launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n);
+      launch(kb_sssp_seed, 1, m_stream, m_dev, m_lanes.p, m_sources.p, n,
+             m_used | reset);
       m_used |= reset;
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
-      launch(kb_list, SCAN_BLOCKS, m_stream, quiet(),
-             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
-      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
+             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur);
     }
-  }
-
-  // The engine's device view without the round report (list passes
-  // outside a round).
-  BatchDev quiet() const {
-    BatchDev d = m_dev;
-    d.status = nullptr;
-    return d;
+    launch(kb_list, SCAN_BLOCKS, m_stream, m_dev, 0);
+    if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
   }
 
   // Frees lanes (their results stay readable until the lane is reused).
-  void releaseLanes(uint64_t lanes) { m_used &= ~lanes; }
+  void releaseLanes(uint64_t lanes) {
+    if (!(m_used & lanes)) return;
+    m_used &= ~lanes;
+    launch(kb_set_used, 1, m_stream, m_dev, m_used);
+  }
 
   // One round for every used lane.
   void step() {
-    if (m_kind == BFS_KIND) {
-      // depths are 16-bit per lane: only lanes still running can outgrow
-      // them (a finished lane kept for its result does not); the host's
-      // view of which lanes run is refreshed when one reaches the limit
-      auto too_deep = [&](uint64_t lanes) {
-        for (int s = 0; s < BW; ++s)
-          if (((lanes >> s) & 1ull) && m_round + 1 - m_hstart[s] > 65534) return true;
-        return false;
-      };
-      if (too_deep(m_used & m_alive_hint) && too_deep(lanesAlive()))
-        throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
-    }
-    m_dev.round = m_round;
-    if (m_kind == BFS_KIND) {
-      launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
-      if (m_dev.pull_div > 0)
-        launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used);
-      launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev, m_par, m_used,
-             m_round);
-      m_par ^= 1;
-      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.front[m_par], (const uint8_t *)m_dev.sl_cur[m_par]);
-      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
-    } else {
-      launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
-      launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
-      launch(kb_list, SCAN_BLOCKS, m_stream, m_dev,
-             (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur[0]);
-      if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
-    }
+    checkDepth(1);
+    enqueueRound();
     ++m_round;
+  }
+
+  // Rounds per replay of the captured round graph; 0 launches each round's
+  // kernels directly.  The graph holds k consecutive rounds' launches;
+  // their arguments do not change between rounds (the round, its parity
+  // and the lanes in use are read from the control block), so one capture
+  // serves every replay until a setting it was captured with changes.
+  void setGraphRounds(int k) {
+    if (k < 0 || k > kMaxGraphRounds)
+      throw Error(GAS_ERR_ARG, "graph rounds outside [0, 8]");
+    dropGraph();
+    m_graph_rounds = k;
+  }
+  int graphRounds() const { return m_graph_rounds; }
+  static constexpr int kMaxGraphRounds = 8;
+
+  // At least `rounds` rounds: whole replays of the round graph while they
+  // fit, single rounds for the rest.
+  void advance(int rounds) {
+    while (rounds > 0) {
+      if (m_graph_rounds > 0 && rounds >= m_graph_rounds) {
+        replay();
+        rounds -= m_graph_rounds;
+      } else {
+        step();
+        --rounds;
+      }
+    }
   }
 
   // step() followed by an asynchronous snapshot of the control block and an
   // event; poll() tells when both are done, aliveSnapshot() reads it.
   void stepAsync(int rounds = 1) {
-    for (int i = 0; i < rounds; ++i) step();
+    advance(rounds);
     snapshotAsync();
   }
   void snapshotAsync() {
@@ -1118,7 +1144,7 @@ class BatchEngine {
     const int r0 = m_round;
     int next = r0;  // the round whose report is awaited
     for (;;) {
-      while (m_round - next < kAhead) step();
+      while (m_round - next < kAhead) advance(std::max(1, m_graph_rounds));
       const int64_t *e = m_status + 2 * (next & (BSTATUS - 1));
       uint64_t alive = 0;
       for (int64_t spin = 0;; ++spin) {
@@ -1142,6 +1168,8 @@ class BatchEngine {
     }
   }
   static constexpr int kAhead = 3;
+  // rounds enqueued past the awaited report stay within the status ring
+  static_assert(kAhead + kMaxGraphRounds < BSTATUS, "status ring too small");
 
   // Result of a lane into a device buffer of V int32 (BFS depth, -1
   // unreached; SSSP distance, 2147383647 unreached).
@@ -1185,14 +1213,70 @@ class BatchEngine {
     return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + NT - 1) / NT));
   }
 
+  // Depths are 16-bit per lane: only lanes still running can outgrow them
+  // in the next `rounds` rounds (a finished lane kept for its result does
+  // not); the host's view of which lanes run is refreshed when one
+  // reaches the limit.
+  void checkDepth(int rounds) {
+    if (m_kind != BFS_KIND) return;
+    auto too_deep = [&](uint64_t lanes) {
+      for (int s = 0; s < BW; ++s)
+        if (((lanes >> s) & 1ull) && m_round + rounds - m_hstart[s] > 65534) return true;
+      return false;
+    };
+    if (too_deep(m_used & m_alive_hint) && too_deep(lanesAlive()))
+      throw Error(GAS_ERR_STATE, "batched BFS deeper than 65534 levels");
+  }
+
+  // The launches of one round (what the round graph captures).
+  void enqueueRound() {
+    if (m_kind == BFS_KIND) {
+      launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
+      if (m_dev.pull_div > 0) launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev);
+      launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev);
+    } else {
+      launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
+      launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
+    }
+    launch(kb_list, SCAN_BLOCKS, m_stream, m_dev, 1);
+    if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
+  }
+
+  // m_graph_rounds rounds by one launch of the captured graph (captured at
+  // first use).
+  void replay() {
+    checkDepth(m_graph_rounds);
+    if (!m_gexec) {
+      check(gasrt_capture_begin(m_stream), "capture");
+      try {
+        for (int i = 0; i < m_graph_rounds; ++i) enqueueRound();
+      } catch (...) {
+        gasrt_graph_exec_t x = nullptr;
+        if (gasrt_capture_end(m_stream, &x) == GAS_OK) gasrt_graph_destroy(x);
+        throw;
+      }
+      check(gasrt_capture_end(m_stream, &m_gexec), "capture end");
+    }
+    check(gasrt_graph_launch(m_gexec, m_stream), "graph launch");
+    m_round += m_graph_rounds;
+  }
+
+  void dropGraph() {
+    if (!m_gexec) return;
+    check(gasrt_stream_synchronize(m_stream), "sync");
+    check(gasrt_graph_destroy(m_gexec), "graph destroy");
+    m_gexec = nullptr;
+  }
+
   Kind m_kind;
   gasrt_stream_t m_stream = nullptr;
   gasrt_event_t m_ev = nullptr;
+  gasrt_graph_exec_t m_gexec = nullptr;  // captured rounds (m_graph_rounds)
+  int m_graph_rounds = 0;
   BatchCtl *m_hctl = nullptr;  // [0] synchronous reads, [1] snapshots
   int32_t m_nV = 0, m_nE = 0;
   const int32_t *m_hnew = nullptr;  // caller's vertex -> engine graph's (host)
-  int m_par = 0;
-  int m_round = 0;
+  int m_round = 0;  // rounds enqueued; the control block's round follows it
   int64_t *m_status = nullptr;  // mapped round reports (BSTATUS pairs)
   uint64_t m_used = 0;
   int32_t m_hstart[BW] = {};
@@ -1204,11 +1288,11 @@ class BatchEngine {
   DevBuf<int32_t> m_list, m_bsum, m_lanes, m_sources,
       m_lane_start, m_dist, m_tmp;
   DevBuf<int64_t> m_esum;
-  DevBuf<uint64_t> m_lor, m_vn, m_front0, m_front1, m_chg, m_chg_next,
+  DevBuf<uint64_t> m_lor, m_vn, m_front, m_chg, m_chg_next,
       m_hub_mask;
   DevBuf<int4> m_hub;
   DevBuf<uint16_t> m_depth;
-  DevBuf<uint8_t> m_sl_next, m_sl0, m_sl1;  // slab activity flags
+  DevBuf<uint8_t> m_sl_next, m_sl_cur;  // slab activity flags
   bool m_may_hub = true;  // some vertex has more than hub_items items
   int64_t m_max_out = 0;  // largest out-degree
 };

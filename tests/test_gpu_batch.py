@@ -378,3 +378,56 @@ def test_batch_degree_order(gpu, kind):
             assert rel_l1(out[0][lane], ref) <= 1e-5
             continue
         assert np.array_equal(out[0][lane], want), lane
+
+
+@pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP])
+@pytest.mark.parametrize("graph", ["road", "rmat"])
+def test_batch_round_graph_replay(gpu, kind, graph):
+    # rounds issued as replays of a captured graph of 1, 2 or 4 rounds give
+    # the results of direct launches, exact per lane: one capture is replayed
+    # many times, lanes join and leave between replays (the lanes in use and
+    # the round live in device memory), and pull / distance-bound settings
+    # recapture it
+    if graph == "road":
+        el = workloads.road(96, 96, seed=31)
+    else:
+        el = workloads.rmat(13, symmetrize=True, weights=True)
+    s, d, w = el.arrays()
+    n = el.n_vertices
+    g = gas.Graph(el)
+    src = _sources(np.random.default_rng(12), el, 48)
+
+    def want(x):
+        return (oracle.check_bfs(n, s, d, int(x)) if kind == gas.BATCH_BFS
+                else oracle.check_dijkstra(n, s, d, w, int(x)))
+
+    ref = {i: want(x) for i, x in enumerate(src)}
+    for k in (1, 2, 4):
+        b = gas.Batch(g, kind)
+        b.set_graph_rounds(k)
+        if kind == gas.BATCH_BFS:
+            b.set_pull_divisor(8 if k == 2 else 0)
+        else:
+            b.set_sssp_delta(16 if k == 2 else 0)
+        # a whole batch through runToCompletion's replays
+        b.add(np.arange(32), src[:32])
+        b.run()
+        res = b.results(range(32))
+        for lane in range(32):
+            assert np.array_equal(res[lane], ref[lane]), (k, lane)
+        b.release((1 << 32) - 1)
+        # continuous: lanes join between replays, a running lane is released
+        b.add(np.arange(8), src[32:40])
+        b.step(k)
+        b.add([8, 9], src[40:42])
+        b.release(1 << 3)  # released while running: its result is dropped
+        b.step(2 * k)
+        b.add([10], [src[42]])
+        while b.alive():
+            b.step(k)
+        res = b.results([0, 1, 2, 4, 5, 6, 7, 8, 9, 10])
+        for lane, qi in ((0, 32), (1, 33), (2, 34), (4, 36), (5, 37), (6, 38),
+                         (7, 39), (8, 40), (9, 41), (10, 42)):
+            assert np.array_equal(res[lane], ref[qi]), (k, lane)
+    with pytest.raises(gas.GasError):
+        gas.Batch(g, kind).set_graph_rounds(9)

@@ -371,6 +371,17 @@ class Graph:
         _check(lib().gas_graph_batch_hub_items(self._h, C.c_int(kind), C.byref(p)))
         return p.value
 
+    def batch_graph_rounds(self, kind: int) -> int:
+        """Rounds per replay of the captured round graph the batch engines
+        on this graph start with (0: direct launches, the default)."""
+        p = C.c_int32(0)
+        _check(lib().gas_graph_batch_graph_rounds(self._h, C.c_int(kind), C.byref(p)))
+        return p.value
+
+    def set_batch_graph_rounds(self, kind: int, rounds: int) -> None:
+        _check(lib().gas_graph_set_batch_graph_rounds(self._h, C.c_int(kind),
+                                                      C.c_int32(rounds)))
+
     def set_push(self, program: int, enable: bool) -> None:
         """Push supersteps of whole SSSP / CC runs (same results and
         superstep counts as the pull)."""
@@ -622,6 +633,11 @@ class Batch:
         """List vertices with more push items than this have their items
         written by a whole workgroup (>= 16)."""
         _check(lib().gas_batch_set_hub_items(self._h, C.c_int32(items)))
+
+    def set_graph_rounds(self, rounds: int) -> None:
+        """BFS / SSSP: issue rounds as replays of a captured graph of
+        `rounds` rounds (1..8); 0 launches each round directly."""
+        _check(lib().gas_batch_set_graph_rounds(self._h, C.c_int32(rounds)))
 
     def set_sssp_delta(self, delta: int) -> None:
         """SSSP lanes relax a vertex only while its distance is below

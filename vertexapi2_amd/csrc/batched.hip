@@ -83,12 +83,10 @@ int gas_batch_add(gas_batch_t b, const int32_t *lanes, const int32_t *sources,
 int gas_batch_step(gas_batch_t b, int32_t rounds) {
   BATCH_OR_FAIL(b);
   return guarded_b([&] {
-    for (int i = 0; i < rounds; ++i) {
-      if (b->e)
-        b->e->step();
-      else
-        b->pp->step();
-    }
+    if (b->e)
+      b->e->advance(rounds);
+    else
+      for (int i = 0; i < rounds; ++i) b->pp->step();
   });
 }
 
@@ -109,11 +107,12 @@ int gas_batch_used(gas_batch_t b, uint64_t *used) {
 
 int gas_batch_release(gas_batch_t b, uint64_t lanes) {
   BATCH_OR_FAIL(b);
-  if (b->e)
-    b->e->releaseLanes(lanes);
-  else
-    b->pp->releaseLanes(lanes);
-  return GAS_OK;
+  return guarded_b([&] {
+    if (b->e)
+      b->e->releaseLanes(lanes);
+    else
+      b->pp->releaseLanes(lanes);
+  });
 }
 
 int gas_batch_run(gas_batch_t b, gas_run_stats *stats) {
@@ -213,6 +212,31 @@ int gas_graph_batch_hub_items(gas_graph_t g, int kind, int32_t *items) {
     return set_error(GAS_ERR_ARG, "gas_graph_batch_hub_items: bad arguments");
   *items = g->batch_hub[kind] ? g->batch_hub[kind] : HUB_ITEMS;
   return GAS_OK;
+}
+
+int gas_graph_batch_graph_rounds(gas_graph_t g, int kind, int32_t *rounds) {
+  if (!g || !rounds || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_graph_batch_graph_rounds: bad arguments");
+  *rounds = g->batch_graph[kind];
+  return GAS_OK;
+}
+
+int gas_graph_set_batch_graph_rounds(gas_graph_t g, int kind, int32_t rounds) {
+  if (!g || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP) || rounds < 0 ||
+      rounds > BatchEngine::kMaxGraphRounds)
+    return set_error(GAS_ERR_ARG, "gas_graph_set_batch_graph_rounds: bad arguments");
+  g->batch_graph[kind] = rounds;
+  return GAS_OK;
+}
+
+int gas_batch_set_graph_rounds(gas_batch_t b, int32_t rounds) {
+  BATCH_OR_FAIL(b);
+  if (!b->e)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_graph_rounds: not a BFS / SSSP batch");
+  if (rounds < 0 || rounds > BatchEngine::kMaxGraphRounds)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_graph_rounds: need 0..%d",
+                     BatchEngine::kMaxGraphRounds);
+  return guarded_b([&] { b->e->setGraphRounds(rounds); });
 }
 
 int gas_batch_set_hub_items(gas_batch_t b, int32_t items) {

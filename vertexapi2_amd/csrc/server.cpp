@@ -485,13 +485,13 @@ struct Session {
         }
         e.lane_query[lane] = -1;
         e.running &= ~(1ull << lane);
-        if (e.be)
-          e.be->releaseLanes(1ull << lane);
-        else
-          e.pe->releaseLanes(1ull << lane);
         ++done;
         finished(i, (tf - q[i].arrival) * 1e3);
       }
+      if (e.be)
+        e.be->releaseLanes(fin_all);
+      else
+        e.pe->releaseLanes(fin_all);
       // state the engine grew while extracting (single-lane scratch)
       const size_t now_b = e.stateBytes() + [&] {
         size_t b = 0;
