@@ -431,6 +431,10 @@ int gas_batch_results_device(gas_batch_t b, uint64_t lanes,
 /* PPR lanes: V float to host memory / to device memory. */
 int gas_batch_result_f32(gas_batch_t b, int32_t lane, float *host_out);
 int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out);
+/* PPR lanes in `lanes` into dev_outs[lane] (V float each, device), in one
+ * pass over the rank rows (only the 64-byte sectors holding those lanes). */
+int gas_batch_results_f32_device(gas_batch_t b, uint64_t lanes,
+                                 float *const *dev_outs);
 /* Push-versus-pull switch of batched BFS: pull when the frontier's
  * out-edges times `divisor` reach |E| (default 0 = never pull: with 64
  * lanes a pull rarely stops early, measured slower on every R-MAT round). */

@@ -806,16 +806,32 @@ static __global__ __launch_bounds__(NT) void kb_bfs_extract_many(
 // vertex); a workgroup reads 64 rows coalesced into LDS and writes each
 // requested lane's 64 consecutive vertices coalesced, so the array is read
 // once for any number of lanes.
-static __global__ __launch_bounds__(NT) void kb_sssp_extract_many(
-    BatchDev d, uint64_t lanes, LaneOutputs outs) {
-  __shared__ int32_t tile[64][BW + 1];
+// Rows hold 64 four-byte lanes: 16 lanes per 64-byte sector.  The lanes of
+// the sectors holding some requested lane.
+__host__ __device__ inline uint64_t lane_sectors(uint64_t lanes) {
+  uint64_t m = 0;
+  for (int q = 0; q < 4; ++q)
+    if ((lanes >> (16 * q)) & 0xffffull) m |= 0xffffull << (16 * q);
+  return m;
+}
+
+// Lane-parallel rows of 64 vertices into LDS, only the sectors of requested
+// lanes, then each requested lane's 64 consecutive vertices written
+// coalesced: the rows are read once for any number of lanes.
+template <class T, class Outs>
+__device__ inline void extract_rows(const T *rows, const int32_t *new_of,
+                                    int32_t nV, uint64_t lanes, const Outs &outs) {
+  __shared__ T tile[64][BW + 1];
   const int w = threadIdx.x / WAVE, lane = lane_id();
-  for (int v0 = blockIdx.x * 64; v0 < d.nV; v0 += gridDim.x * 64) {
-    const int nv = min(64, d.nV - v0);
+  const uint64_t sect = lane_sectors(lanes);
+  for (int v0 = blockIdx.x * 64; v0 < nV; v0 += gridDim.x * 64) {
+    const int nv = min(64, nV - v0);
     for (int i = threadIdx.x; i < 64 * BW; i += NT) {
       const int j = i / BW, s = i % BW;
-      const int u = j < nv ? (d.new_of ? d.new_of[v0 + j] : v0 + j) : 0;
-      tile[j][s] = j < nv ? d.dist[(size_t)u * BW + s] : 0;
+      if (j < nv && ((sect >> s) & 1ull)) {
+        const int u = new_of ? new_of[v0 + j] : v0 + j;
+        tile[j][s] = rows[(size_t)u * BW + s];
+      }
     }
     __syncthreads();
     int r = 0;
@@ -826,6 +842,11 @@ static __global__ __launch_bounds__(NT) void kb_sssp_extract_many(
     }
     __syncthreads();
   }
+}
+
+static __global__ __launch_bounds__(NT) void kb_sssp_extract_many(
+    BatchDev d, uint64_t lanes, LaneOutputs outs) {
+  extract_rows(d.dist, d.new_of, d.nV, lanes, outs);
 }
 
 // ---- host side -----------------------------------------------------------------
@@ -960,7 +981,6 @@ class BatchEngine {
   ~BatchEngine() {
     if (m_stream) gasrt_stream_synchronize(m_stream);
     if (m_gexec) gasrt_graph_destroy(m_gexec);
-    if (m_ev) gasrt_event_destroy(m_ev);
     gasrt_host_free(m_hctl);
     gasrt_host_free((void *)m_status);
     gasrt_stream_destroy(m_stream);
@@ -1093,29 +1113,32 @@ class BatchEngine {
     }
   }
 
-  // step() followed by an asynchronous snapshot of the control block and an
-  // event; poll() tells when both are done, aliveSnapshot() reads it.
+  // `rounds` rounds without waiting; poll() tells when the last of them has
+  // reported, aliveSnapshot() reads its report.  Each round's list pass
+  // stores its report (tag, lanes alive) into the host-mapped status ring,
+  // so a poll reads host memory: no copy or event joins the engine's stream.
   void stepAsync(int rounds = 1) {
     advance(rounds);
-    snapshotAsync();
-  }
-  void snapshotAsync() {
     m_added_since_snap = 0;
-    if (!m_ev) check(gasrt_event_create(&m_ev), "event");
-    check(gasrt_memcpy_d2h_async(&m_hctl[1], m_ctl.p, sizeof(BatchCtl),
-                                 m_stream),
-          "ctl snapshot");
-    check(gasrt_event_record(m_ev, m_stream), "event");
   }
   bool poll() {
-    int done = 1;
-    if (m_ev) check(gasrt_event_query(m_ev, &done), "event query");
-    return done != 0;
+    if (m_round == 0) return true;
+    const int64_t *e = m_status + 2 * ((m_round - 1) & (BSTATUS - 1));
+    if (__atomic_load_n(e, __ATOMIC_ACQUIRE) == (int64_t)m_round) return true;
+    int idle = 0;
+    check(gasrt_stream_query(m_stream, &idle), "stream query");
+    if (idle && __atomic_load_n(e, __ATOMIC_ACQUIRE) != (int64_t)m_round)
+      throw Error(GAS_ERR_STATE, "batch round status was not reported");
+    return false;
   }
+  // Lanes alive after the last round enqueued (poll() returned true).
   uint64_t aliveSnapshot() {
-    // lanes added after the snapshot was taken are not in it
-    m_alive_hint &= m_hctl[1].lanes_alive | m_added_since_snap;
-    return m_hctl[1].lanes_alive & m_used;
+    const int64_t *e = m_status + 2 * ((m_round - 1) & (BSTATUS - 1));
+    const uint64_t alive =
+        m_round ? (uint64_t)__atomic_load_n(e + 1, __ATOMIC_RELAXED) : m_alive_hint;
+    // lanes added after that round was enqueued are not in it
+    m_alive_hint &= alive | m_added_since_snap;
+    return alive & m_used;
   }
 
   // Lanes whose query still has work (reads the device; synchronises).
@@ -1270,7 +1293,6 @@ class BatchEngine {
 
   Kind m_kind;
   gasrt_stream_t m_stream = nullptr;
-  gasrt_event_t m_ev = nullptr;
   gasrt_graph_exec_t m_gexec = nullptr;  // captured rounds (m_graph_rounds)
   int m_graph_rounds = 0;
   BatchCtl *m_hctl = nullptr;  // [0] synchronous reads, [1] snapshots
@@ -1281,7 +1303,7 @@ class BatchEngine {
   uint64_t m_used = 0;
   int32_t m_hstart[BW] = {};
   uint64_t m_alive_hint = 0;        // lanes not yet seen finished by the host
-  uint64_t m_added_since_snap = 0;  // lanes added after the last snapshotAsync
+  uint64_t m_added_since_snap = 0;  // lanes added after the last stepAsync
   BatchDev m_dev{};
   DevBuf<BatchCtl> m_ctl;
   DevBuf<int4> m_item;
@@ -1454,6 +1476,17 @@ static __global__ __launch_bounds__(NT) void kp_extract(PprDev d, int p, int s,
                                                  float *out) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT)
     out[v] = d.rank[p][(size_t)(d.new_of ? d.new_of[v] : v) * BW + s];
+}
+
+struct LaneOutputsF {
+  float *p[BW];
+};
+
+// Ranks of every lane in `lanes`, lane s into outs.p[s], in one pass.
+static __global__ __launch_bounds__(NT) void kp_extract_many(PprDev d, int p,
+                                                      uint64_t lanes,
+                                                      LaneOutputsF outs) {
+  extract_rows(d.rank[p], d.new_of, d.nV, lanes, outs);
 }
 
 class PprBatch {
@@ -1639,6 +1672,20 @@ class PprBatch {
   void laneResultDevice(int lane, float *dev_out) {
     if (lane < 0 || lane >= BW) throw Error(GAS_ERR_ARG, "lane out of range");
     launch(kp_extract, gridFor(m_nV), m_stream, m_dev, m_par, lane, dev_out);
+  }
+
+  // Ranks of every lane in `lanes` into outs[lane] (device, V floats
+  // each), in one pass over the rank rows.
+  void laneResultsDevice(uint64_t lanes, float *const *outs) {
+    if (!lanes) return;
+    LaneOutputsF o{};
+    for (uint64_t m = lanes; m; m &= m - 1) {
+      const int s = __builtin_ctzll(m);
+      if (!outs[s]) throw Error(GAS_ERR_ARG, "null lane output");
+      o.p[s] = outs[s];
+    }
+    launch(kp_extract_many, (unsigned)std::min<int64_t>(2048, (m_nV + 63) / 64),
+           m_stream, m_dev, m_par, lanes, o);
   }
 
   void laneResult(int lane, float *host_out) {

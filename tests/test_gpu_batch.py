@@ -58,6 +58,11 @@ def test_batch_rmat_hubs(gpu, kind):
         assert np.array_equal(many[lane], want), lane
     for lane in (0, 17, 63):
         assert np.array_equal(b.result(lane), many[lane])
+    # lanes of one, two and four sectors of the distance rows
+    for lanes in ([7], [16, 31], [1, 20, 33, 50]):
+        part = b.results(lanes)
+        for lane in lanes:
+            assert np.array_equal(part[lane], many[lane]), lane
 
 
 @pytest.mark.parametrize("divisor", [0, 1, 4, 64])
@@ -159,6 +164,12 @@ def test_ppr_plain_and_batched_vs_fp64(gpu):
     for lane in (0, 1, 17, 63):
         ref = oracle.check_ppr_fp64(n, s, d, int(src[lane]), 10)
         assert rel_l1(b.result(lane), ref) <= 1e-5, lane
+    # several lanes in one pass (lanes in one, two and four 64-byte sectors
+    # of the rank rows) equal the single-lane extraction bit for bit
+    for lanes in ([5], [0, 15], [3, 16, 40, 63], list(range(64))):
+        many = b.results(lanes)
+        for lane in lanes:
+            assert np.array_equal(many[lane], b.result(lane)), lane
 
 
 def test_ppr_alone_equals_mixed_bitwise(gpu):

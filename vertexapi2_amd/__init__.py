@@ -613,6 +613,9 @@ class Batch:
         for s, b in bufs.items():
             ptrs[s] = b.ptr
             mask |= 1 << s
+        if self.kind == BATCH_PPR:
+            _check(lib().gas_batch_results_f32_device(self._h, C.c_uint64(mask), ptrs))
+            return {s: b.to_numpy(np.float32) for s, b in bufs.items()}
         _check(lib().gas_batch_results_device(self._h, C.c_uint64(mask), ptrs))
         return {s: b.to_numpy() for s, b in bufs.items()}
 

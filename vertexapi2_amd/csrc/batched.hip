@@ -184,6 +184,17 @@ int gas_batch_result_f32_device(gas_batch_t b, int32_t lane, float *dev_out) {
   });
 }
 
+int gas_batch_results_f32_device(gas_batch_t b, uint64_t lanes,
+                                 float *const *dev_outs) {
+  BATCH_OR_FAIL(b);
+  if (!dev_outs || !b->pp)
+    return set_error(GAS_ERR_ARG, "gas_batch_results_f32_device: null or not PPR");
+  return guarded_b([&] {
+    b->pp->laneResultsDevice(lanes, dev_outs);
+    b->pp->sync();
+  });
+}
+
 int gas_batch_set_pull_divisor(gas_batch_t b, int32_t divisor) {
   BATCH_OR_FAIL(b);
   if (divisor < 0)

@@ -465,14 +465,18 @@ struct Session {
       const uint64_t fin_all = e.running & ~alive;
       if (!fin_all) continue;
       int32_t *outs[BW] = {};
+      float *outs_f[BW] = {};
       for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
         const int lane = __builtin_ctzll(fin);
         if (e.be)
           outs[lane] = e.res_i[lane].p;
         else
-          e.pe->laneResultDevice(lane, e.res_f[lane].p);
+          outs_f[lane] = e.res_f[lane].p;
       }
-      if (e.be) e.be->laneResultsDevice(fin_all, outs);
+      if (e.be)
+        e.be->laneResultsDevice(fin_all, outs);
+      else
+        e.pe->laneResultsDevice(fin_all, outs_f);
       check(gasrt_stream_synchronize(e.stream()), "sync");
       const double tf = clock();
       for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
