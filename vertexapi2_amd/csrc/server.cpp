@@ -185,9 +185,22 @@ struct Engine {
   DevBuf<float> res_f[BW];
   size_t charged = 0;               // bytes of the budget this engine holds
   double last_used = 0;             // clock of its last admission
+  // extractions queued on the stream: an event after the extraction and
+  // the queries whose results it writes (done when the event is)
+  struct Extraction {
+    gasrt_event_t ev = nullptr;
+    std::vector<int> queries;
+  };
+  std::deque<Extraction> ext;
+  std::vector<gasrt_event_t> spare_ev;
+  ~Engine() {
+    if (!ext.empty()) gasrt_stream_synchronize(stream());
+    for (auto &x : ext) gasrt_event_destroy(x.ev);
+    for (auto ev : spare_ev) gasrt_event_destroy(ev);
+  }
   gasrt_stream_t stream() const { return be ? be->stream() : pe->stream(); }
   size_t stateBytes() const { return be ? be->deviceBytes() : pe->deviceBytes(); }
-  bool idle() const { return !running && !inflight && add_lanes.empty(); }
+  bool idle() const { return !running && !inflight && add_lanes.empty() && ext.empty(); }
   bool hasResult(int lane) const {
     return be ? res_i[lane].n >= (size_t)V : res_f[lane].n >= (size_t)V;
   }
@@ -447,14 +460,35 @@ struct Session {
     for (int w = 0; w < kSoloThreads; ++w)
       if (workers[w].pending > 0) return true;
     for (auto &ep : s->engines)
-      if (ep && (ep->running || ep->inflight || !ep->add_lanes.empty())) return true;
+      if (ep && !ep->idle()) return true;
     return false;
   }
 
   // One pass of the loop; true if anything happened.
   bool iterate() {
     bool progressed = false;
-    // 1. engines whose round finished: retire the queries that ended
+    // 0. extractions whose results are in HBM: their queries are done
+    for (auto &ep : s->engines) {
+      if (!ep) continue;
+      Engine &e = *ep;
+      while (!e.ext.empty()) {
+        int ok = 0;
+        check(gasrt_event_query(e.ext.front().ev, &ok), "event query");
+        if (!ok) break;
+        const double tf = clock();
+        for (int i : e.ext.front().queries) {
+          ++done;
+          finished(i, (tf - q[i].arrival) * 1e3);
+        }
+        e.spare_ev.push_back(e.ext.front().ev);
+        e.ext.pop_front();
+        progressed = true;
+      }
+    }
+    // 1. engines whose round finished: extract the results of the queries
+    // that ended and free their lanes (stream order keeps the extraction
+    // ahead of any reuse of the lanes); without a host copy to make, the
+    // loop does not wait for the extraction
     for (auto &ep : s->engines) {
       if (!ep || !ep->inflight) continue;
       Engine &e = *ep;
@@ -477,7 +511,22 @@ struct Session {
         e.be->laneResultsDevice(fin_all, outs);
       else
         e.pe->laneResultsDevice(fin_all, outs_f);
-      check(gasrt_stream_synchronize(e.stream()), "sync");
+      bool to_host = false;
+      for (uint64_t fin = fin_all; fin; fin &= fin - 1)
+        to_host |= q[e.lane_query[__builtin_ctzll(fin)]].host_out != nullptr;
+      Engine::Extraction x;
+      if (to_host) {
+        check(gasrt_stream_synchronize(e.stream()), "sync");
+      } else {
+        if (e.spare_ev.empty()) {
+          gasrt_event_t ev = nullptr;
+          check(gasrt_event_create(&ev), "event");
+          e.spare_ev.push_back(ev);
+        }
+        x.ev = e.spare_ev.back();
+        e.spare_ev.pop_back();
+        check(gasrt_event_record(x.ev, e.stream()), "event");
+      }
       const double tf = clock();
       for (uint64_t fin = fin_all; fin; fin &= fin - 1) {
         const int lane = __builtin_ctzll(fin);
@@ -489,9 +538,14 @@ struct Session {
         }
         e.lane_query[lane] = -1;
         e.running &= ~(1ull << lane);
-        ++done;
-        finished(i, (tf - q[i].arrival) * 1e3);
+        if (to_host) {
+          ++done;
+          finished(i, (tf - q[i].arrival) * 1e3);
+        } else {
+          x.queries.push_back(i);
+        }
       }
+      if (!to_host) e.ext.push_back(std::move(x));
       if (e.be)
         e.be->releaseLanes(fin_all);
       else
@@ -671,7 +725,7 @@ struct Session {
     for (int w = 0; w < kSoloThreads; ++w)
       if (workers[w].pending > 0) return true;
     for (auto &ep : s->engines)
-      if (ep && ep->inflight) return true;
+      if (ep && (ep->inflight || !ep->ext.empty())) return true;
     return false;
   }
 
