@@ -1345,12 +1345,14 @@ struct PprDev {
   uint8_t *nz[3];                 // [V] some running lane is nonzero, by
                                   // sweep mod 3 (read / written / cleared)
   int32_t rot;                    // sweep mod 3: nz[rot] is this sweep's input
+  int32_t *nzc;                   // [3] bytes set in nz[q] (vertices written
+                                  // nonzero by a sweep; seeds not counted)
   float *part_s, *part_c;         // [slots][BW]
   const int32_t *src_lane;        // [BW]
   const int32_t *new_of;          // caller's vertex -> the engine graph's, or NULL
 };
 
-__device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
+__device__ inline bool ppr_write(const PprDev &d, int p, int v, uint64_t run,
                                  float sum) {
   const int lane = lane_id();
   const bool on = (run >> lane) & 1ull;
@@ -1361,6 +1363,7 @@ __device__ inline void ppr_write(const PprDev &d, int p, int v, uint64_t run,
   d.contrib[p ^ 1][(size_t)v * BW + lane] = od > 0 ? val / (float)od : 0.0f;
   const uint64_t m = __ballot(val != 0.0f);
   if (lane == 0 && m) d.nz[(d.rot + 1) % 3][v] = 1;
+  return m != 0;
 }
 
 static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
@@ -1368,6 +1371,11 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
   const int waves = gridDim.x * (NT / WAVE);
   const float *ct = d.contrib[p];
   const uint8_t *nzin = d.nz[d.rot];
+  // an input with most rows nonzero is read without the byte test (a zero
+  // row adds +0.0 to a lane: its sum and compensation stay as they are)
+  const bool dense = 2 * (int64_t)d.nzc[d.rot] > (int64_t)d.nV;
+  int written = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.nzc[(d.rot + 2) % 3] = 0;
   // the byte array the next sweep writes, cleared while this one reads nz[rot]
   {
     uint32_t *clr = reinterpret_cast<uint32_t *>(d.nz[(d.rot + 2) % 3]);
@@ -1387,7 +1395,7 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
       bool f_l = false;
       if (k < e1) {
         u_l = d.csc_src[k];
-        f_l = nzin[u_l] != 0;
+        f_l = dense || nzin[u_l] != 0;
       }
       uint64_t nz = __ballot(f_l);
       const bool mine = (run >> lane) & 1ull;
@@ -1419,17 +1427,19 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
     }
     const int ps = d.pslot[it];
     if (ps < 0) {
-      ppr_write(d, p, v, run, s + c);
+      written += ppr_write(d, p, v, run, s + c);
     } else {
       d.part_s[(size_t)ps * BW + lane] = s;
       d.part_c[(size_t)ps * BW + lane] = c;
     }
   }
+  if (lane_id() == 0 && written) atomicAdd(&d.nzc[(d.rot + 1) % 3], written);
 }
 
 static __global__ __launch_bounds__(NT) void kp_fix(PprDev d, int p, uint64_t run) {
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
+  int written = 0;
   for (int i = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE; i < d.n_multi;
        i += waves) {
     const int v = d.mv[i];
@@ -1443,8 +1453,9 @@ static __global__ __launch_bounds__(NT) void kp_fix(PprDev d, int p, uint64_t ru
       c = c + bc + ((s - (t - bp)) + (bs - bp));
       s = t;
     }
-    ppr_write(d, p, v, run, s + c);
+    written += ppr_write(d, p, v, run, s + c);
   }
+  if (lane == 0 && written) atomicAdd(&d.nzc[(d.rot + 1) % 3], written);
 }
 
 static __global__ __launch_bounds__(NT) void kp_reset(PprDev d, int p,
@@ -1563,6 +1574,9 @@ class PprBatch {
     d.contrib[0] = m_contrib0.p;
     d.contrib[1] = m_contrib1.p;
     for (int i = 0; i < 3; ++i) d.nz[i] = m_nz[i].p;
+    m_nzc.alloc(3);
+    check(gasrt_memset(m_nzc.p, 0, 3 * sizeof(int32_t), m_stream), "memset");
+    d.nzc = m_nzc.p;
     d.rot = 0;
     d.part_s = m_ps.p;
     d.part_c = m_pc.p;
@@ -1585,7 +1599,7 @@ class PprBatch {
                m_contrib0.bytes() + m_contrib1.bytes() + m_ps.bytes() + m_pc.bytes() +
                m_tmp.bytes();
     for (const auto &z : m_nz) b += z.bytes();
-    return b;
+    return b + m_nzc.bytes();
   }
   uint64_t lanesUsed() const { return m_used; }
   int iterations() const { return m_iters; }
@@ -1715,6 +1729,7 @@ class PprBatch {
       m_lanes, m_sources;
   DevBuf<float> m_rank0, m_rank1, m_contrib0, m_contrib1, m_ps, m_pc, m_tmp;
   DevBuf<uint8_t> m_nz[3];
+  DevBuf<int32_t> m_nzc;
 };
 
 }  // namespace gas
