@@ -426,9 +426,8 @@ static __global__ void kb_bfs_seed(BatchDev d, const int32_t *lanes,
 // and one lane per out-edge; all their loads are issued before any use.
 // New bits (frontier mask minus the destination's visited bits) are OR-ed
 // into the destination's next mask.
-static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d) {
+__device__ __forceinline__ void bfs_push(const BatchDev &d) {
   const BatchCtl *c = d.ctl;
-  if (c->pull) return;
   const int n_items = c->n_items;
   const uint64_t used = c->used;
   const int lane = lane_id();
@@ -470,9 +469,8 @@ static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d) {
     }
   }
 }
-static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d) {
+__device__ __forceinline__ void bfs_pull(const BatchDev &d) {
   const BatchCtl *c = d.ctl;
-  if (!c->pull) return;
   // a finished lane cannot reach more vertices
   const uint64_t used = c->used & c->lanes_alive;
   const uint64_t *fr = d.front;
@@ -535,8 +533,20 @@ static __global__ __launch_bounds__(NT) void kb_bfs_pull(BatchDev d) {
   }
 }
 
-// New bits become visited and form the next frontier (parity p^1); depth
-// rows get round + 1 - lane_start.  Contiguous chunks, with the list
+// A round's traversal: push from the list, or (engines that may pull) the
+// direction the list pass chose, in one launch.
+static __global__ __launch_bounds__(NT) void kb_bfs_push(BatchDev d) {
+  if (!d.ctl->pull) bfs_push(d);
+}
+static __global__ __launch_bounds__(NT) void kb_bfs_step(BatchDev d) {
+  if (d.ctl->pull)
+    bfs_pull(d);
+  else
+    bfs_push(d);
+}
+
+// New bits become visited and form the next frontier; depth rows get
+// round + 1 - lane_start.  Contiguous chunks, with the list
 // counters of the new frontier (pass 1 of kb_list).
 static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d) {
   __shared__ int lds[NT / WAVE];
@@ -1254,8 +1264,10 @@ class BatchEngine {
   // The launches of one round (what the round graph captures).
   void enqueueRound() {
     if (m_kind == BFS_KIND) {
-      launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
-      if (m_dev.pull_div > 0) launch(kb_bfs_pull, SCAN_BLOCKS, m_stream, m_dev);
+      if (m_dev.pull_div > 0)
+        launch(kb_bfs_step, 4 * SCAN_BLOCKS, m_stream, m_dev);
+      else
+        launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev);
     } else {
       launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
