@@ -206,6 +206,7 @@ def roofline_table(stats_list, models_list):
 REGIONS = {"road1024": 1, "grid64_bfs": 2, "rmat21_pagerank20": 3, "rmat21s_cc": 4,
            "rmat22s_batched_bfs": 5, "rmat22s_batched_sssp": 6,
            "road1024_batched_bfs": 7, "road1024_batched_sssp": 8, "copy_calibration": 9}
+REGIONS.update({f"served_{r}": 10 + i for i, r in enumerate(SERVED_RATES)})
 
 
 def mark(region, end=False):
@@ -595,7 +596,12 @@ def served_lines(graphs, edge_lists=None, rates=SERVED_RATES, mix=(0.5, 0.3, 0.2
             ppr = np.nonzero(st["kinds"] == gas.BATCH_PPR)[0]
             keep = [int(q) for gi in range(len(graphs))
                     for q in ppr[st["graph_ids"][ppr] == gi][:2 if gi == 0 else 1]]
+        region = f"served_{rate}" if mix == (0.5, 0.3, 0.2) else None
+        if region:
+            mark(region)
         res, lat, results = serve.serve(srv, st, keep=keep)
+        if region:
+            mark(region, end=True)
         lat_p, end_p = serve.plain_replay(graphs, st)
         plain_qps = n / end_p
         entry = {
