@@ -523,6 +523,19 @@ class GASEngineGPU {
     }
   }
 
+  // Lanes per frontier vertex in k_small_lds: the largest out-degree
+  // rounded up to a power of two (at most the wave).
+  int smallLdsGroup() const {
+    int g = 1;
+    while (g < m_max_out && g < WAVE) g <<= 1;
+    return g;
+  }
+
+  // Gather-free programs on graphs of at most SMALL_LDS_V vertices and
+  // SMALL_LDS_E edges run the small-frontier supersteps with the graph in
+  // LDS (k_small_lds; default on).
+  void setSmallLds(bool on) { m_small_lds = on; }
+
   // Largest list frontier handed to k_small (0: never).
   void setSmallFrontier(int max_vertices) {
     if (max_vertices < 0) throw Error(GAS_ERR_ARG, "small frontier must be >= 0");
@@ -757,12 +770,22 @@ class GASEngineGPU {
     m_dev.pub = 0;
     const bool timed = kernel_timing_flag() != 0;
     if (timed) timedMark(true);
-    if (haveGather)
+    if (haveGather) {
       launch_wg(k_small<Program, true>, 1, SMALL_NT, m_stream, m_dev, m_par,
                 1 << 20, m_small_max);
-    else
-      launch_wg(k_small<Program, false>, 1, SMALL_NT, m_stream, m_dev, m_par,
-                1 << 20, m_small_max);
+    } else {
+      bool lds = false;
+      if constexpr (sizeof(VertexData) <= 4)
+        lds = m_nV <= SMALL_LDS_V && m_nE <= SMALL_LDS_E && m_small_lds;
+      if constexpr (sizeof(VertexData) <= 4) {
+        if (lds)
+          launch_wg(k_small_lds<Program>, 1, SMALL_LDS_NT, m_stream, m_dev, m_par,
+                    1 << 20, m_small_max, smallLdsGroup());
+      }
+      if (!lds)
+        launch_wg(k_small<Program, false>, 1, SMALL_NT, m_stream, m_dev, m_par,
+                  1 << 20, m_small_max);
+    }
     if (timed) timedMark(false);
     const Ctl c = readCtl();  // synchronises
     const int k = c.small_steps;
@@ -1139,6 +1162,7 @@ class GASEngineGPU {
   static constexpr int kSmallGraph = 8192;  // default k_small graphs: V <= this
   bool m_other_stale = true;        // vnext does not mirror vdata
   bool m_srcval_stale = true;       // srcval does not match the state
+  bool m_small_lds = true;          // setSmallLds
   bool m_flags_dirty = true;        // a scatter ran since the last clear
   gas_graph_t m_graph = nullptr;
   gas_graph_t m_own_graph = nullptr;

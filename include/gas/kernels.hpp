@@ -1948,6 +1948,152 @@ __global__ __launch_bounds__(SMALL_NT) void k_small(Dev<P> d, int p,
   }
 }
 
+// k_small for programs without a gather on graphs small enough that the
+// out-edges, the vertex state, both lists and both activation byte arrays
+// sit in LDS (SMALL_LDS_V vertices, SMALL_LDS_E edges, 4-byte state): the
+// CSR and state are staged once, every superstep then touches LDS only
+// (apply, claim, append, two barriers), and the state, the list left and its
+// activation bytes go back to global memory at the end -- the same
+// supersteps, values, list and report as k_small.
+constexpr int SMALL_LDS_V = 4096;
+constexpr int SMALL_LDS_E = 16384;
+constexpr int SMALL_LDS_NT = 256;  // four waves: cheap barriers
+template <class P>
+__global__ __launch_bounds__(SMALL_LDS_NT) void k_small_lds(Dev<P> d, int p,
+                                                        int max_steps,
+                                                        int small_max, int g) {
+  typedef typename P::VertexData VD;
+  static_assert(sizeof(VD) <= 4, "LDS-resident small mode: 4-byte state");
+  __shared__ int32_t s_off[SMALL_LDS_V + 1];
+  __shared__ int32_t s_dst[SMALL_LDS_E];
+  __shared__ VD s_vd[SMALL_LDS_V];
+  __shared__ int32_t s_q[2][SMALL_LDS_V];
+  __shared__ __align__(4) uint8_t s_fl[2][SMALL_LDS_V];
+  // next-list counts by superstep mod 3: superstep L appends into s_next[L %
+  // 3], which every thread reads after the barrier ending L; the slot of
+  // L + 2 is cleared then (its last readers read it before that barrier)
+  __shared__ int s_n, s_next[3];
+  Ctl *c = d.ctl;
+  const int V = d.nV, E = d.nE, tid = threadIdx.x;
+  // staging: eight loads per thread in flight before their stores
+  auto stage = [&](auto *dst, const auto *src, int count) __attribute__((always_inline)) {
+    for (int b = 0; b < count; b += 8 * SMALL_LDS_NT) {
+      typename std::remove_const<typename std::remove_pointer<decltype(src)>::type>::type x[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = b + k * SMALL_LDS_NT + tid;
+        if (i < count) x[k] = src[i];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = b + k * SMALL_LDS_NT + tid;
+        if (i < count) dst[i] = x[k];
+      }
+    }
+  };
+  stage(s_off, d.csr_off, V + 1);
+  stage(s_dst, d.csr_dst, E);
+  stage(s_vd, d.vdata, V);
+  for (int i = tid; i < SMALL_LDS_V / 4; i += SMALL_LDS_NT) {
+    reinterpret_cast<uint32_t *>(s_fl[0])[i] = 0u;
+    reinterpret_cast<uint32_t *>(s_fl[1])[i] = 0u;
+  }
+  if (tid == 0) {
+    s_n = c->ring[d.seq & 3].n_active;
+    s_next[0] = s_next[1] = s_next[2] = 0;
+    s_superstep = d.iter;
+  }
+  __syncthreads();
+  int n = s_n;
+  // the input list, its activation bytes cleared in global memory (the
+  // supersteps below keep theirs in LDS)
+  for (int i = tid; i < n; i += SMALL_LDS_NT) {
+    const int v = d.q[p][i];
+    s_q[0][i] = v;
+    d.fl[p][v] = 0;
+  }
+  __syncthreads();
+  int cur = 0, steps = 0;
+  int64_t scattered = 0, active = 0, returned = 0;
+  while (n > 0 && n <= small_max && steps < max_steps) {
+    int *nx = &s_next[steps % 3];
+    // g lanes per frontier vertex (g >= its out-degree, a power of two up to
+    // the wave): the group's first lane applies, every lane takes one
+    // out-edge, and the wave reserves its appends with one atomic
+    for (int b = 0; b < n; b += SMALL_LDS_NT / g) {
+      const int i = b + tid / g, lig = tid & (g - 1);
+      const bool valid = i < n;
+      int v = 0;
+      bool ret = false;
+      if (valid) {
+        v = s_q[cur][i];
+        if (lig == 0) {
+          s_fl[cur][v] = 0;
+          ++active;
+          VD x = s_vd[v];
+          ret = P::apply(&x, P::gatherZero);
+          s_vd[v] = x;
+        }
+      }
+      ret = __shfl(ret ? 1 : 0, lane_id() & ~(g - 1), WAVE) != 0;
+      int u = -1;
+      if (ret) {
+        const int o = s_off[v], o1 = s_off[v + 1];
+        if (lig == 0) {
+          ++returned;
+          scattered += o1 - o;
+        }
+        if (o + lig < o1) u = s_dst[o + lig];
+      }
+      const bool cl = u >= 0 && claim_vertex(s_fl[cur ^ 1], u);
+      const uint64_t m = __ballot(cl);
+      if (m) {
+        const int lead = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane_id() == lead) base = atomicAdd(nx, __popcll(m));
+        base = __shfl(base, lead, WAVE);
+        if (cl) s_q[cur ^ 1][base + __popcll(m & ((1ull << lane_id()) - 1))] = u;
+      }
+    }
+    __syncthreads();
+    n = *nx;
+    if (tid == 0) {
+      s_next[(steps + 2) % 3] = 0;
+      s_superstep = d.iter + steps + 1;
+    }
+    __syncthreads();
+    cur ^= 1;
+    ++steps;
+  }
+  // state, the list left and its activation bytes back to global memory,
+  // in the parity the grid kernels continue with
+  const int pf = p ^ (steps & 1);
+  for (int i = tid; i < V; i += SMALL_LDS_NT) d.vdata[i] = s_vd[i];
+  for (int i = tid; i < n; i += SMALL_LDS_NT) {
+    const int v = s_q[cur][i];
+    d.q[pf][i] = v;
+    d.fl[pf][v] = 1;
+  }
+  __shared__ unsigned long long s_stat[4];
+  if (tid < 4) s_stat[tid] = 0;
+  __syncthreads();
+  if (scattered) atomicAdd(&s_stat[1], (unsigned long long)scattered);
+  if (active) atomicAdd(&s_stat[2], (unsigned long long)active);
+  if (returned) atomicAdd(&s_stat[3], (unsigned long long)returned);
+  __syncthreads();
+  if (tid < 4 && s_stat[tid])
+    atomicAdd(reinterpret_cast<unsigned long long *>(d.stat_acc + tid * SCAN_BLOCKS),
+              s_stat[tid]);
+  if (tid < 4 * STEP_COUNT_WORDS) reinterpret_cast<int32_t *>(c->ring)[tid] = 0;
+  __syncthreads();
+  if (tid == 0) {
+    c->ring[(d.seq + steps) & 3].n_active = n;
+    c->small_steps = steps;
+    c->small_n = n;
+    c->small_empty = n == 0;
+  }
+}
+
 // Ordered compaction of the activation bytes of the next parity into the
 // next list (replaces scatter_if_inputloc_twophase, scatter_if_mgpu.h:130-157),
 // used by the step-by-step API so its frontier is in ascending vertex order

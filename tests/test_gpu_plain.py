@@ -344,8 +344,10 @@ def test_small_frontier_mode(gpu, small_max):
     # road graph's frontiers outgrow the limit and shrink back (hand-back to
     # the grid kernels and return to the workgroup), the grid's stay small.
     # Results and superstep counts against the oracle, run statistics against
-    # the grid kernels alone.
-    for el in (workloads.road(128, 128, seed=9), workloads.grid64()):
+    # the grid kernels alone.  BFS on graphs of <= 4096 vertices and 16384
+    # edges (the grid, the 48x48 road) runs with the graph in LDS.
+    for el in (workloads.road(128, 128, seed=9), workloads.grid64(),
+               workloads.road(48, 48, seed=13)):
         s, d, w = el.arrays()
         n = el.n_vertices
         g = gas.Graph(el)
