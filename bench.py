@@ -223,10 +223,15 @@ def _profile(name):
 
 def roofline_entry(dom, table, workload):
     e = table[dom]
+    ks = _profile(f"kstats_{workload}.json")
+    # the engine books k_small_lds (the LDS-resident form) in k_small's slot
+    kname = dom
+    if ks is not None and dom not in ks.get("kernels", {}) and f"{dom}_lds" in ks["kernels"]:
+        kname = f"{dom}_lds"
     traffic = None
     rec = _profile(f"pmc_{workload}.json")
     if rec is not None:
-        k = rec.get("kernels", {}).get(dom)
+        k = rec.get("kernels", {}).get(kname)
         if k is not None:
             traffic = k.get("hbm_bytes_per_launch")
     achieved = e.get("achieved_gbs")
@@ -234,12 +239,12 @@ def roofline_entry(dom, table, workload):
            "unit": "GB/s", "frac": e.get("frac"), "traffic": traffic,
            "avg_launch_us": e["avg_launch_us"], "launches": e["launches"],
            "algorithmic_bytes_per_launch": e.get("algorithmic_bytes_per_launch")}
-    ks = _profile(f"kstats_{workload}.json")
     b = e.get("algorithmic_bytes_per_launch")
-    if ks is not None and b and dom in ks.get("kernels", {}):
-        us = ks["kernels"][dom]["avg_us"]
+    if ks is not None and b and kname in ks.get("kernels", {}):
+        us = ks["kernels"][kname]["avg_us"]
         gbs = b / (us * 1e-6) / 1e9
-        out.update({"rocprof_avg_us": us, "achieved_rocprof": round(gbs, 2),
+        out.update({"rocprof_kernel": kname, "rocprof_avg_us": us,
+                    "achieved_rocprof": round(gbs, 2),
                     "frac_rocprof": round(gbs / HBM_PEAK_GBS, 5)})
     return out
 
