@@ -2,7 +2,7 @@
 // resident graph share every edge read.  Lane s of a batch is query s; a
 // wave64 is exactly one vertex's row of lanes.
 //
-//   BFS   per-vertex uint64 masks: visited, frontier (by parity), next;
+//   BFS   per-vertex uint64 masks: visited, frontier, next;
 //         depth stored as 16 bits per (lane, vertex) (0xffff = unreached),
 //         levels up to 65534.  A round is either a push from the frontier list
 //         (items of up to 16 out-edges carrying the frontier mask, 4 items
@@ -390,7 +390,7 @@ static __global__ __launch_bounds__(NT) void kb_hub_items(BatchDev d) {
 }
 
 // ---- BFS ---------------------------------------------------------------------
-// Clears lanes `reset` (visited bits, frontier bits of parity p, depth rows).
+// Clears lanes `reset` (visited bits, frontier bits, depth rows).
 static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d,
                                                    uint64_t reset) {
   const int stride = gridDim.x * NT;
@@ -1097,8 +1097,8 @@ class BatchEngine {
 
   // Rounds per replay of the captured round graph; 0 launches each round's
   // kernels directly.  The graph holds k consecutive rounds' launches;
-  // their arguments do not change between rounds (the round, its parity
-  // and the lanes in use are read from the control block), so one capture
+  // their arguments do not change between rounds (the round and the lanes
+  // in use are read from the control block), so one capture
   // serves every replay until a setting it was captured with changes.
   void setGraphRounds(int k) {
     if (k < 0 || k > kMaxGraphRounds)
