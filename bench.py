@@ -703,7 +703,9 @@ def main():
                 tuned[name] = {"bfs_pull_divisor": gg.autotune_batch(gas.BATCH_BFS),
                                "sssp_delta": gg.autotune_batch(gas.BATCH_SSSP),
                                "bfs_hub_items": gg.batch_hub_items(gas.BATCH_BFS),
-                               "sssp_hub_items": gg.batch_hub_items(gas.BATCH_SSSP)}
+                               "sssp_hub_items": gg.batch_hub_items(gas.BATCH_SSSP),
+                               "bfs_item_edges": gg.batch_item_edges(gas.BATCH_BFS),
+                               "sssp_item_edges": gg.batch_item_edges(gas.BATCH_SSSP)}
             line["batch_tuning"] = tuned
             log(f"batch tuning {tuned}")
             if "batched" in want:

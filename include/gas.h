@@ -452,9 +452,10 @@ int gas_batch_set_sssp_delta(gas_batch_t b, int32_t delta);
  * 1.1x the default on any sample -- GAS_BATCH_BFS: pull divisor in {0, 2, 8, 32};
  * GAS_BATCH_SSSP: distance step in {0, w/4, w/2, w} for the largest edge
  * weight w -- then, with that setting, the hub threshold
- * (gas_graph_batch_hub_items).  Batch engines created on the graph afterwards
- * (gas_batch_create, the server's engines) start with both; param_out (may
- * be NULL) receives the first. */
+ * (gas_graph_batch_hub_items), then the push item size
+ * (gas_graph_batch_item_edges).  Batch engines created on the graph
+ * afterwards (gas_batch_create, the server's engines) start with all three;
+ * param_out (may be NULL) receives the first. */
 int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out);
 /* The setting batch engines on this graph start with (0 until tuned). */
 int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param);
@@ -466,6 +467,14 @@ int gas_graph_batch_param(gas_graph_t g, int kind, int32_t *param);
 int gas_graph_batch_hub_items(gas_graph_t g, int kind, int32_t *items);
 /* Sets one batch engine's hub threshold (>= 16 items). */
 int gas_batch_set_hub_items(gas_batch_t b, int32_t items);
+/* Out-edges per push item of the batch engines on this graph: the tile a
+ * list vertex's out-edges are cut into (BFS 16 or 8, SSSP 64 or 32).  The
+ * tuned value (the autotuner's third stage, under the same rule), the
+ * larger size until tuned.  No counterpart in the reference. */
+int gas_graph_batch_item_edges(gas_graph_t g, int kind, int32_t *edges);
+/* Sets one batch engine's item size (a power of two, BFS 8..16, SSSP
+ * 32..64); takes effect from the next list. */
+int gas_batch_set_item_edges(gas_batch_t b, int32_t edges);
 /* Rounds per launch of a BFS / SSSP batch engine's captured round graph:
  * k in 1..8 records k consecutive rounds' kernel launches once (hipGraph
  * stream capture) and issues rounds as replays of that graph; 0 launches

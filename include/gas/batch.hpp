@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -43,6 +44,10 @@ namespace gas {
 constexpr int BW = 64;            // lanes per batch (= wave size)
 constexpr int BFS_ITEM_EDGES = 16;   // BFS push: out-edges per item, 4 items per wave
 constexpr int SSSP_ITEM_EDGES = 64;  // SSSP push: out-edges per item (one wave)
+// Items may be cut shorter (gas_batch_set_item_edges): down to these sizes,
+// which size the item and hub lists.
+constexpr int BFS_ITEM_EDGES_MIN = 8;
+constexpr int SSSP_ITEM_EDGES_MIN = 32;
 constexpr int BFS_ITEMS_PER_TRIP = 8;  // BFS push: items in flight per wave
 constexpr int HUB_ITEMS = 32;     // a vertex with more items is written by
                                   // a whole workgroup of kb_hub_items (the
@@ -910,7 +915,8 @@ class BatchEngine {
     setHubItems(HUB_ITEMS);
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
-    m_item.alloc((size_t)gv.n_edges / d.item_edges + V + 1);
+    const int ie_min = kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
+    m_item.alloc((size_t)gv.n_edges / ie_min + V + 1);
     m_bsum.alloc(3 * SCAN_BLOCKS);
     m_esum.alloc(SCAN_BLOCKS);
     m_lor.alloc(SCAN_BLOCKS);
@@ -957,8 +963,8 @@ class BatchEngine {
     d.item = m_item.p;
     // a hub has > HUB_ITEMS_MIN items, so at most
     // E / (HUB_ITEMS_MIN * item_edges) of them
-    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * d.item_edges) + 1);
-    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * d.item_edges) + 1);
+    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * ie_min) + 1);
+    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * ie_min) + 1);
     d.hub = m_hub.p;
     d.hub_mask = m_hub_mask.p;
     d.bsum = m_bsum.p;
@@ -981,6 +987,11 @@ class BatchEngine {
                                     &hub),
           "batch hub items");
     setHubItems(hub);
+    int32_t ie = 0;
+    check(gas_graph_batch_item_edges(caller, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP,
+                                     &ie),
+          "batch item edges");
+    setItemEdges(ie);
     int32_t graph_rounds = 0;
     check(gas_graph_batch_graph_rounds(
               caller, kind == BFS_KIND ? GAS_BATCH_BFS : GAS_BATCH_SSSP, &graph_rounds),
@@ -1036,6 +1047,20 @@ class BatchEngine {
     m_dev.hub_items = items;
     m_may_hub = m_max_out > (int64_t)items * m_dev.item_edges;
   }
+
+  // Out-edges per push item (a power of two from the kind's minimum up to
+  // its default: BFS 8..16, SSSP 32..64); takes effect from the next list.
+  void setItemEdges(int edges) {
+    const int lo = m_kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
+    const int hi = m_kind == BFS_KIND ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
+    if (edges < lo || edges > hi || (edges & (edges - 1)))
+      throw Error(GAS_ERR_ARG, "item edges: a power of two from " + std::to_string(lo) +
+                                   " to " + std::to_string(hi));
+    if (m_dev.item_edges != edges) dropGraph();
+    m_dev.item_edges = edges;
+    m_may_hub = m_max_out > (int64_t)m_dev.hub_items * edges;
+  }
+  int itemEdges() const { return m_dev.item_edges; }
 
   // Starts queries: lane[i] runs from source[i].  The lanes must be free.
   void addLanes(const int32_t *lanes, const int32_t *sources, int n) {

@@ -306,6 +306,8 @@ def test_batch_autotune(gpu):
     assert pull in (0, 2, 8, 32)
     for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
         assert g.batch_hub_items(kind) in (16, 32, 64, 128)
+    assert g.batch_item_edges(gas.BATCH_BFS) in (8, 16)
+    assert g.batch_item_edges(gas.BATCH_SSSP) in (32, 64)
     # batches created now start with the tuned settings; results stay exact
     src = _sources(np.random.default_rng(4), el, 10)
     for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
@@ -442,3 +444,34 @@ def test_batch_round_graph_replay(gpu, kind, graph):
             assert np.array_equal(res[lane], ref[qi]), (k, lane)
     with pytest.raises(gas.GasError):
         gas.Batch(g, kind).set_graph_rounds(9)
+
+
+@pytest.mark.parametrize("kind", [gas.BATCH_BFS, gas.BATCH_SSSP])
+def test_batch_item_edges(gpu, kind):
+    # push items cut shorter (BFS 8 instead of 16 out-edges, SSSP 32 instead
+    # of 64) give the same exact per-lane results, with hubs written by
+    # workgroups and by list threads, on R-MAT-14 and a road graph; sizes
+    # outside the kind's range are refused
+    small, large = (8, 16) if kind == gas.BATCH_BFS else (32, 64)
+    for el in (workloads.rmat(14, symmetrize=True, weights=True),
+               workloads.road(96, 96, seed=17)):
+        s, d, w = el.arrays()
+        n = el.n_vertices
+        g = gas.Graph(el)
+        assert g.batch_item_edges(kind) == large
+        src = _sources(np.random.default_rng(21), el, 24)
+        ref = [oracle.check_bfs(n, s, d, int(x)) if kind == gas.BATCH_BFS
+               else oracle.check_dijkstra(n, s, d, w, int(x)) for x in src]
+        for edges, hub in ((small, 16), (small, 1 << 20), (large, 16)):
+            b = gas.Batch(g, kind)
+            b.set_item_edges(edges)
+            b.set_hub_items(hub)
+            b.add(np.arange(24), src)
+            b.run()
+            res = b.results(range(24))
+            for lane in range(24):
+                assert np.array_equal(res[lane], ref[lane]), (edges, hub, lane)
+    b = gas.Batch(g, kind)
+    for bad in (small // 2, large * 2, small + 1):
+        with pytest.raises(gas.GasError):
+            b.set_item_edges(bad)

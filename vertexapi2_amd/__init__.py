@@ -371,6 +371,13 @@ class Graph:
         _check(lib().gas_graph_batch_hub_items(self._h, C.c_int(kind), C.byref(p)))
         return p.value
 
+    def batch_item_edges(self, kind: int) -> int:
+        """Out-edges per push item of the batch engines on this graph (the
+        tuned value; BFS 16, SSSP 64 until tuned)."""
+        p = C.c_int32(0)
+        _check(lib().gas_graph_batch_item_edges(self._h, C.c_int(kind), C.byref(p)))
+        return p.value
+
     def batch_graph_rounds(self, kind: int) -> int:
         """Rounds per replay of the captured round graph the batch engines
         on this graph start with (0: direct launches, the default)."""
@@ -636,6 +643,10 @@ class Batch:
         """List vertices with more push items than this have their items
         written by a whole workgroup (>= 16)."""
         _check(lib().gas_batch_set_hub_items(self._h, C.c_int32(items)))
+
+    def set_item_edges(self, edges: int) -> None:
+        """BFS / SSSP: out-edges per push item (BFS 8 or 16, SSSP 32 or 64)."""
+        _check(lib().gas_batch_set_item_edges(self._h, C.c_int32(edges)))
 
     def set_graph_rounds(self, rounds: int) -> None:
         """BFS / SSSP: issue rounds as replays of a captured graph of

@@ -250,6 +250,21 @@ int gas_batch_set_graph_rounds(gas_batch_t b, int32_t rounds) {
   return guarded_b([&] { b->e->setGraphRounds(rounds); });
 }
 
+int gas_graph_batch_item_edges(gas_graph_t g, int kind, int32_t *edges) {
+  if (!g || !edges || (kind != GAS_BATCH_BFS && kind != GAS_BATCH_SSSP))
+    return set_error(GAS_ERR_ARG, "gas_graph_batch_item_edges: bad arguments");
+  *edges = g->batch_items[kind] ? g->batch_items[kind]
+                                : (kind == GAS_BATCH_BFS ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES);
+  return GAS_OK;
+}
+
+int gas_batch_set_item_edges(gas_batch_t b, int32_t edges) {
+  BATCH_OR_FAIL(b);
+  if (!b->e)
+    return set_error(GAS_ERR_ARG, "gas_batch_set_item_edges: not a BFS / SSSP batch");
+  return guarded_b([&] { b->e->setItemEdges(edges); });
+}
+
 int gas_batch_set_hub_items(gas_batch_t b, int32_t items) {
   BATCH_OR_FAIL(b);
   if (items < HUB_ITEMS_MIN)
@@ -307,12 +322,14 @@ int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
     BatchEngine e(g, kind == GAS_BATCH_BFS ? BatchEngine::BFS_KIND
                                            : BatchEngine::SSSP_KIND);
     // per-sample times of one setting
-    auto time_one = [&](int32_t c, int32_t hub) {
+    const int ie_default = kind == GAS_BATCH_BFS ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
+    auto time_one = [&](int32_t c, int32_t hub, int ie) {
       if (kind == GAS_BATCH_BFS)
         e.setPullDivisor(c);
       else
         e.setSsspDelta(c);
       e.setHubItems(hub);
+      e.setItemEdges(ie);
       std::vector<double> t;
       for (const auto &src : samples) {
         const uint64_t all = src.size() == 64 ? ~0ull : ((1ull << src.size()) - 1);
@@ -332,13 +349,13 @@ int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
       for (double x : t) s += x;
       return s;
     };
-    time_one(cand[0], HUB_ITEMS);  // warm-up
-    const std::vector<double> base = time_one(cand[0], HUB_ITEMS);
+    time_one(cand[0], HUB_ITEMS, ie_default);  // warm-up
+    const std::vector<double> base = time_one(cand[0], HUB_ITEMS, ie_default);
     int32_t best = cand[0];
     double best_s = total(base);
     std::vector<double> best_t = base;
     for (size_t i = 1; i < cand.size(); ++i) {
-      const std::vector<double> t = time_one(cand[i], HUB_ITEMS);
+      const std::vector<double> t = time_one(cand[i], HUB_ITEMS, ie_default);
       bool safe = true;
       for (int k = 0; k < kSamples; ++k) safe = safe && t[k] <= 1.1 * base[k];
       if (safe && total(t) < best_s) {
@@ -353,15 +370,31 @@ int gas_graph_autotune_batch(gas_graph_t g, int kind, int32_t *param_out) {
     int32_t best_hub = HUB_ITEMS;
     const std::vector<double> hub_base = best_t;
     for (int32_t h : {16, 64, 128}) {
-      const std::vector<double> t = time_one(best, h);
+      const std::vector<double> t = time_one(best, h, ie_default);
       bool safe = true;
       for (int k = 0; k < kSamples; ++k) safe = safe && t[k] <= 1.1 * hub_base[k];
       if (safe && total(t) < best_s) {
         best_s = total(t);
         best_hub = h;
+        best_t = t;
       }
     }
     g->batch_hub[kind] = best_hub;
+    // third stage: out-edges per push item (the tile a list vertex is cut
+    // into: BFS 16 or 8, SSSP 64 or 32), same rule
+    int32_t best_ie = ie_default;
+    const std::vector<double> ie_base = best_t;
+    const int ie_alt = kind == GAS_BATCH_BFS ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
+    {
+      const std::vector<double> t = time_one(best, best_hub, ie_alt);
+      bool safe = true;
+      for (int k = 0; k < kSamples; ++k) safe = safe && t[k] <= 1.1 * ie_base[k];
+      if (safe && total(t) < best_s) {
+        best_s = total(t);
+        best_ie = ie_alt;
+      }
+    }
+    g->batch_items[kind] = best_ie;
     if (param_out) *param_out = best;
   });
 }

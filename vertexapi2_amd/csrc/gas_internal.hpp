@@ -48,6 +48,8 @@ struct gas_graph {
   int32_t batch_hub[2] = {0, 0};
   // rounds per replay of their captured round graph (0: direct launches)
   int32_t batch_graph[2] = {0, 0};
+  // out-edges per push item of those batch engines (0: the default)
+  int32_t batch_items[2] = {0, 0};
   // Engines kept bound to this graph by the whole-run entry points
   // (gas_bfs, ...), one per program and engine set, released with the
   // graph.  A host thread uses the set engine_set() names, so threads of
