@@ -80,6 +80,12 @@ struct BatchCtl {
   uint64_t used;
 };
 
+struct BatchScan {
+  int32_t n_list, n_items, n_hubs, pad_;
+  int64_t list_edges;
+  uint64_t lanes_alive;
+};
+
 struct BatchDev {
   int32_t nV, nE;
   const int32_t *csc_off, *csc_src, *csr_off, *csr_dst, *csr_w;
@@ -104,6 +110,8 @@ struct BatchDev {
                                         // {v, first item, e0, e1}
   uint64_t *hub_mask;                   // their masks
   int32_t *bsum;                        // [3][SCAN_BLOCKS] count | items | hubs
+  int32_t *bpre;                        // [3][SCAN_BLOCKS] their exclusive prefixes
+  struct BatchScan *scan;               // totals of the count pass (kb_scan)
   int64_t *esum;                        // [SCAN_BLOCKS] out-edges
   uint64_t *lor;                        // [SCAN_BLOCKS] OR of masks
   int32_t pull_div;                     // pull when list_edges*div >= E
@@ -261,6 +269,67 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
   }
 }
 
+// Between the count pass and the list pass: exclusive prefixes of the
+// per-workgroup counts (list slots, items, hubs) and the totals, once, in
+// one workgroup of SCAN_BLOCKS threads (thread i: chunk i).
+constexpr int SCAN_NT = SCAN_BLOCKS;
+static __global__ __launch_bounds__(SCAN_NT) void kb_scan(BatchDev d) {
+  __shared__ int64_t w_a[SCAN_NT / WAVE], w_b[SCAN_NT / WAVE], w_e[SCAN_NT / WAVE];
+  __shared__ uint64_t w_o[SCAN_NT / WAVE];
+  const int i = threadIdx.x, lane = lane_id(), w = i / WAVE;
+  const int64_t cnt = d.bsum[i], items = d.bsum[SCAN_BLOCKS + i],
+                hubs = d.bsum[2 * SCAN_BLOCKS + i];
+  // list slots and hubs in one 64-bit scan (each total < 2^31)
+  const int64_t a0 = (hubs << 32) | cnt, b0 = items;
+  int64_t a = a0, b = b0, e = d.esum[i];
+  uint64_t o = d.lor[i];
+#pragma unroll
+  for (int k = 1; k < WAVE; k <<= 1) {
+    const int64_t ya = __shfl_up(a, k, WAVE), yb = __shfl_up(b, k, WAVE);
+    if (lane >= k) {
+      a += ya;
+      b += yb;
+    }
+  }
+#pragma unroll
+  for (int k = WAVE / 2; k > 0; k >>= 1) {
+    e += __shfl_xor(e, k, WAVE);
+    o |= ((uint64_t)__shfl_xor((uint32_t)(o >> 32), k, WAVE) << 32) |
+         __shfl_xor((uint32_t)o, k, WAVE);
+  }
+  if (lane == WAVE - 1) {
+    w_a[w] = a;
+    w_b[w] = b;
+  }
+  if (lane == 0) {
+    w_e[w] = e;
+    w_o[w] = o;
+  }
+  __syncthreads();
+  int64_t pa = 0, pb = 0, ta = 0, tb = 0, te = 0;
+  uint64_t to = 0;
+  for (int k = 0; k < SCAN_NT / WAVE; ++k) {
+    pa += k < w ? w_a[k] : 0;
+    pb += k < w ? w_b[k] : 0;
+    ta += w_a[k];
+    tb += w_b[k];
+    te += w_e[k];
+    to |= w_o[k];
+  }
+  const int64_t xa = pa + a - a0, xb = pb + b - b0;
+  d.bpre[i] = (int32_t)(xa & 0xffffffff);
+  d.bpre[SCAN_BLOCKS + i] = (int32_t)xb;
+  d.bpre[2 * SCAN_BLOCKS + i] = (int32_t)(xa >> 32);
+  if (i == 0) {
+    BatchScan *sc = d.scan;
+    sc->n_list = (int32_t)(ta & 0xffffffff);
+    sc->n_hubs = (int32_t)(ta >> 32);
+    sc->n_items = (int32_t)tb;
+    sc->list_edges = te;
+    sc->lanes_alive = to;
+  }
+}
+
 // Pass 2: ordered list of vertices with nonzero masks (the BFS frontier or
 // the SSSP changes), item offsets and the owner of every item; workgroup 0
 // publishes totals and the direction and, at the end of a round
@@ -273,15 +342,13 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
   __shared__ uint64_t s_m[NT];
   const uint64_t *m = d.dist ? d.chg : d.front;
   const uint8_t *slm = d.sl_cur;
-  int pre, total, ipre, itotal, hpre, htotal;
-  chunk_prefix(d.bsum, blockIdx.x, pre, total, lds);
-  chunk_prefix(d.bsum + 2 * SCAN_BLOCKS, blockIdx.x, hpre, htotal, lds);
-  chunk_prefix(d.bsum + SCAN_BLOCKS, blockIdx.x, ipre, itotal, lds);
+  const BatchScan sc = *d.scan;
+  const int pre = d.bpre[blockIdx.x], ipre = d.bpre[SCAN_BLOCKS + blockIdx.x],
+            hpre = d.bpre[2 * SCAN_BLOCKS + blockIdx.x];
+  const int total = sc.n_list, itotal = sc.n_items, htotal = sc.n_hubs;
   // every workgroup derives the direction from the edge totals; a round
   // that pulls needs no list
-  int64_t ed = 0;
-  for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) ed += d.esum[i];
-  ed = block_sum64(ed, lds64);
+  const int64_t ed = sc.list_edges;
   const bool pull = d.pull_div > 0 && ed * (int64_t)d.pull_div >= (int64_t)d.nE;
   int carry = pre, icarry = ipre, hcarry = hpre;
   const int ie = d.item_edges;
@@ -350,9 +417,7 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
     hcarry += htot;
   }
   if (blockIdx.x == 0) {
-    uint64_t any = 0;
-    for (int i = threadIdx.x; i < SCAN_BLOCKS; i += NT) any |= d.lor[i];
-    any = block_or(any, ldsu);
+    const uint64_t any = sc.lanes_alive;
     if (threadIdx.x == 0) {
       BatchCtl *c = d.ctl;
       c->n_list = total;
@@ -918,6 +983,8 @@ class BatchEngine {
     const int ie_min = kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
     m_item.alloc((size_t)gv.n_edges / ie_min + V + 1);
     m_bsum.alloc(3 * SCAN_BLOCKS);
+    m_bpre.alloc(3 * SCAN_BLOCKS);
+    m_scan.alloc(1);
     m_esum.alloc(SCAN_BLOCKS);
     m_lor.alloc(SCAN_BLOCKS);
     m_lanes.alloc(BW);
@@ -968,6 +1035,8 @@ class BatchEngine {
     d.hub = m_hub.p;
     d.hub_mask = m_hub_mask.p;
     d.bsum = m_bsum.p;
+    d.bpre = m_bpre.p;
+    d.scan = m_scan.p;
     d.esum = m_esum.p;
     d.lor = m_lor.p;
     BatchCtl c{};
@@ -1013,6 +1082,7 @@ class BatchEngine {
   // Device memory the engine holds (lane state for all 64 lanes, lists).
   size_t deviceBytes() const {
     return m_ctl.bytes() + m_item.bytes() + m_list.bytes() + m_bsum.bytes() +
+           m_bpre.bytes() + m_scan.bytes() +
            m_lanes.bytes() + m_sources.bytes() + m_lane_start.bytes() + m_dist.bytes() +
            m_tmp.bytes() + m_esum.bytes() + m_lor.bytes() + m_vn.bytes() +
            m_front.bytes() + m_chg.bytes() + m_chg_next.bytes() +
@@ -1102,6 +1172,7 @@ class BatchEngine {
       launch(kb_count, SCAN_BLOCKS, m_stream, m_dev,
              (const uint64_t *)m_dev.chg, (const uint8_t *)m_dev.sl_cur);
     }
+    launch_wg(kb_scan, 1, SCAN_NT, m_stream, m_dev);
     launch(kb_list, SCAN_BLOCKS, m_stream, m_dev, 0);
     if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
   }
@@ -1298,6 +1369,7 @@ class BatchEngine {
       launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
     }
+    launch_wg(kb_scan, 1, SCAN_NT, m_stream, m_dev);
     launch(kb_list, SCAN_BLOCKS, m_stream, m_dev, 1);
     if (m_may_hub) launch(kb_hub_items, 256, m_stream, m_dev);
   }
@@ -1344,7 +1416,8 @@ class BatchEngine {
   BatchDev m_dev{};
   DevBuf<BatchCtl> m_ctl;
   DevBuf<int4> m_item;
-  DevBuf<int32_t> m_list, m_bsum, m_lanes, m_sources,
+  DevBuf<BatchScan> m_scan;
+  DevBuf<int32_t> m_list, m_bsum, m_bpre, m_lanes, m_sources,
       m_lane_start, m_dist, m_tmp;
   DevBuf<int64_t> m_esum;
   DevBuf<uint64_t> m_lor, m_vn, m_front, m_chg, m_chg_next,
