@@ -271,18 +271,30 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
 
 // Between the count pass and the list pass: exclusive prefixes of the
 // per-workgroup counts (list slots, items, hubs) and the totals, once, in
-// one workgroup of SCAN_BLOCKS threads (thread i: chunk i).
-constexpr int SCAN_NT = SCAN_BLOCKS;
+// one workgroup of four waves (thread t: chunks 4t .. 4t + 3) -- small
+// enough to find room on a busy GPU (the server runs many engines at once).
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_PER = SCAN_BLOCKS / SCAN_NT;
+static_assert(SCAN_PER * SCAN_NT == SCAN_BLOCKS, "scan chunks per thread");
 static __global__ __launch_bounds__(SCAN_NT) void kb_scan(BatchDev d) {
   __shared__ int64_t w_a[SCAN_NT / WAVE], w_b[SCAN_NT / WAVE], w_e[SCAN_NT / WAVE];
   __shared__ uint64_t w_o[SCAN_NT / WAVE];
-  const int i = threadIdx.x, lane = lane_id(), w = i / WAVE;
-  const int64_t cnt = d.bsum[i], items = d.bsum[SCAN_BLOCKS + i],
-                hubs = d.bsum[2 * SCAN_BLOCKS + i];
-  // list slots and hubs in one 64-bit scan (each total < 2^31)
-  const int64_t a0 = (hubs << 32) | cnt, b0 = items;
-  int64_t a = a0, b = b0, e = d.esum[i];
-  uint64_t o = d.lor[i];
+  const int t = threadIdx.x, lane = lane_id(), w = t / WAVE;
+  // list slots and hubs in one 64-bit sum (each total < 2^31)
+  int64_t ca[SCAN_PER], cb[SCAN_PER];
+  int64_t a0 = 0, b0 = 0, e = 0;
+  uint64_t o = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    const int i = t * SCAN_PER + k;
+    ca[k] = ((int64_t)d.bsum[2 * SCAN_BLOCKS + i] << 32) | (int64_t)d.bsum[i];
+    cb[k] = d.bsum[SCAN_BLOCKS + i];
+    a0 += ca[k];
+    b0 += cb[k];
+    e += d.esum[i];
+    o |= d.lor[i];
+  }
+  int64_t a = a0, b = b0;
 #pragma unroll
   for (int k = 1; k < WAVE; k <<= 1) {
     const int64_t ya = __shfl_up(a, k, WAVE), yb = __shfl_up(b, k, WAVE);
@@ -316,11 +328,18 @@ static __global__ __launch_bounds__(SCAN_NT) void kb_scan(BatchDev d) {
     te += w_e[k];
     to |= w_o[k];
   }
-  const int64_t xa = pa + a - a0, xb = pb + b - b0;
-  d.bpre[i] = (int32_t)(xa & 0xffffffff);
-  d.bpre[SCAN_BLOCKS + i] = (int32_t)xb;
-  d.bpre[2 * SCAN_BLOCKS + i] = (int32_t)(xa >> 32);
-  if (i == 0) {
+  // exclusive prefix of this thread's first chunk, then its chunks in order
+  int64_t xa = pa + a - a0, xb = pb + b - b0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    const int i = t * SCAN_PER + k;
+    d.bpre[i] = (int32_t)(xa & 0xffffffff);
+    d.bpre[SCAN_BLOCKS + i] = (int32_t)xb;
+    d.bpre[2 * SCAN_BLOCKS + i] = (int32_t)(xa >> 32);
+    xa += ca[k];
+    xb += cb[k];
+  }
+  if (t == 0) {
     BatchScan *sc = d.scan;
     sc->n_list = (int32_t)(ta & 0xffffffff);
     sc->n_hubs = (int32_t)(ta >> 32);
