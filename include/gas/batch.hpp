@@ -747,7 +747,7 @@ static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
 // marked changed in the destination whenever its candidate beats the value
 // it read: a superset of the lanes actually lowered (the read may be older
 // than a concurrent atomicMin), which can only cause a redundant push.
-static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
+static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
   const BatchCtl *c = d.ctl;
   const int n_items = c->n_items;
   const int lane = lane_id();
@@ -821,6 +821,8 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
     }
   }
 }
+
+static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) { sssp_push(d); }
 
 // chg <- chg_next, chg_next <- 0, with the list counters (pass 1 of kb_list).
 static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
@@ -1495,7 +1497,7 @@ __device__ inline bool ppr_write(const PprDev &d, int p, int v, uint64_t run,
   return m != 0;
 }
 
-static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
+static __device__ __forceinline__ void ppr_items(const PprDev &d, int p, uint64_t run) {
   const int lane = lane_id();
   const int waves = gridDim.x * (NT / WAVE);
   const float *ct = d.contrib[p];
@@ -1563,6 +1565,10 @@ static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t r
     }
   }
   if (lane_id() == 0 && written) atomicAdd(&d.nzc[(d.rot + 1) % 3], written);
+}
+
+static __global__ __launch_bounds__(NT) void kp_item(PprDev d, int p, uint64_t run) {
+  ppr_items(d, p, run);
 }
 
 static __global__ __launch_bounds__(NT) void kp_fix(PprDev d, int p, uint64_t run) {
