@@ -60,6 +60,13 @@ constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks togethe
 // let the dispatcher balance them (R-MAT-22 SSSP batch 39.2 -> 33.3 ms, PPR
 // 70.8 -> 64.3 ms against 4 x 1024).
 constexpr int PUSH_GRID = 32 * 1024;
+// Graphs of average out-degree <= LOW_DEGREE (road-like meshes): items of a
+// few edges with a few lanes each; the SSSP push unrolls 4 edges and runs on
+// 2,048 workgroups (road1024 64-source SSSP 14.9 -> 14.25 ms; on R-MAT-22
+// either change alone costs 5-13%).
+constexpr int LOW_DEGREE = 8;
+constexpr int SSSP_UNROLL_LOW = 4;
+constexpr int PUSH_GRID_LOW = 2048;
 constexpr int32_t SSSP_INF = INT32_MAX - 100000;  // sssp.cu:32-33
 
 struct BatchCtl {
@@ -743,10 +750,11 @@ static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
 
 // Push from the changed list: one wave per item, lane = query.  The item's
 // edges are loaded lane-parallel once and broadcast by shuffles; each trip
-// issues SSSP_UNROLL distance-row loads before any compare.  A lane is
+// issues U distance-row loads before any compare.  A lane is
 // marked changed in the destination whenever its candidate beats the value
 // it read: a superset of the lanes actually lowered (the read may be older
 // than a concurrent atomicMin), which can only cause a redundant push.
+template <int U>
 static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
   const BatchCtl *c = d.ctl;
   const int n_items = c->n_items;
@@ -789,26 +797,26 @@ static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
     int mx = 0;
 #pragma unroll
     for (int g = 0; g < G; ++g) mx = max(mx, ne[g]);
-    for (int j0 = 0; j0 < mx; j0 += SSSP_UNROLL) {
-      int w[G][SSSP_UNROLL], cand[G][SSSP_UNROLL], cur[G][SSSP_UNROLL];
+    for (int j0 = 0; j0 < mx; j0 += U) {
+      int w[G][U], cand[G][U], cur[G][U];
 #pragma unroll
       for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int x = 0; x < SSSP_UNROLL; ++x) {
+        for (int x = 0; x < U; ++x) {
           w[g][x] = __shfl(my_w[g], j0 + x, WAVE);
           cand[g][x] = du[g] + __shfl(my_wt[g], j0 + x, WAVE);
         }
 #pragma unroll
       for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int x = 0; x < SSSP_UNROLL; ++x)
+        for (int x = 0; x < U; ++x)
           cur[g][x] = (act[g] && j0 + x < ne[g])
                           ? d.dist[(size_t)w[g][x] * BW + lane]
                           : SSSP_INF;
 #pragma unroll
       for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int x = 0; x < SSSP_UNROLL; ++x) {
+        for (int x = 0; x < U; ++x) {
           const bool imp = act[g] && j0 + x < ne[g] && cand[g][x] < cur[g][x];
           if (imp) atomicMin(&d.dist[(size_t)w[g][x] * BW + lane], cand[g][x]);
           const uint64_t bm = __ballot(imp);
@@ -822,7 +830,10 @@ static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
   }
 }
 
-static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) { sssp_push(d); }
+template <int U>
+static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
+  sssp_push<U>(d);
+}
 
 // chg <- chg_next, chg_next <- 0, with the list counters (pass 1 of kb_list).
 static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
@@ -998,6 +1009,7 @@ class BatchEngine {
       for (int32_t v = 0; v < gv.n_vertices; ++v)
         m_max_out = std::max<int64_t>(m_max_out, off[v + 1] - off[v]);
     }
+    m_low_degree = (int64_t)gv.n_edges <= (int64_t)LOW_DEGREE * gv.n_vertices;
     setHubItems(HUB_ITEMS);
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
@@ -1387,7 +1399,10 @@ class BatchEngine {
         launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev);
     } else {
-      launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
+      if (m_low_degree)
+        launch(kb_sssp_push<SSSP_UNROLL_LOW>, PUSH_GRID_LOW, m_stream, m_dev);
+      else
+        launch(kb_sssp_push<SSSP_UNROLL>, PUSH_GRID, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
     }
     launch_wg(kb_scan, 1, SCAN_NT, m_stream, m_dev);
@@ -1448,6 +1463,7 @@ class BatchEngine {
   DevBuf<uint8_t> m_sl_next, m_sl_cur;  // slab activity flags
   bool m_may_hub = true;  // some vertex has more than hub_items items
   int64_t m_max_out = 0;  // largest out-degree
+  bool m_low_degree = false;  // average out-degree <= LOW_DEGREE
 };
 
 // ---- personalised PageRank, 64 FP32 lanes ----------------------------------

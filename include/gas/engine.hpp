@@ -953,7 +953,10 @@ class GASEngineGPU {
 
     const size_t V = (size_t)m_nV;
     const int nfwords = (int)((V + 3) / 4);
-    const size_t tiles = ((size_t)m_nE + V) / MP_NV + 2;
+    // tile partials: merge-path tiles of edges + vertices, or span tiles of
+    // SPAN_TE edges (buildSpan), whichever form has more
+    const size_t tiles =
+        std::max(((size_t)m_nE + V) / MP_NV, (size_t)m_nE / SPAN_TE) + 2;
     ExternalMemory *x = m_ext;
     m_ctl.alloc(1, x);
     m_q0.alloc(V, x);

@@ -239,7 +239,7 @@ static int scratch_address(void **out) {
 // vertices and nE edges (every buffer of GASEngineGPU::bindGraph, each
 // rounded up to 256 bytes, and the hub lists at their largest).
 static size_t noalloc_pagerank_bytes(int64_t nV, int64_t nE) {
-  const int64_t tiles = (nE + nV) / MP_NV + 2;
+  const int64_t tiles = std::max((nE + nV) / MP_NV, nE / SPAN_TE) + 2;
   const int64_t per_vertex = 2 * 8 /* states */ + 4 /* srcval */ + 2 * 4 /* lists */ +
                              4 * 1 /* activation, changed */ + 1 /* apply flags */ +
                              4 /* gather results */ + 4 * 4 /* hub lists, offsets */ +
