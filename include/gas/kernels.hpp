@@ -987,6 +987,10 @@ __device__ __forceinline__ void hgather_tiles(const Dev<P> &d, int p) {
   __shared__ int32_t s_e[PADDED];
   __shared__ int32_t s_k[is_source_only<P>::value ? 1 : PADDED];
   __shared__ GR s_val[PADDED];
+  // source-only programs: the hottest source values staged as in the span
+  // form (R-MAT-21 symmetrised CC: 104 -> 90 us per launch)
+  constexpr int HOT = is_source_only<P>::value ? SPAN_HOT : 1;
+  __shared__ GR s_hot[HOT];
 
   if (frontier_size(d) == 0) return;
   const bool all = d.in_mode == IN_ALL;
@@ -995,6 +999,13 @@ __device__ __forceinline__ void hgather_tiles(const Dev<P> &d, int p) {
   const int N = h.items + h.n;
   const int ntiles = (N + MP_NV - 1) / MP_NV;
   const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= ntiles) return;
+  int nhot = 0;
+  if constexpr (is_source_only<P>::value) {
+    nhot = min(HOT, d.nV);
+    for (int k = tid; k < nhot; k += NT) s_hot[k] = d.srcval[k];
+    __syncthreads();
+  }
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int d0 = t * MP_NV;
@@ -1055,7 +1066,8 @@ __device__ __forceinline__ void hgather_tiles(const Dev<P> &d, int p) {
       if (ee[it] >= 0) {
         int dst = 0;
         if constexpr (!is_source_only<P>::value) dst = s_vtx[s_k[j + (j >> 5)]];
-        s_val[j + (j >> 5)] = gather_value(d, dst, src[it], ee[it]);
+        s_val[j + (j >> 5)] = src[it] < nhot ? s_hot[src[it]]
+                                             : gather_value(d, dst, src[it], ee[it]);
       }
     }
     __syncthreads();
@@ -1165,6 +1177,7 @@ __device__ __forceinline__ void hgather_span(const Dev<P> &d) {
   const int hi = d.span_hi, e0 = d.span_e0, e1 = d.span_e1;
   const int ntiles = (e1 - e0 + SPAN_TE - 1) / SPAN_TE;
   const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= ntiles) return;
   const int nhot = min(SPAN_HOT, d.nV);
   for (int k = tid; k < nhot; k += NT) s_hot[k] = d.srcval[k];
   __syncthreads();
