@@ -110,7 +110,8 @@ struct BatchDev {
   uint64_t *chg, *chg_next;             // SSSP
   int32_t *list;                        // current list (slab order)
   int4 *item;                           // push items: BFS {e0, e1, mask};
-                                        // SSSP {u, e0, e1, 0}
+                                        // SSSP {u, e0, e1, octet (0 unless
+                                        // octets)}
   int32_t item_edges;                   // out-edges per item
   int32_t hub_items;                    // hub threshold in items
   int4 *hub;                            // vertices with > hub_items items:
@@ -124,6 +125,9 @@ struct BatchDev {
   int32_t pull_div;                     // pull when list_edges*div >= E
                                         // (0: never, SSSP)
   int32_t sssp_delta;                   // SSSP distance bound step (0: none)
+  int32_t octets;                       // SSSP items per (vertex, octet of
+                                        // lanes with a change): see
+                                        // kb_sssp_push_oct
   // Activity of 64-vertex slabs (SLAB): a slab whose flag is 0 holds no
   // nonzero mask of that array, so the O(V) passes skip it.  sl_next: some
   // vertex of the slab got bits in next (BFS) / chg_next (SSSP) this round;
@@ -237,6 +241,45 @@ __device__ inline int slab_vertex(int r) {
   return ((int)blockIdx.x + (idx / SLAB) * SCAN_BLOCKS) * SLAB + idx % SLAB;
 }
 
+// Octets of a 64-lane mask that hold some set lane, as 8 bits.
+__device__ inline uint32_t octet_mask(uint64_t m) {
+  uint32_t om = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) om |= ((m >> (8 * i)) & 0xffull) ? 1u << i : 0u;
+  return om;
+}
+__device__ inline int nth_set_bit(uint32_t m, int n) {
+  for (int i = 0; i < n; ++i) m &= m - 1;
+  return __ffs(m) - 1;
+}
+
+// Push items of a vertex with mask m and deg out-edges: chunks of
+// item_edges out-edges, each once per octet of m with a set lane when the
+// engine has octets.
+__device__ inline int item_count(const BatchDev &d, uint64_t m, int deg) {
+  const int c = (deg + d.item_edges - 1) / d.item_edges;
+  return d.octets ? c * __popc(octet_mask(m)) : c;
+}
+// Push item k of vertex v (mask m, out-edges [o, o1)): octet-major.
+__device__ inline int4 push_item(const BatchDev &d, int v, uint64_t m, int o,
+                                 int o1, int k) {
+  const int ie = d.item_edges;
+  if (!d.dist) {
+    const int e0 = o + k * ie;
+    return make_int4(e0, min(e0 + ie, o1), (int)(uint32_t)m,
+                     (int)(uint32_t)(m >> 32));
+  }
+  int oct = 0;
+  if (d.octets) {
+    const int c = (o1 - o + ie - 1) / ie;
+    const int oi = k / c;
+    k -= oi * c;
+    oct = nth_set_bit(octet_mask(m), oi);
+  }
+  const int e0 = o + k * ie;
+  return make_int4(v, e0, min(e0 + ie, o1), oct);
+}
+
 // Pass 1 over contiguous chunks of vertices with mask array m: per chunk the
 // number of nonzero masks, their push items and out-edges, and the OR.
 static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t *m,
@@ -255,7 +298,7 @@ static __global__ __launch_bounds__(NT) void kb_count(BatchDev d, const uint64_t
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      const int nit = item_count(d, x, deg);
       items += nit;
       hubs += nit > d.hub_items;
       edges += deg;
@@ -377,7 +420,6 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
   const int64_t ed = sc.list_edges;
   const bool pull = d.pull_div > 0 && ed * (int64_t)d.pull_div >= (int64_t)d.nE;
   int carry = pre, icarry = ipre, hcarry = hpre;
-  const int ie = d.item_edges;
   const int nr = pull ? 0 : slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
@@ -390,7 +432,7 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
     if (on) {
       o = d.csr_off[v];
       o1 = d.csr_off[v + 1];
-      nit = (o1 - o + ie - 1) / ie;
+      nit = item_count(d, mv, o1 - o);
     }
     // one scan for four counters: list slots | hub slots (16 bits each,
     // at most NT per round) and items | non-hub items (32 bits each)
@@ -428,14 +470,9 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
         if (s_jpos[mid] <= j) lo = mid; else hi = mid;
       }
       const int k = j - s_jpos[lo];
-      const int e0 = s_o[lo] + k * ie;
-      const int e1 = min(e0 + ie, s_o1[lo]);
-      const uint64_t mk = s_m[lo];
       // BFS items carry the frontier mask, SSSP items the vertex
       d.item[icarry + s_ipos[lo] + k] =
-          d.dist ? make_int4(s_v[lo], e0, e1, 0)
-                 : make_int4(e0, e1, (int)(uint32_t)mk,
-                             (int)(uint32_t)(mk >> 32));
+          push_item(d, s_v[lo], s_m[lo], s_o[lo], s_o1[lo], k);
     }
     __syncthreads();
     carry += tot;
@@ -470,18 +507,12 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
 // Items of the list's hubs: one workgroup per hub.
 static __global__ __launch_bounds__(NT) void kb_hub_items(BatchDev d) {
   const int n = d.ctl->n_hubs;
-  const int ie = d.item_edges;
   for (int h = blockIdx.x; h < n; h += gridDim.x) {
     const int4 hb = d.hub[h];
     const uint64_t mk = d.hub_mask[h];
-    const int nit = (hb.w - hb.z + ie - 1) / ie;
-    for (int k = threadIdx.x; k < nit; k += NT) {
-      const int e0 = hb.z + k * ie;
-      const int e1 = min(e0 + ie, hb.w);
-      d.item[hb.y + k] = d.dist ? make_int4(hb.x, e0, e1, 0)
-                                : make_int4(e0, e1, (int)(uint32_t)mk,
-                                            (int)(uint32_t)(mk >> 32));
-    }
+    const int nit = item_count(d, mk, hb.w - hb.z);
+    for (int k = threadIdx.x; k < nit; k += NT)
+      d.item[hb.y + k] = push_item(d, hb.x, mk, hb.z, hb.w, k);
   }
 }
 
@@ -693,7 +724,7 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d) {
     if (nv) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      const int nit = item_count(d, nv, deg);
       items += nit;
       hubs += nit > d.hub_items;
       edges += deg;
@@ -835,6 +866,99 @@ static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
   sssp_push<U>(d);
 }
 
+// Push from octet items (engines with octets: graphs of low degree, whose
+// list vertices have a few changed lanes spread over the 64): a wave takes
+// 8 items at a time, lanes 8g .. 8g + 7 serve item g's octet (queries
+// 8 * octet .. + 7), so a trip relaxes 8 (vertex, octet) pairs where the
+// vertex-wide form relaxes one vertex for all 64 lanes.  The trip is three
+// dependent loads deep: the item; the change mask, the octet's distances
+// (one 32-byte sector, read whatever the mask says) and the first 8
+// out-edges (a lane each, walked by shuffles) together; then the
+// destinations' sectors.  The next trip's item loads during this one.
+// Change marks as in sssp_push, per group: its lanes' bits, shifted to the
+// octet.
+static __device__ __forceinline__ void sssp_push_oct(const BatchDev &d) {
+  const BatchCtl *c = d.ctl;
+  const int n_items = c->n_items;
+  const int round = c->round;
+  const int lane = lane_id();
+  const int g = lane >> 3, sub = lane & 7, g0 = lane & ~7;
+  const int waves = gridDim.x * (NT / WAVE);
+  const int wid = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE;
+  int base = wid * 8;
+  int4 next = base + g < n_items ? d.item[base + g] : make_int4(0, 0, 0, 0);
+  for (; base < n_items; base += waves * 8) {
+    const int4 item = next;
+    {
+      const int i = base + waves * 8 + g;
+      next = i < n_items ? d.item[i] : make_int4(0, 0, 0, 0);
+    }
+    const int u = item.x, e0 = item.y, ne = item.z - item.y;
+    const int sh = 8 * item.w;  // the octet's first lane
+    const int q = sh + sub;     // this lane's query
+    uint64_t cm = 0;
+    int du = SSSP_INF, my_w = 0, my_wt = 0;
+    if (ne > 0) {
+      cm = d.chg[u];
+      du = d.dist[(size_t)u * BW + q];
+    }
+    if (sub < ne) {
+      my_w = d.csr_dst[e0 + sub];
+      my_wt = d.csr_w[e0 + sub];
+    }
+    const bool changed = (cm >> q) & 1ull;
+    const bool act = changed && du < sssp_bound(d, q, round);
+    // changed lanes above the bound wait for a later round
+    const uint64_t wait = __ballot(changed && !act);
+    const uint64_t wb = (wait >> (8 * g)) & 0xffull;
+    if (sub == 0 && wb) {
+      atomicOr((unsigned long long *)&d.chg_next[u],
+               (unsigned long long)(wb << sh));
+      d.sl_next[u / SLAB] = 1;
+    }
+    // the wave's longest item
+    int mx = ne;
+#pragma unroll
+    for (int o = 8; o < WAVE; o <<= 1) mx = max(mx, __shfl_xor(mx, o, WAVE));
+    for (int c0 = 0; c0 < mx; c0 += 8) {
+      if (c0 > 0) {
+        my_w = my_wt = 0;
+        if (c0 + sub < ne) {
+          my_w = d.csr_dst[e0 + c0 + sub];
+          my_wt = d.csr_w[e0 + c0 + sub];
+        }
+      }
+      const int nx = min(8, mx - c0);
+      for (int x0 = 0; x0 < nx; x0 += 4) {
+        int w[4], cand[4], cur[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          w[x] = __shfl(my_w, g0 + x0 + x, WAVE);
+          cand[x] = du + __shfl(my_wt, g0 + x0 + x, WAVE);
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          cur[x] = (act && c0 + x0 + x < ne) ? d.dist[(size_t)w[x] * BW + q]
+                                              : SSSP_INF;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const bool imp = act && c0 + x0 + x < ne && cand[x] < cur[x];
+          if (imp) atomicMin(&d.dist[(size_t)w[x] * BW + q], cand[x]);
+          const uint64_t bm = (__ballot(imp) >> (8 * g)) & 0xffull;
+          if (sub == 0 && bm) {
+            atomicOr((unsigned long long *)&d.chg_next[w[x]],
+                     (unsigned long long)(bm << sh));
+            d.sl_next[w[x] / SLAB] = 1;
+          }
+        }
+      }
+    }
+  }
+}
+static __global__ __launch_bounds__(NT) void kb_sssp_push_oct(BatchDev d) {
+  sssp_push_oct(d);
+}
+
 // chg <- chg_next, chg_next <- 0, with the list counters (pass 1 of kb_list).
 static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
   __shared__ int lds[NT / WAVE];
@@ -864,7 +988,7 @@ static __global__ __launch_bounds__(NT) void kb_sssp_finalize(BatchDev d) {
     if (x) {
       const int deg = d.csr_off[v + 1] - d.csr_off[v];
       ++cnt;
-      const int nit = (deg + d.item_edges - 1) / d.item_edges;
+      const int nit = item_count(d, x, deg);
       items += nit;
       hubs += nit > d.hub_items;
       edges += deg;
@@ -1010,11 +1134,15 @@ class BatchEngine {
         m_max_out = std::max<int64_t>(m_max_out, off[v + 1] - off[v]);
     }
     m_low_degree = (int64_t)gv.n_edges <= (int64_t)LOW_DEGREE * gv.n_vertices;
+    const int ie_min = kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
+    // at most one item per ie_min out-edges and vertex, 8 times that with
+    // octets (whose item counts must stay within int32)
+    const int64_t max_items = (int64_t)gv.n_edges / ie_min + (int64_t)V + 1;
+    d.octets = kind == SSSP_KIND && m_low_degree && 8 * max_items < INT32_MAX;
     setHubItems(HUB_ITEMS);
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
-    const int ie_min = kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
-    m_item.alloc((size_t)gv.n_edges / ie_min + V + 1);
+    m_item.alloc((size_t)max_items * (d.octets ? 8 : 1));
     m_bsum.alloc(3 * SCAN_BLOCKS);
     m_bpre.alloc(3 * SCAN_BLOCKS);
     m_scan.alloc(1);
@@ -1148,7 +1276,14 @@ class BatchEngine {
       throw Error(GAS_ERR_ARG, "hub threshold outside [16, 2^24] items");
     if (m_dev.hub_items != items) dropGraph();
     m_dev.hub_items = items;
-    m_may_hub = m_max_out > (int64_t)items * m_dev.item_edges;
+    updateMayHub();
+  }
+  // kb_hub_items runs only where some vertex can have more than hub_items
+  // items
+  void updateMayHub() {
+    const int64_t ie = m_dev.item_edges;
+    const int64_t most = (m_max_out + ie - 1) / ie * (m_dev.octets ? 8 : 1);
+    m_may_hub = most > (int64_t)m_dev.hub_items;
   }
 
   // Out-edges per push item (a power of two from the kind's minimum up to
@@ -1161,7 +1296,7 @@ class BatchEngine {
                                    " to " + std::to_string(hi));
     if (m_dev.item_edges != edges) dropGraph();
     m_dev.item_edges = edges;
-    m_may_hub = m_max_out > (int64_t)m_dev.hub_items * edges;
+    updateMayHub();
   }
   int itemEdges() const { return m_dev.item_edges; }
 
@@ -1399,7 +1534,9 @@ class BatchEngine {
         launch(kb_bfs_push, 4 * SCAN_BLOCKS, m_stream, m_dev);
       launch(kb_bfs_finalize, SCAN_BLOCKS, m_stream, m_dev);
     } else {
-      if (m_low_degree)
+      if (m_dev.octets)
+        launch(kb_sssp_push_oct, PUSH_GRID_LOW, m_stream, m_dev);
+      else if (m_low_degree)
         launch(kb_sssp_push<SSSP_UNROLL_LOW>, PUSH_GRID_LOW, m_stream, m_dev);
       else
         launch(kb_sssp_push<SSSP_UNROLL>, PUSH_GRID, m_stream, m_dev);
