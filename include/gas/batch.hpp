@@ -60,12 +60,10 @@ constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks togethe
 // let the dispatcher balance them (R-MAT-22 SSSP batch 39.2 -> 33.3 ms, PPR
 // 70.8 -> 64.3 ms against 4 x 1024).
 constexpr int PUSH_GRID = 32 * 1024;
-// Graphs of average out-degree <= LOW_DEGREE (road-like meshes): items of a
-// few edges with a few lanes each; the SSSP push unrolls 4 edges and runs on
-// 2,048 workgroups (road1024 64-source SSSP 14.9 -> 14.25 ms; on R-MAT-22
-// either change alone costs 5-13%).
+// Graphs of average out-degree <= LOW_DEGREE (road-like meshes): list
+// vertices of a few out-edges with a few changed lanes each; the SSSP push
+// takes (vertex, octet) items (kb_sssp_push_oct) on PUSH_GRID_LOW workgroups.
 constexpr int LOW_DEGREE = 8;
-constexpr int SSSP_UNROLL_LOW = 4;
 constexpr int PUSH_GRID_LOW = 2048;
 constexpr int32_t SSSP_INF = INT32_MAX - 100000;  // sssp.cu:32-33
 
@@ -781,12 +779,12 @@ static __global__ void kb_sssp_seed(BatchDev d, const int32_t *lanes,
 
 // Push from the changed list: one wave per item, lane = query.  The item's
 // edges are loaded lane-parallel once and broadcast by shuffles; each trip
-// issues U distance-row loads before any compare.  A lane is
+// issues SSSP_UNROLL distance-row loads before any compare.  A lane is
 // marked changed in the destination whenever its candidate beats the value
 // it read: a superset of the lanes actually lowered (the read may be older
 // than a concurrent atomicMin), which can only cause a redundant push.
-template <int U>
 static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
+  constexpr int U = SSSP_UNROLL;
   const BatchCtl *c = d.ctl;
   const int n_items = c->n_items;
   const int lane = lane_id();
@@ -861,9 +859,8 @@ static __device__ __forceinline__ void sssp_push(const BatchDev &d) {
   }
 }
 
-template <int U>
 static __global__ __launch_bounds__(NT) void kb_sssp_push(BatchDev d) {
-  sssp_push<U>(d);
+  sssp_push(d);
 }
 
 // Push from octet items (engines with octets: graphs of low degree, whose
@@ -1536,10 +1533,8 @@ class BatchEngine {
     } else {
       if (m_dev.octets)
         launch(kb_sssp_push_oct, PUSH_GRID_LOW, m_stream, m_dev);
-      else if (m_low_degree)
-        launch(kb_sssp_push<SSSP_UNROLL_LOW>, PUSH_GRID_LOW, m_stream, m_dev);
       else
-        launch(kb_sssp_push<SSSP_UNROLL>, PUSH_GRID, m_stream, m_dev);
+        launch(kb_sssp_push, PUSH_GRID, m_stream, m_dev);
       launch(kb_sssp_finalize, SCAN_BLOCKS, m_stream, m_dev);
     }
     launch_wg(kb_scan, 1, SCAN_NT, m_stream, m_dev);
