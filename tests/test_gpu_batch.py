@@ -125,6 +125,44 @@ def test_continuous_batching_alone_equals_mixed(gpu, kind):
     assert np.array_equal(b.result(8), want)
 
 
+@pytest.mark.parametrize("item_edges", [32, 64])
+def test_batch_sssp_octets(gpu, item_edges):
+    # a low-degree graph (road grid plus one vertex of degree 150): SSSP items
+    # per (vertex, octet of lanes with a change, chunk of out-edges); lanes
+    # spread over the octets, few per octet, and the hub's items (chunks x
+    # octets above the hub threshold) written by the hub pass
+    base = workloads.road(64, 64, seed=23)
+    s0, d0, w0 = base.arrays()
+    n = base.n_vertices + 1
+    hub = n - 1
+    rng = np.random.default_rng(11)
+    nb = rng.choice(hub, size=150, replace=False).astype(np.int32)
+    wt = rng.integers(1, 65, size=150).astype(np.int32)
+    s = np.concatenate([s0, np.full(150, hub, np.int32), nb])
+    d = np.concatenate([d0, nb, np.full(150, hub, np.int32)])
+    w = np.concatenate([w0, wt, wt])
+    el = gas.Edges.from_arrays(n, s, d, w)
+    g = gas.Graph(el)
+    b = gas.Batch(g, gas.BATCH_SSSP)
+    b.set_item_edges(item_edges)
+    b.set_hub_items(16)
+    lanes = [0, 3, 12, 27, 40, 41, 55, 63]
+    src = [hub, int(nb[0]), 5, 900, hub, 2047, int(nb[1]), 4000]
+    b.add(lanes, src)
+    b.run()
+    got = b.results(lanes)
+    for lane, sv in zip(lanes, src):
+        assert np.array_equal(got[lane], oracle.check_dijkstra(n, s, d, w, sv)), lane
+    # a second wave of queries into the same engine, other octets
+    b.release(sum(1 << x for x in lanes))
+    lanes2 = [8, 9, 30, 47]
+    src2 = [hub, 17, int(nb[2]), 3333]
+    b.add(lanes2, src2)
+    b.run()
+    for lane, sv in zip(lanes2, src2):
+        assert np.array_equal(b.result(lane), oracle.check_dijkstra(n, s, d, w, sv)), lane
+
+
 def test_batch_errors(gpu):
     el = workloads.road(32, 32, seed=1)
     g = gas.Graph(el)
