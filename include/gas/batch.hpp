@@ -101,8 +101,8 @@ struct BatchDev {
   uint64_t *front;                      // BFS frontier masks (the finalize
                                         // pass overwrites them in place:
                                         // push reads the items instead)
-  uint16_t *depth16;                    // BFS [BW][rowV]
-  int32_t rowV;                         // depth row stride (V rounded to 4)
+  uint16_t *depth16;                    // BFS depths in tiles (depth_at)
+  int32_t rowV;                         // V rounded up to DT
   int32_t *lane_start;                  // [BW]: round the lane began
   int32_t *dist;                        // SSSP [V][BW]
   uint64_t *chg, *chg_next;             // SSSP
@@ -237,6 +237,19 @@ __device__ inline int slab_rounds(int nV) {
 __device__ inline int slab_vertex(int r) {
   const int idx = r * NT + (int)threadIdx.x;
   return ((int)blockIdx.x + (idx / SLAB) * SCAN_BLOCKS) * SLAB + idx % SLAB;
+}
+
+// BFS depths (uint16, 0xffff unreached) in tiles of DT = SLAB vertices:
+// tile t holds lane s's depths of vertices DT t .. DT t + DT - 1 at
+// [(t BW + s) DT, + DT), one 128-byte line per (tile, lane).  A lane's
+// depths of a slab share a line (extraction reads 2 bytes per vertex and
+// lane; the finalize pass's store for one lane writes one line), and the
+// finalize pass may instead write a vertex's new lanes from the vertex's
+// own thread (as many stores as the most new lanes of a vertex in the
+// wave, not one per lane new anywhere in it).
+constexpr int DT = SLAB;
+__device__ inline size_t depth_at(int v, int s) {
+  return ((size_t)(v / DT) * BW + s) * DT + (v % DT);
 }
 
 // Octets of a 64-lane mask that hold some set lane, as 8 bits.
@@ -523,13 +536,14 @@ static __global__ __launch_bounds__(NT) void kb_bfs_reset(BatchDev d,
     d.vn[2 * (size_t)v] &= ~reset;
     d.front[v] &= ~reset;
   }
-  // depth rows of the reset lanes, two depths per thread
-  uint32_t *rows = reinterpret_cast<uint32_t *>(d.depth16);
-  const int wpr = d.rowV / 2;
+  // depths of the reset lanes: their sector of every tile, two depths per
+  // thread
+  uint32_t *w = reinterpret_cast<uint32_t *>(d.depth16);
+  const int per = DT / 2, words = d.rowV / DT * per;
   for (int s = 0; s < BW; ++s) {
     if (!((reset >> s) & 1ull)) continue;
-    for (int i = blockIdx.x * NT + threadIdx.x; i < wpr; i += stride)
-      rows[(size_t)s * wpr + i] = 0xffffffffu;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < words; i += stride)
+      w[((size_t)(i / per) * BW + s) * per + i % per] = 0xffffffffu;
   }
 }
 
@@ -544,7 +558,7 @@ static __global__ void kb_bfs_seed(BatchDev d, const int32_t *lanes,
   atomicOr((unsigned long long *)&d.vn[2 * (size_t)v], (unsigned long long)bit);
   atomicOr((unsigned long long *)&d.front[v], (unsigned long long)bit);
   d.sl_cur[v / SLAB] = 1;
-  d.depth16[(size_t)s * d.rowV + v] = 0;
+  d.depth16[depth_at(v, s)] = 0;
 }
 
 // Push: a wave takes BFS_ITEMS_PER_TRIP items at a time, 16 lanes per item
@@ -706,14 +720,25 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d) {
       }
       nf[v] = nv;
     }
-    // depths of the newly visited lanes: a wave holds 64 consecutive
-    // vertices of a slab and writes lane by lane, one contiguous run of
-    // the lane's depth row per store
+    // depths of the newly visited lanes: each thread over its own new lanes
+    // when that loop is at most half as long (sparse masks: road-like
+    // graphs), else the wave over the lanes new anywhere in it, a store
+    // writing one line, that lane's depths of the slab (dense masks: R-MAT)
     const uint64_t wnv = wave_or(nv);
-    for (uint64_t wx = wnv; wx; wx &= wx - 1) {
-      const int s = __ffsll((unsigned long long)wx) - 1;
-      if ((nv >> s) & 1ull)
-        d.depth16[(size_t)s * d.rowV + v] = (uint16_t)(round + 1 - s_start[s]);
+    int most = __popcll(nv);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) most = max(most, __shfl_xor(most, o, WAVE));
+    if (2 * most <= __popcll(wnv)) {
+      for (uint64_t x = nv; x; x &= x - 1) {
+        const int s = __ffsll((unsigned long long)x) - 1;
+        d.depth16[depth_at(v, s)] = (uint16_t)(round + 1 - s_start[s]);
+      }
+    } else {
+      for (uint64_t wx = wnv; wx; wx &= wx - 1) {
+        const int s = __ffsll((unsigned long long)wx) - 1;
+        if ((nv >> s) & 1ull)
+          d.depth16[depth_at(v, s)] = (uint16_t)(round + 1 - s_start[s]);
+      }
     }
     if (lane_id() == 0 && v < d.nV) {
       slf[sl] = wnv != 0;
@@ -746,9 +771,8 @@ static __global__ __launch_bounds__(NT) void kb_bfs_finalize(BatchDev d) {
 // Depth of lane s as int32 (-1 unreached), for the caller.
 static __global__ __launch_bounds__(NT) void kb_bfs_extract(BatchDev d, int s,
                                                      int32_t *out) {
-  const uint16_t *row = d.depth16 + (size_t)s * d.rowV;
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
-    const uint16_t x = row[d.new_of ? d.new_of[v] : v];
+    const uint16_t x = d.depth16[depth_at(d.new_of ? d.new_of[v] : v, s)];
     out[v] = x == 0xffff ? -1 : (int32_t)x;
   }
 }
@@ -1022,14 +1046,15 @@ struct LaneOutputs {
   int32_t *p[BW];
 };
 
-// BFS: depth rows are lane-major, each output is a row conversion.
+// BFS: a lane's depths of DT consecutive vertices share a sector; each
+// output is a conversion of those.
 static __global__ __launch_bounds__(NT) void kb_bfs_extract_many(
     BatchDev d, uint64_t lanes, LaneOutputs outs) {
   for (int v = blockIdx.x * NT + threadIdx.x; v < d.nV; v += gridDim.x * NT) {
     const int u = d.new_of ? d.new_of[v] : v;
     for (uint64_t m = lanes; m; m &= m - 1) {
       const int s = __ffsll((unsigned long long)m) - 1;
-      const uint16_t x = d.depth16[(size_t)s * d.rowV + u];
+      const uint16_t x = d.depth16[depth_at(u, s)];
       outs.p[s][v] = x == 0xffff ? -1 : (int32_t)x;
     }
   }
@@ -1148,7 +1173,7 @@ class BatchEngine {
     m_lanes.alloc(BW);
     m_sources.alloc(BW);
     if (kind == BFS_KIND) {
-      d.rowV = (int32_t)((V + 3) / 4 * 4);
+      d.rowV = (int32_t)((V + DT - 1) / DT * DT);
       m_vn.alloc(2 * V);
       m_front.alloc(V);
       m_depth.alloc((size_t)d.rowV * BW);
