@@ -1629,7 +1629,15 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
     uint8_t *fn = d.fl[p ^ 1];
     const int lane = lane_id();
     const bool all = d.in_mode == IN_ALL;
-    int active = 0, pushed = 0, frontier = 0;
+    // per-thread counters in LDS, not in locals the lambdas capture (those
+    // the compiler keeps in scratch here)
+    __shared__ int s_cnt3[3][NT];
+    int *const c_active = &s_cnt3[0][threadIdx.x];
+    int *const c_pushed = &s_cnt3[1][threadIdx.x];
+    int *const c_front = &s_cnt3[2][threadIdx.x];
+    *c_active = 0;
+    *c_pushed = 0;
+    *c_front = 0;
     // a push can only matter below the destination's value in vdata (vnext
     // ends at most there: its own frontier entry lowers it to that), and a
     // push below it makes the destination change whatever else lands, so
@@ -1669,8 +1677,8 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
             }
             push = e[0] >= 0;
           }
-          ++frontier;
-          if (push) ++active;
+          ++*c_front;
+          if (push) ++*c_active;
         }
         if (push) {
           // every destination's value loaded before the first store (a byte
@@ -1687,7 +1695,7 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
                 atomicMin(d.vnext + e[k], x);
                 fn[e[k]] = 1;
               }
-              ++pushed;
+              ++*c_pushed;
             }
         }
       });
@@ -1699,13 +1707,13 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
         a = d.vdata[v];
         so = d.csr_off[v];
         deg = d.csr_off[v + 1] - so;
-        ++frontier;
+        ++*c_front;
         if (!all) atomicMin(d.vnext + v, a);  // all: the buffers are equal
       }
       bool push = valid && deg > 0;
       if constexpr (has_push_skip<P>::value)
         push = push && !P::pushSkip(a);
-      if (push) ++active;
+      if (push) ++*c_active;
       if (push && deg <= d.st1) {
         int dst[T1_MAX];
         ED w[T1_MAX];
@@ -1728,7 +1736,7 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
               fn[dst[k]] = 1;
             }
           }
-        pushed += deg;
+        *c_pushed += deg;
       }
       uint64_t m = __ballot(push && deg > d.st1);
       while (m) {
@@ -1742,9 +1750,10 @@ __global__ __launch_bounds__(NT) void k_push(Dev<P> d, int p) {
           if constexpr (!std::is_empty<ED>::value) w = d.edata_csr[oj + lane];
           push_one(P::gatherMap(nullptr, &aj, &w), d.csr_dst[oj + lane]);
         }
-        if (lane == j) pushed += dj;
+        if (lane == j) *c_pushed += dj;
       }
     });
+    int active = *c_active, pushed = *c_pushed, frontier = *c_front;
     {
       __shared__ int lds3[3 * (NT / WAVE)];
       block_sum3(active, pushed, frontier, lds3);
