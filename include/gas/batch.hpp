@@ -53,6 +53,10 @@ constexpr int HUB_ITEMS = 32;     // a vertex with more items is written by
                                   // a whole workgroup of kb_hub_items (the
                                   // default; gas_batch_set_hub_items)
 constexpr int HUB_ITEMS_MIN = 16; // smallest hub threshold (sizes the hub list)
+// kb_list rounds whose vertices have at most LIST_FEW push items each write
+// them per vertex thread (no hubs among them: LIST_FEW < HUB_ITEMS_MIN)
+constexpr int LIST_FEW = 8;
+static_assert(LIST_FEW < HUB_ITEMS_MIN, "a few-item round holds no hub");
 constexpr int SSSP_UNROLL = 8;    // edges per item per wave trip in the SSSP push
 constexpr int SSSP_ITEMS_PER_TRIP = 1;  // SSSP push: items a wave walks together
 // Workgroups of the SSSP push and the PPR item pass: items differ widely in
@@ -458,9 +462,19 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
     const int pos = carry + (pk & 0xffff);
     const int iex = (int)(pi >> 32), jex = (int)(pi & 0xffffffff);
     if (on) d.list[pos] = v;
-    // the workgroup writes the round's non-hub items together: the j-th of
-    // them belongs to the last thread whose exclusive count of non-hub
-    // items (s_jpos) is <= j; hubs' items are left to kb_hub_items
+    // a round whose vertices have at most LIST_FEW items each (no hub among
+    // them): each thread writes its own vertex's items
+    if (!__syncthreads_or(nit > LIST_FEW)) {
+      for (int k = 0; k < nit; ++k)
+        d.item[icarry + iex + k] = push_item(d, v, mv, o, o1, k);
+      carry += tot;
+      icarry += itot;
+      hcarry += htot;
+      continue;
+    }
+    // otherwise the workgroup writes the round's non-hub items together:
+    // the j-th of them belongs to the last thread whose exclusive count of
+    // non-hub items (s_jpos) is <= j; hubs' items are left to kb_hub_items
     s_ipos[threadIdx.x] = iex;
     s_jpos[threadIdx.x] = jex;
     s_v[threadIdx.x] = v;
