@@ -581,15 +581,38 @@ class GASEngineGPU {
     gasrt_event_t a, b;
     check(gasrt_event_create(&a), "event");
     check(gasrt_event_create(&b), "event");
+    // the scatter is timed as supersteps run it, fused into the apply pass
+    // over every vertex (from the state saved here, restored before each
+    // repetition and at the end; the gather results of one gather pass),
+    // then the hub scatter; engines on caller memory time the unfused pass
+    DevBuf<VertexData> saved;
+    const bool fused = tuneScatter && !m_ext;
+    const size_t state_bytes = sizeof(VertexData) * (size_t)m_nV;
+    auto restore = [&]() {
+      check(gasrt_memcpy_d2d(m_dev.vdata, saved.p, state_bytes, m_stream), "restore");
+    };
     auto time_phase = [&](bool gather) {
       float best = 1e30f;
       for (int r = 0; r < reps; ++r) {
         setActive(0, m_nV);
-        if (!gather)
+        if (!gather && fused) restore();
+        if (!gather && !fused)
           check(gasrt_memset(m_rflag.p, 1, (size_t)m_nV, m_stream), "memset");
         check(gasrt_event_record(a, m_stream), "event");
         if (gather) {
           gatherPhase();
+        } else if (fused) {
+          m_out_list = 0;
+          m_flags_dirty = true;
+          if (tuneGather)
+            klaunch(GAS_K_APPLY, 1, k_apply<Program, true, true>, SCAN_BLOCKS);
+          else
+            klaunch(GAS_K_APPLY, 1, k_apply<Program, false, true>, SCAN_BLOCKS);
+          if (outHubs()) {
+            klaunch(GAS_K_HSCAN1, 1, k_hscan1<Program, 1>, SCAN_BLOCKS);
+            klaunch(GAS_K_HSCAN2, 1, k_hscan2<Program, 1>, SCAN_BLOCKS);
+            klaunch(GAS_K_HSCATTER, 1, k_hscatter<Program>, MP_BLOCKS);
+          }
         } else {
           m_out_list = 0;
           m_flags_dirty = true;
@@ -632,7 +655,20 @@ class GASEngineGPU {
         setThresholds(m_gt1, m_gt2, bt1, bt2);
     };
     if (tuneGather) sweep(true);
+    if (fused) {
+      saved.alloc((size_t)m_nV);
+      check(gasrt_memcpy_d2d(saved.p, m_dev.vdata, state_bytes, m_stream), "save");
+      if (tuneGather) {
+        setActive(0, m_nV);
+        gatherPhase();  // the apply passes' gather results
+      }
+    }
     if (tuneScatter) sweep(false);
+    if (fused) {
+      restore();
+      m_srcval_stale = true;
+      check(gasrt_stream_synchronize(m_stream), "sync");
+    }
     gasrt_event_destroy(a);
     gasrt_event_destroy(b);
     setActive(0, 0);
