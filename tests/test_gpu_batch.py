@@ -163,6 +163,55 @@ def test_batch_sssp_octets(gpu, item_edges):
         assert np.array_equal(b.result(lane), oracle.check_dijkstra(n, s, d, w, sv)), lane
 
 
+def _random_graph(rng, n, avg_deg, hub_deg):
+    # symmetric weighted graph: random pairs plus one vertex of high degree
+    # (n need not be a multiple of 64: the last slab and depth tile are
+    # partial)
+    m = n * avg_deg // 2
+    a = rng.integers(0, n, size=m)
+    b = rng.integers(0, n, size=m)
+    hub = int(rng.integers(0, n))
+    nb = rng.choice(n, size=min(hub_deg, n - 1), replace=False)
+    a = np.concatenate([a, np.full(len(nb), hub)])
+    b = np.concatenate([b, nb])
+    keep = a != b
+    a, b = a[keep], b[keep]
+    w = rng.integers(1, 100, size=len(a))
+    s = np.concatenate([a, b]).astype(np.int32)
+    d = np.concatenate([b, a]).astype(np.int32)
+    return s, d, np.concatenate([w, w]).astype(np.int32)
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_batch_randomized(gpu, case):
+    # seeded random graphs of low and high average degree (octet SSSP items
+    # or vertex-wide ones), random lane subsets, item sizes, hub thresholds
+    # and distance steps; every lane against the oracle
+    rng = np.random.default_rng(100 + case)
+    n = int(rng.choice([999, 1000, 4099, 6000]))
+    avg = int(rng.choice([3, 6, 24]))
+    s, d, w = _random_graph(rng, n, avg, int(rng.choice([40, 300])))
+    el = gas.Edges.from_arrays(n, s, d, w)
+    g = gas.Graph(el)
+    for kind in (gas.BATCH_BFS, gas.BATCH_SSSP):
+        b = gas.Batch(g, kind)
+        edges = int(rng.choice([8, 16] if kind == gas.BATCH_BFS else [32, 64]))
+        b.set_item_edges(edges)
+        b.set_hub_items(int(rng.choice([16, 32, 1 << 20])))
+        if kind == gas.BATCH_SSSP:
+            b.set_sssp_delta(int(rng.choice([0, 16, 64])))
+        k = int(rng.integers(1, 65))
+        lanes = np.sort(rng.choice(64, size=k, replace=False)).astype(np.int32)
+        src = rng.integers(0, n, size=k).astype(np.int32)
+        b.add(lanes, src)
+        b.run()
+        got = b.results(lanes)
+        for lane, sv in zip(lanes, src):
+            want = (oracle.check_bfs(n, s, d, int(sv)) if kind == gas.BATCH_BFS
+                    else oracle.check_dijkstra(n, s, d, w, int(sv)))
+            assert np.array_equal(got[int(lane)], want), (case, kind, int(lane))
+
+
 def test_batch_errors(gpu):
     el = workloads.road(32, 32, seed=1)
     g = gas.Graph(el)

@@ -613,7 +613,9 @@ class Batch:
 
     def results(self, lanes) -> dict:
         """{lane: result} for several lanes, extracted in one device pass."""
-        lanes = list(lanes)
+        lanes = [int(s) for s in lanes]  # numpy ints would overflow 1 << s
+        if any(s < 0 or s >= 64 for s in lanes):
+            raise GasError(1, "lanes are 0..63")
         bufs = {s: DeviceBuffer(4 * self.graph.n_vertices) for s in lanes}
         ptrs = (C.c_void_p * 64)()
         mask = 0
