@@ -1162,19 +1162,26 @@ class BatchEngine {
     d.csr_w = gv.csr_weights;
     d.pull_div = 0;  // push only: see setPullDivisor
     d.item_edges = kind == BFS_KIND ? BFS_ITEM_EDGES : SSSP_ITEM_EDGES;
-    {
-      // kb_hub_items runs only where some vertex can have > hub_items items
-      std::vector<int32_t> off((size_t)gv.n_vertices + 1);
-      check(gas_graph_host_csr(g, off.data(), nullptr, nullptr), "csr");
-      for (int32_t v = 0; v < gv.n_vertices; ++v)
-        m_max_out = std::max<int64_t>(m_max_out, off[v + 1] - off[v]);
-    }
     m_low_degree = (int64_t)gv.n_edges <= (int64_t)LOW_DEGREE * gv.n_vertices;
     const int ie_min = kind == BFS_KIND ? BFS_ITEM_EDGES_MIN : SSSP_ITEM_EDGES_MIN;
     // at most one item per ie_min out-edges and vertex, 8 times that with
     // octets (whose item counts must stay within int32)
     const int64_t max_items = (int64_t)gv.n_edges / ie_min + (int64_t)V + 1;
     d.octets = kind == SSSP_KIND && m_low_degree && 8 * max_items < INT32_MAX;
+    // the hub list holds every vertex that can have more than HUB_ITEMS_MIN
+    // items (8 times its chunks with octets)
+    int64_t max_hubs = 1;
+    {
+      // kb_hub_items runs only where some vertex can have > hub_items items
+      std::vector<int32_t> off((size_t)gv.n_vertices + 1);
+      check(gas_graph_host_csr(g, off.data(), nullptr, nullptr), "csr");
+      const int64_t mult = d.octets ? 8 : 1;
+      for (int32_t v = 0; v < gv.n_vertices; ++v) {
+        const int64_t deg = off[v + 1] - off[v];
+        m_max_out = std::max<int64_t>(m_max_out, deg);
+        max_hubs += (deg + ie_min - 1) / ie_min * mult > HUB_ITEMS_MIN;
+      }
+    }
     setHubItems(HUB_ITEMS);
     m_ctl.alloc(1);
     m_list.alloc(V + 1);
@@ -1227,8 +1234,8 @@ class BatchEngine {
     d.item = m_item.p;
     // a hub has > HUB_ITEMS_MIN items, so at most
     // E / (HUB_ITEMS_MIN * item_edges) of them
-    m_hub.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * ie_min) + 1);
-    m_hub_mask.alloc((size_t)gv.n_edges / ((size_t)HUB_ITEMS_MIN * ie_min) + 1);
+    m_hub.alloc((size_t)max_hubs);
+    m_hub_mask.alloc((size_t)max_hubs);
     d.hub = m_hub.p;
     d.hub_mask = m_hub_mask.p;
     d.bsum = m_bsum.p;

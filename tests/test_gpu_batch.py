@@ -163,6 +163,35 @@ def test_batch_sssp_octets(gpu, item_edges):
         assert np.array_equal(b.result(lane), oracle.check_dijkstra(n, s, d, w, sv)), lane
 
 
+def test_batch_sssp_octet_many_hubs(gpu):
+    # a low-degree graph (average out-degree <= 8, so octet items) with
+    # 1,500 vertices of degree ~80: with every octet changed such a vertex
+    # has 3 chunks x 8 octets = 24 items, above a hub threshold of 16, so a
+    # round can list far more hubs than one per 16 x 32 edges
+    rng = np.random.default_rng(77)
+    n, nh, k = 40000, 1500, 80
+    hubs = rng.choice(n, size=nh, replace=False)
+    a = np.repeat(hubs, k)
+    b = rng.integers(0, n, size=nh * k)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    w = rng.integers(1, 50, size=len(a))
+    s = np.concatenate([a, b]).astype(np.int32)
+    d = np.concatenate([b, a]).astype(np.int32)
+    w = np.concatenate([w, w]).astype(np.int32)
+    assert len(s) <= 8 * n
+    g = gas.Graph(gas.Edges.from_arrays(n, s, d, w))
+    b_ = gas.Batch(g, gas.BATCH_SSSP)
+    b_.set_item_edges(32)
+    b_.set_hub_items(16)
+    src = rng.choice(hubs, size=64, replace=False).astype(np.int32)
+    b_.add(np.arange(64), src)
+    b_.run()
+    got = b_.results(range(0, 64, 9))
+    for lane, res in got.items():
+        assert np.array_equal(res, oracle.check_dijkstra(n, s, d, w, int(src[lane]))), lane
+
+
 def _random_graph(rng, n, avg_deg, hub_deg):
     # symmetric weighted graph: random pairs plus one vertex of high degree
     # (n need not be a multiple of 64: the last slab and depth tile are
