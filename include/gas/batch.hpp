@@ -435,7 +435,8 @@ static __global__ __launch_bounds__(NT) void kb_list(BatchDev d, int advance) {
   const int64_t ed = sc.list_edges;
   const bool pull = d.pull_div > 0 && ed * (int64_t)d.pull_div >= (int64_t)d.nE;
   int carry = pre, icarry = ipre, hcarry = hpre;
-  const int nr = pull ? 0 : slab_rounds(d.nV);
+  // a workgroup whose slabs the count pass found empty has nothing to list
+  const int nr = (pull || d.bsum[blockIdx.x] == 0) ? 0 : slab_rounds(d.nV);
   for (int r = 0; r < nr; ++r) {
     const int v = slab_vertex(r);
     const bool act = v < d.nV && slm[v / SLAB];
